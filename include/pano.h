@@ -1,0 +1,206 @@
+/*
+ * pano.h -- C ABI of the MI355X (gfx950) Eulerian fluid step.
+ *
+ * Drop-in boundary for the fluid step of msiglreith/panopaea: the DEC
+ * operators of Manifold2d (panopaea/src/dec/manifold.rs:19-84, implemented
+ * for Grid2d in panopaea/src/dec/grid.rs:202-408), the preconditioned CG
+ * solver (panopaea/src/pcg.rs:4-82), and the semi-Lagrangian advection and
+ * step of examples/dec_fluid.rs:46-291. Plain pointers and sizes only.
+ *
+ * Conventions (mirroring the reference):
+ *  - Fields are caller-owned device buffers (the reference's new_simplex_*
+ *    allocate zeroed arrays, grid.rs:358-371; here pano_new_simplex_* or any
+ *    device allocation). Outputs come first, inputs after, as in the trait
+ *    methods (`&mut dst, &src`). Output and input must not alias.
+ *  - Layouts (row-major, x fastest): cells ny x nx (2D) / nz x ny x nx (3D);
+ *    2D faces [vy: (ny+1) x nx | vx: ny x (nx+1)] as Simplex1
+ *    (grid.rs:36-61); 3D faces [vz: (nz+1)ny nx | vy: nz(ny+1)nx |
+ *    vx: nz ny(nx+1)]; 2D vertices (ny+1) x (nx+1).
+ *  - Every call is enqueued on the context's HIP stream and returns without
+ *    waiting unless documented otherwise. Calls on one context must not run
+ *    concurrently from several threads; separate contexts are independent.
+ *  - Errors: the reference panics (ndarray shape mismatch, unimplemented!())
+ *    or returns nothing; here every call returns a pano_status and
+ *    pano_last_error() describes the last failure on the calling thread.
+ */
+#ifndef PANO_H
+#define PANO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pano_status;
+#define PANO_OK 0
+#define PANO_NOT_CONVERGED 1          /* result written; stop test not met in max_iterations */
+#define PANO_ERR_INVALID (-1)         /* null pointer, aliasing, unknown enum value */
+#define PANO_ERR_BAD_SHAPE (-2)       /* dims unsupported for the requested operation */
+#define PANO_ERR_NONFINITE (-3)       /* NaN/Inf in the right-hand side or a CG scalar */
+#define PANO_ERR_OOM (-4)             /* device allocation failed or budget exceeded */
+#define PANO_ERR_DEVICE (-5)          /* HIP runtime error */
+#define PANO_ERR_UNIMPLEMENTED (-6)   /* as the reference's unimplemented!() */
+#define PANO_ERR_NO_DEVICE (-7)       /* no GPU visible */
+
+#define PANO_F32 0
+#define PANO_F64 1
+
+typedef struct pano_ctx_s* pano_ctx;
+
+/* Grid2d{dim:(y,x)} (panopaea/src/domain/grid.rs:2-15) and its 3D
+ * counterpart (Grid3d{dim:(z,y,x)}, domain/grid.rs:17-20, a stub there).
+ * bc: x-, x+, y-, y+, z-, z+ ; 0 = closed wall (zero flux, the reference's
+ * only boundary), 1 = open (Dirichlet p = 0 in the ghost cell). */
+typedef struct {
+    int32_t ndim;   /* 2 or 3 */
+    int32_t dtype;  /* PANO_F32 or PANO_F64 */
+    int64_t nz, ny, nx;
+    int32_t bc[6];
+} pano_grid;
+
+/* ---- context and memory ------------------------------------------------ */
+pano_status pano_ctx_create(int32_t device, pano_ctx* out);
+pano_status pano_ctx_destroy(pano_ctx ctx);
+pano_status pano_ctx_synchronize(pano_ctx ctx);
+void* pano_ctx_stream(pano_ctx ctx); /* the context's hipStream_t */
+pano_status pano_malloc(pano_ctx ctx, size_t bytes, void** out); /* zero-filled */
+pano_status pano_free(pano_ctx ctx, void* ptr);
+pano_status pano_memcpy_htod(pano_ctx ctx, void* dst, const void* src, size_t bytes); /* synchronous */
+pano_status pano_memcpy_dtoh(pano_ctx ctx, void* dst, const void* src, size_t bytes); /* synchronous */
+pano_status pano_fill_zero(pano_ctx ctx, void* dst, size_t bytes); /* fill kernel, async */
+/* Kernel timer: HIP events around every launch of the named kernel on the
+ * context's stream until pano_timer_stop, which returns the summed duration,
+ * the launch count (launches the solver's converged flag made no-ops are
+ * excluded) and the kernel's algorithmic bytes per launch. Names:
+ * "smooth_l0", "apply_dot", "update", "residual_l0", "restrict_l0",
+ * "prolong_l0", "advect_mac", "advect", "project", "neg_div". */
+pano_status pano_timer_start(pano_ctx ctx, const char* kernel);
+pano_status pano_timer_stop(pano_ctx ctx, double* total_ms, int64_t* launches, double* bytes_per_launch);
+const char* pano_status_string(pano_status s);
+const char* pano_last_error(void);
+int32_t pano_version(void);
+
+/* ---- Manifold2d (dec/manifold.rs:19-84; grid.rs:343-371) ------------------ */
+int64_t pano_num_elem_0(const pano_grid* g); /* grid.rs:346-348 */
+int64_t pano_num_elem_1(const pano_grid* g); /* grid.rs:350-352 (3D: all face components) */
+int64_t pano_num_elem_2(const pano_grid* g); /* grid.rs:354-356 (3D: cells) */
+pano_status pano_new_simplex_0(pano_ctx ctx, const pano_grid* g, void** out); /* grid.rs:358-360 */
+pano_status pano_new_simplex_1(pano_ctx ctx, const pano_grid* g, void** out); /* grid.rs:362-367 */
+pano_status pano_new_simplex_2(pano_ctx ctx, const pano_grid* g, void** out); /* grid.rs:369-371 */
+
+/* Hodge stars. hodge_k_primal(dual <- primal) = Hodge::apply,
+ * hodge_k_dual(primal <- dual) = Hodge::apply_inv (manifold.rs:64-83). */
+pano_status pano_hodge_0_primal(pano_ctx, const pano_grid*, void* dual, const void* primal);  /* grid.rs:106-146 */
+pano_status pano_hodge_2_dual(pano_ctx, const pano_grid*, void* primal, const void* dual);    /* grid.rs:147-191 */
+pano_status pano_hodge_1_primal(pano_ctx, const pano_grid*, void* dual, const void* primal);  /* grid.rs:206-221 */
+pano_status pano_hodge_1_dual(pano_ctx, const pano_grid*, void* primal, const void* dual);    /* grid.rs:223-238 */
+pano_status pano_hodge_2_primal(pano_ctx, const pano_grid*, void* dual, const void* primal);  /* grid.rs:253-255 */
+pano_status pano_hodge_0_dual(pano_ctx, const pano_grid*, void* primal, const void* dual);    /* grid.rs:257-259 */
+
+/* Exterior derivatives (manifold.rs:46-62). derivative_0_dual writes interior
+ * faces only and leaves boundary faces untouched (grid.rs:318-334). */
+pano_status pano_derivative_0_primal(pano_ctx, const pano_grid*, void* edges, const void* vertices); /* grid.rs:274-288 */
+pano_status pano_derivative_1_primal(pano_ctx, const pano_grid*, void* cells, const void* edges);    /* grid.rs:295-305 */
+pano_status pano_derivative_0_dual(pano_ctx, const pano_grid*, void* edges, const void* cells);      /* grid.rs:318-334 */
+pano_status pano_derivative_1_dual(pano_ctx, const pano_grid*, void* vertices, const void* edges);   /* grid.rs:308-312: unimplemented */
+
+/* ---- fluid-step operators (2D and 3D) ----------------------------------- */
+/* b = -div u: hodge_1_dual -> derivative_1_primal -> negate, dec_fluid.rs:68-83. */
+pano_status pano_neg_divergence(pano_ctx, const pano_grid*, void* b, const void* faces);
+/* q = A p, the operator closure dec_fluid.rs:100-119 (hodge_2_primal ->
+ * derivative_0_dual -> [obstacle mask] -> hodge_1_dual -> derivative_1_primal
+ * -> *dt). face_mask: optional device byte per face (0 = closed), 2D only. */
+pano_status pano_poisson_apply(pano_ctx, const pano_grid*, double dt, const uint8_t* face_mask,
+                               void* q, const void* p);
+/* u += dt * derivative_0_dual(p); closed walls zeroed (dec_fluid.rs:124-141). */
+pano_status pano_project(pano_ctx, const pano_grid*, double dt, void* faces, const void* p);
+/* Cell-centred semi-Lagrangian advection, advect (dec_fluid.rs:173-211). */
+pano_status pano_advect(pano_ctx, const pano_grid*, void* dst, const void* src, double dt,
+                        const void* vel);
+/* Face-centred semi-Lagrangian advection, advect_mac (dec_fluid.rs:213-291). */
+pano_status pano_advect_mac(pano_ctx, const pano_grid*, void* dst, const void* src, double dt,
+                            const void* vel);
+
+/* ---- preconditioned conjugate gradient (pcg.rs:4-82) --------------------- */
+#define PANO_PRECOND_IDENTITY 0  /* the reference's `()` preconditioner, pcg.rs:8-12 */
+#define PANO_PRECOND_JACOBI 1
+#define PANO_PRECOND_MULTIGRID 2 /* geometric MG V-cycle, red-black Gauss-Seidel */
+
+#define PANO_TOL_ABS_LINF 0      /* ||r||_inf < threshold, the reference's test pcg.rs:58-60 */
+#define PANO_TOL_REL_LINF 1      /* ||r||_inf < threshold * ||b||_inf */
+
+typedef struct {
+    int32_t precond;
+    int32_t mg_levels;       /* >= 2; each dim must divide by 2^(levels-1) with an even
+                                x extent on every smoothed level */
+    int32_t mg_nu1, mg_nu2;  /* pre / post red-black sweeps */
+    int32_t tol_kind;
+    int32_t max_iterations;  /* pcg.rs:18 */
+    double threshold;        /* pcg.rs:19 */
+    double dt;               /* operator scale (the closure's timestep) */
+    int32_t remove_mean;     /* -1 auto (closed box), 0 off, 1 on */
+    int32_t check_every;     /* iterations between host convergence polls; 0 = auto */
+    const uint8_t* face_mask;/* optional obstacle mask (2D, identity/Jacobi only) */
+} pano_pcg_params;
+
+typedef struct {
+    int32_t iterations;      /* operator applications (the reference prints i = iterations-1) */
+    int32_t converged;
+    double b_linf;
+    double r_linf;           /* recursive residual L-inf at exit */
+    double threshold;        /* absolute threshold applied */
+} pano_pcg_info;
+
+/* Preconditioner::apply(dst, src) (pcg.rs:4-6): dst = M src for the
+ * preconditioner in params (identity copy, Jacobi, or one multigrid V-cycle
+ * from zero). Natural layout; asynchronous. */
+pano_status pano_precond_apply(pano_ctx, const pano_grid*, const pano_pcg_params*, void* dst,
+                               const void* src);
+
+/* precond_conjugate_gradient (pcg.rs:14-28): x, b, residual, auxiliary,
+ * search are caller-owned device cell fields (the reference's scratch
+ * buffers); the closure `a` becomes the named operator pano_poisson_apply
+ * with params->dt and params->face_mask. Synchronous: returns after the
+ * solve (PANO_OK or PANO_NOT_CONVERGED). b is not modified. */
+pano_status pano_precond_conjugate_gradient(pano_ctx, const pano_grid*, const pano_pcg_params*,
+                                            void* x, const void* b, void* residual,
+                                            void* auxiliary, void* search, pano_pcg_info* info);
+
+/* ---- one fluid step (dec_fluid.rs:46-141) ---------------------------------- */
+typedef struct {
+    double dt;
+    pano_pcg_params pcg;     /* pcg.dt is ignored (the step's dt is used) */
+    double buoyancy;         /* smoke buoyancy on the z (3D) faces; 0 = off */
+    double src_center[3];    /* (z, y, x) centre of the density source sphere */
+    double src_radius;       /* <= 0: no source */
+} pano_step_params;
+
+/* source -> advect density and MAC velocity -> b = -div u -> PCG ->
+ * projection. density_in/velocity_in are read, density_out/velocity_out
+ * written (they must be distinct buffers); pressure (optional, may be NULL)
+ * receives the solved pressure. Synchronous. */
+pano_status pano_fluid_step(pano_ctx, const pano_grid*, const pano_step_params*,
+                            const void* density_in, const void* velocity_in, void* density_out,
+                            void* velocity_out, void* pressure, pano_pcg_info* info);
+
+/* ---- seeded synthetic inputs (host, deterministic) ------------------------- */
+/* Band-limited velocity: each face component is the sum of n_modes
+ * sinusoids amplitude*sin(2*pi*(k.X)/N + phi), integer k with
+ * 1 <= |k| <= k_max, phi ~ U[0, 2pi), drawn with SplitMix64 from seed;
+ * X at face centres; closed-wall faces zeroed. Writes host memory. */
+pano_status pano_gen_bandlimited_faces(const pano_grid* g, uint64_t seed, int32_t n_modes,
+                                       double k_max, double amplitude, void* host_faces);
+/* The same field computed on the device (bit-identical to the host one). */
+pano_status pano_gen_bandlimited_faces_device(pano_ctx, const pano_grid* g, uint64_t seed,
+                                              int32_t n_modes, double k_max, double amplitude,
+                                              void* faces);
+/* Gaussian blob exp(-r^2 / (2 sigma^2)) at cell centres around center (z,y,x). */
+pano_status pano_gen_gaussian_cells(const pano_grid* g, const double center[3], double sigma,
+                                    void* host_cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANO_H */

@@ -1,0 +1,300 @@
+// Fluid-step operators on the GPU, 2D and 3D: semi-Lagrangian advection of
+// cells and MAC faces (examples/dec_fluid.rs:173-291), b = -div u
+// (dec_fluid.rs:68-83), the pressure operator closure (dec_fluid.rs:100-119)
+// and the projection (dec_fluid.rs:124-141). All are HBM-bound gathers or
+// stencils; threads are tiled 64 (x, coalesced) x 4 (y) per z plane and the
+// workgroup order is remapped so each XCD sweeps a contiguous slab.
+#include "fluid_device.hpp"
+
+namespace pano {
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_advect_cell(Dims D, Grid3 G, T* dst, const T* q, T dt,
+                                                     Faces<const T> v, Sphere S) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    dst[cell_index(D, z, y, x)] = advect_cell_value<T, DIM>(D, q, S, dt, v, z, y, x);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_advect_fx(Dims D, Grid3 G, T* dst, const T* qx, T dt,
+                                                   Faces<const T> v) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x > D.w || y >= D.h) return;
+    dst[fx_idx(D, z, y, x)] = advect_face_x<T, DIM>(D, qx, dt, v, z, y, x);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_advect_fy(Dims D, Grid3 G, T* dst, const T* qy, T dt,
+                                                   Faces<const T> v) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y > D.h) return;
+    dst[fy_idx(D, z, y, x)] = advect_face_y<T, DIM>(D, qy, dt, v, z, y, x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_advect_fz(Dims D, Grid3 G, T* dst, const T* qz, T dt,
+                                                   Faces<const T> v, T buoy_c, const T* rho,
+                                                   Sphere S) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    dst[fz_idx(D, z, y, x)] = advect_face_z<T>(D, qz, dt, v, z, y, x, buoy_c, rho, S);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_neg_div(Dims D, Grid3 G, T* b, Faces<const T> v) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    b[cell_index(D, z, y, x)] = -div_value<T, DIM>(D, v, z, y, x);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_poisson(Dims D, Grid3 G, T* q, const T* p, T dt) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    q[cell_index(D, z, y, x)] = poisson_value<T, DIM, NATURAL>(D, p, dt, z, y, x);
+}
+
+// The literal closure with the reference's obstacle mask (dec_fluid.rs:104-112):
+// masked dual faces are zeroed before hodge_1_dual. 2D only.
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson_masked2(Dims D, Grid3 G, T* q, const T* p, T dt,
+                                                         const uint8_t* mask) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    const int h = D.h, w = D.w;
+    const T zero = (T)0;
+    const uint8_t* my = mask;                       // vy faces (h+1) x w
+    const uint8_t* mx = mask + (int64_t)(h + 1) * w; // vx faces h x (w+1)
+    T pc = p[(int64_t)y * w + x];
+    // dual edges (derivative_0_dual), zero at walls and masked faces
+    T ey_t = (y >= 1 && my[(int64_t)y * w + x]) ? -(pc - p[(int64_t)(y - 1) * w + x]) : zero;
+    T ey_b = (y + 1 <= h - 1 && my[(int64_t)(y + 1) * w + x]) ? -(p[(int64_t)(y + 1) * w + x] - pc) : zero;
+    T ex_l = (x >= 1 && mx[(int64_t)y * (w + 1) + x]) ? p[(int64_t)y * w + x - 1] - pc : zero;
+    T ex_r = (x + 1 <= w - 1 && mx[(int64_t)y * (w + 1) + x + 1]) ? pc - p[(int64_t)y * w + x + 1] : zero;
+    // hodge_1_dual: vy part negated, vx part kept; derivative_1_primal
+    T top = -ey_t, bottom = -ey_b;
+    T r = -bottom + top - ex_l + ex_r;
+    q[(int64_t)y * w + x] = r * dt;
+}
+
+template <typename T, int DIM, int L_>
+__global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x > D.w || y >= D.h) return;
+    int64_t i = fx_idx(D, z, y, x);
+    ux[i] = project_x<T, L_>(D, ux[i], p, dt, z, y, x);
+}
+template <typename T, int DIM, int L_>
+__global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const T* p, T dt) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y > D.h) return;
+    int64_t i = fy_idx(D, z, y, x);
+    uy[i] = project_y<T, L_>(D, uy[i], p, dt, z, y, x);
+}
+template <typename T, int L_>
+__global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const T* p, T dt) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    int64_t i = fz_idx(D, z, y, x);
+    uz[i] = project_z<T, L_>(D, uz[i], p, dt, z, y, x);
+}
+
+static Sphere make_sphere(const double* s) {
+    Sphere S{0, 0, 0, 0, 0};
+    if (s && s[3] > 0) {
+        S.cz = s[0];
+        S.cy = s[1];
+        S.cx = s[2];
+        S.r2 = s[3] * s[3];
+        S.on = 1;
+    }
+    return S;
+}
+
+#define LAUNCH3(kern, G, ...)                                                                  \
+    do {                                                                                        \
+        if ((G).nb > 0)                                                                         \
+            hipLaunchKernelGGL(kern, dim3((G).nb), dim3(kTX, kTY), 0, c->stream, __VA_ARGS__); \
+        PANO_HIP(hipGetLastError());                                                            \
+    } while (0)
+
+template <typename T>
+pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
+                          const double* src_sphere) {
+    Faces<const T> v = split_faces(D, vel);
+    Grid3 G = grid3(D.w, D.h, D.d);
+    Sphere S = make_sphere(src_sphere);
+    const double cells = (double)D.d * D.h * D.w * sizeof(T);
+    // read rho and every velocity component once, write rho'
+    PANO_TIMED(c, "advect", (D.d > 1 ? 5.0 : 4.0) * cells, {
+        if (D.d > 1) LAUNCH3((k_advect_cell<T, 3>), G, D, G, dst, src, (T)dt, v, S);
+        else LAUNCH3((k_advect_cell<T, 2>), G, D, G, dst, src, (T)dt, v, S);
+    });
+    return PANO_OK;
+}
+
+template <typename T>
+pano_status launch_advect_mac(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
+                              double buoy_c, const T* rho, const double* src_sphere) {
+    Faces<const T> v = split_faces(D, vel);
+    Faces<const T> q = split_faces(D, src);
+    Faces<T> o = split_faces(D, dst);
+    Sphere S = make_sphere(src_sphere);
+    Grid3 Gx = grid3(D.w + 1, D.h, D.d), Gy = grid3(D.w, D.h + 1, D.d);
+    if (D.d > 1) {
+        Grid3 Gz = grid3(D.w, D.h, D.d + 1);
+        LAUNCH3((k_advect_fx<T, 3>), Gx, D, Gx, o.x, q.x, (T)dt, v);
+        LAUNCH3((k_advect_fy<T, 3>), Gy, D, Gy, o.y, q.y, (T)dt, v);
+        LAUNCH3((k_advect_fz<T>), Gz, D, Gz, o.z, q.z, (T)dt, v, (T)buoy_c, rho, S);
+    } else {
+        LAUNCH3((k_advect_fx<T, 2>), Gx, D, Gx, o.x, q.x, (T)dt, v);
+        LAUNCH3((k_advect_fy<T, 2>), Gy, D, Gy, o.y, q.y, (T)dt, v);
+    }
+    return PANO_OK;
+}
+
+template <typename T>
+pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces) {
+    Faces<const T> v = split_faces(D, faces);
+    Grid3 G = grid3(D.w, D.h, D.d);
+    if (D.d > 1) LAUNCH3((k_neg_div<T, 3>), G, D, G, b, v);
+    else LAUNCH3((k_neg_div<T, 2>), G, D, G, b, v);
+    return PANO_OK;
+}
+
+template <typename T>
+pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t* mask, T* q,
+                                 const T* p) {
+    Grid3 G = grid3(D.w, D.h, D.d);
+    if (mask) {
+        if (D.d != 1 || D.bc != 0)
+            return fail(PANO_ERR_BAD_SHAPE, "face_mask needs a 2D grid with closed walls");
+        LAUNCH3((k_poisson_masked2<T>), G, D, G, q, p, (T)dt, mask);
+    } else if (D.d > 1) {
+        LAUNCH3((k_poisson<T, 3>), G, D, G, q, p, (T)dt);
+    } else {
+        LAUNCH3((k_poisson<T, 2>), G, D, G, q, p, (T)dt);
+    }
+    return PANO_OK;
+}
+
+template <typename T, int L_>
+static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
+    Faces<T> u = split_faces(D, faces);
+    Grid3 Gx = grid3(D.w + 1, D.h, D.d), Gy = grid3(D.w, D.h + 1, D.d);
+    if (D.d > 1) {
+        Grid3 Gz = grid3(D.w, D.h, D.d + 1);
+        LAUNCH3((k_project_x<T, 3, L_>), Gx, D, Gx, u.x, p, (T)dt);
+        LAUNCH3((k_project_y<T, 3, L_>), Gy, D, Gy, u.y, p, (T)dt);
+        LAUNCH3((k_project_z<T, L_>), Gz, D, Gz, u.z, p, (T)dt);
+    } else {
+        LAUNCH3((k_project_x<T, 2, L_>), Gx, D, Gx, u.x, p, (T)dt);
+        LAUNCH3((k_project_y<T, 2, L_>), Gy, D, Gy, u.y, p, (T)dt);
+    }
+    return PANO_OK;
+}
+
+template <typename T>
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
+    return launch_project_l<T, NATURAL>(c, D, dt, faces, p);
+}
+template <typename T>
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
+    return launch_project_l<T, REDBLACK>(c, D, dt, faces, p);
+}
+
+#define INST(T)                                                                                     \
+    template pano_status launch_advect<T>(Ctx*, const Dims&, T*, const T*, double, const T*,         \
+                                          const double*);                                           \
+    template pano_status launch_advect_mac<T>(Ctx*, const Dims&, T*, const T*, double, const T*,     \
+                                              double, const T*, const double*);                     \
+    template pano_status launch_neg_div<T>(Ctx*, const Dims&, T*, const T*);                         \
+    template pano_status launch_poisson_apply<T>(Ctx*, const Dims&, double, const uint8_t*, T*,       \
+                                                 const T*);                                         \
+    template pano_status launch_project<T>(Ctx*, const Dims&, double, T*, const T*);                 \
+    template pano_status launch_project_rb<T>(Ctx*, const Dims&, double, T*, const T*);
+INST(float)
+INST(double)
+#undef INST
+
+// ---- C ABI -----------------------------------------------------------------------------------
+
+static pano_status pre(pano_ctx ctx, const pano_grid* g, Ctx** c) {
+    *c = ctx_of(ctx);
+    if (!*c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    PANO_HIP(hipSetDevice((*c)->device));
+    return PANO_OK;
+}
+
+}  // namespace pano
+
+using namespace pano;
+
+extern "C" {
+
+pano_status pano_neg_divergence(pano_ctx ctx, const pano_grid* g, void* b, const void* faces) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!b || !faces || b == faces) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    return g->dtype == PANO_F64 ? launch_neg_div<double>(c, D, (double*)b, (const double*)faces)
+                                : launch_neg_div<float>(c, D, (float*)b, (const float*)faces);
+}
+
+pano_status pano_poisson_apply(pano_ctx ctx, const pano_grid* g, double dt, const uint8_t* mask,
+                               void* q, const void* p) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!q || !p || q == p) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    return g->dtype == PANO_F64
+               ? launch_poisson_apply<double>(c, D, dt, mask, (double*)q, (const double*)p)
+               : launch_poisson_apply<float>(c, D, dt, mask, (float*)q, (const float*)p);
+}
+
+pano_status pano_project(pano_ctx ctx, const pano_grid* g, double dt, void* faces, const void* p) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!faces || !p || faces == p) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    return g->dtype == PANO_F64 ? launch_project<double>(c, D, dt, (double*)faces, (const double*)p)
+                                : launch_project<float>(c, D, dt, (float*)faces, (const float*)p);
+}
+
+pano_status pano_advect(pano_ctx ctx, const pano_grid* g, void* dst, const void* src, double dt,
+                        const void* vel) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!dst || !src || !vel || dst == src || dst == vel) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    return g->dtype == PANO_F64
+               ? launch_advect<double>(c, D, (double*)dst, (const double*)src, dt, (const double*)vel, nullptr)
+               : launch_advect<float>(c, D, (float*)dst, (const float*)src, dt, (const float*)vel, nullptr);
+}
+
+pano_status pano_advect_mac(pano_ctx ctx, const pano_grid* g, void* dst, const void* src, double dt,
+                            const void* vel) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!dst || !src || !vel || dst == src || dst == vel) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    return g->dtype == PANO_F64
+               ? launch_advect_mac<double>(c, D, (double*)dst, (const double*)src, dt, (const double*)vel, 0.0,
+                                           nullptr, nullptr)
+               : launch_advect_mac<float>(c, D, (float*)dst, (const float*)src, dt, (const float*)vel, 0.0,
+                                          nullptr, nullptr);
+}
+
+}  // extern "C"

@@ -1,0 +1,387 @@
+// Device-side per-element formulas of the fluid step. Every expression keeps
+// the evaluation order of the reference (examples/dec_fluid.rs,
+// panopaea/src/dec/grid.rs, panopaea/src/math/interp.rs) so that, compiled
+// with -ffp-contract=off, each element rounds exactly as the CPU restatement.
+#pragma once
+
+#include "pano_internal.hpp"
+
+namespace pano {
+
+// Layout of a cell field: natural row-major, or red-black compacted
+// ("RB": colour c = (x+y+z)&1, plane c holds the cells of that colour at
+// compact index (z, y, x>>1); requires an even x extent).
+enum CellLayout { NATURAL = 0, REDBLACK = 1 };
+
+__device__ __forceinline__ int64_t cell_index(const Dims& D, int z, int y, int x) {
+    return ((int64_t)z * D.h + y) * D.w + x;
+}
+__device__ __forceinline__ int64_t rb_index(const Dims& D, int z, int y, int x) {
+    int wh = D.w >> 1;
+    int64_t n2 = (int64_t)D.d * D.h * wh;
+    int c = (x + y + z) & 1;
+    return c * n2 + ((int64_t)z * D.h + y) * wh + (x >> 1);
+}
+template <int L>
+__device__ __forceinline__ int64_t lidx(const Dims& D, int z, int y, int x) {
+    if constexpr (L == REDBLACK) return rb_index(D, z, y, x);
+    else return cell_index(D, z, y, x);
+}
+
+// ---- interpolation, math/interp.rs:7-36 -----------------------------------------
+template <typename T>
+__device__ __forceinline__ T lerp(T a0, T a1, T s) {
+    return a0 * ((T)1 - s) + a1 * s;
+}
+template <typename T>
+__device__ __forceinline__ T bilerp(T a00, T a01, T a10, T a11, T s, T t) {
+    return lerp(lerp(a00, a01, s), lerp(a10, a11, s), t);
+}
+template <typename T>
+__device__ __forceinline__ T trilerp(T a000, T a001, T a010, T a011, T a100, T a101, T a110,
+                                     T a111, T s, T t, T u) {
+    return lerp(bilerp(a000, a001, a010, a011, s, t), bilerp(a100, a101, a110, a111, s, t), u);
+}
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+__device__ __forceinline__ float tfloor(float v) { return floorf(v); }
+__device__ __forceinline__ double tfloor(double v) { return floor(v); }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+// ---- face layout views ---------------------------------------------------------------
+template <typename T>
+struct Faces {
+    T* z;  // (d+1) h w   (null in 2D)
+    T* y;  // d (h+1) w
+    T* x;  // d h (w+1)
+};
+template <typename T>
+__host__ __device__ inline Faces<T> split_faces(const Dims& D, T* f) {
+    Faces<T> r;
+    int64_t d = D.d, h = D.h, w = D.w;
+    if (d == 1) {
+        r.z = nullptr;
+        r.y = f;
+        r.x = f + (h + 1) * w;
+    } else {
+        r.z = f;
+        r.y = f + (d + 1) * h * w;
+        r.x = r.y + d * (h + 1) * w;
+    }
+    return r;
+}
+__device__ __forceinline__ int64_t fz_idx(const Dims& D, int z, int y, int x) {
+    return ((int64_t)z * D.h + y) * D.w + x;
+}
+__device__ __forceinline__ int64_t fy_idx(const Dims& D, int z, int y, int x) {
+    return ((int64_t)z * (D.h + 1) + y) * D.w + x;
+}
+__device__ __forceinline__ int64_t fx_idx(const Dims& D, int z, int y, int x) {
+    return ((int64_t)z * D.h + y) * (D.w + 1) + x;
+}
+
+// ---- density source sphere ----------------------------------------------------------
+struct Sphere {
+    double cz, cy, cx, r2;
+    int on;
+};
+__device__ __forceinline__ bool in_sphere(const Sphere& S, const Dims& D, int z, int y, int x) {
+    double dz = (double)z + 0.5 - S.cz;
+    double dy = (double)y + 0.5 - S.cy;
+    double dx = (double)x + 0.5 - S.cx;
+    if (D.d == 1) dz = 0.0;
+    return dz * dz + dy * dy + dx * dx <= S.r2;
+}
+// Density as seen after the source reset (rho = 1 inside the sphere),
+// dec_fluid.rs:48-57 analogue.
+template <typename T>
+__device__ __forceinline__ T rho_eff(const T* rho, const Sphere& S, const Dims& D, int z, int y,
+                                     int x) {
+    if (S.on && in_sphere(S, D, z, y, x)) return (T)1;
+    return rho[cell_index(D, z, y, x)];
+}
+
+// ---- advect (dec_fluid.rs:173-211), per axis of the cell-centred field ----------------
+template <typename T>
+__device__ __forceinline__ void cell_coord(T pp, int n, int& i0, int& i1, T& f) {
+    if (n == 1) {
+        i0 = 0;
+        i1 = 0;
+        f = (T)0;
+        return;
+    }
+    const T half = (T)0.5;
+    T p = tmin(tmax(pp - half, (T)0), (T)n - (T)1.00001);
+    int i = (int)tfloor(p);
+    if (i < 0) i = 0;
+    i0 = i;
+    i1 = imin(i + 1, n - 1);
+    f = p - (T)i;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T advect_cell_value(const Dims& D, const T* q, const Sphere& S, T dt,
+                                               const Faces<const T>& v, int z, int y, int x) {
+    const T half = (T)0.5, two = (T)2.0, ndt = -dt;
+    T vcx = (v.x[fx_idx(D, z, y, x)] + v.x[fx_idx(D, z, y, x + 1)]) / two;
+    T vcy = (v.y[fy_idx(D, z, y, x)] + v.y[fy_idx(D, z, y + 1, x)]) / two;
+    T ppx = ((T)x + half) + ndt * vcx;
+    T ppy = ((T)y + half) + ndt * vcy;
+    int x0, x1, y0, y1;
+    T s, t;
+    cell_coord(ppx, D.w, x0, x1, s);
+    cell_coord(ppy, D.h, y0, y1, t);
+    if constexpr (DIM == 3) {
+        T vcz = (v.z[fz_idx(D, z, y, x)] + v.z[fz_idx(D, z + 1, y, x)]) / two;
+        T ppz = ((T)z + half) + ndt * vcz;
+        int z0, z1;
+        T u;
+        cell_coord(ppz, D.d, z0, z1, u);
+        return trilerp(rho_eff(q, S, D, z0, y0, x0), rho_eff(q, S, D, z0, y0, x1),
+                       rho_eff(q, S, D, z0, y1, x0), rho_eff(q, S, D, z0, y1, x1),
+                       rho_eff(q, S, D, z1, y0, x0), rho_eff(q, S, D, z1, y0, x1),
+                       rho_eff(q, S, D, z1, y1, x0), rho_eff(q, S, D, z1, y1, x1), s, t, u);
+    } else {
+        return bilerp(rho_eff(q, S, D, 0, y0, x0), rho_eff(q, S, D, 0, y0, x1),
+                      rho_eff(q, S, D, 0, y1, x0), rho_eff(q, S, D, 0, y1, x1), s, t);
+    }
+}
+
+// ---- advect_mac (dec_fluid.rs:213-291) ----------------------------------------------
+// Index clamps of :236-244: i = max(floor(pp - off), 0), i0 = min(i, nmax),
+// i1 = min(i+1, nmax), weight clamp(pp - off - i, 0, 1). i is also capped at
+// nmax + 1 so the float-to-int conversion never overflows; this changes
+// nothing for departure points less than one cell beyond the far wall.
+template <typename T>
+__device__ __forceinline__ void face_coord(T pp, T off, int nmax, int& i0, int& i1, T& f) {
+    T fi = tmin(tmax(tfloor(pp - off), (T)0), (T)(nmax + 1));
+    int i = (int)fi;
+    i0 = imin(i, nmax);
+    i1 = imin(i + 1, nmax);
+    f = tmax(tmin(pp - off - (T)i, (T)1), (T)0);
+}
+
+// X faces at (x, y+1/2, z+1/2).
+template <typename T, int DIM>
+__device__ __forceinline__ T advect_face_x(const Dims& D, const T* qx, T dt, const Faces<const T>& v,
+                                           int z, int y, int x) {
+    const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
+    int xa = imin(x, D.w - 1), xb = x > 0 ? x - 1 : 0;
+    T velx = v.x[fx_idx(D, z, y, x)];
+    T vely = (v.y[fy_idx(D, z, y, xa)] + v.y[fy_idx(D, z, y + 1, xa)] + v.y[fy_idx(D, z, y, xb)] +
+              v.y[fy_idx(D, z, y + 1, xb)]) / four;
+    T ppx = ((T)x + zero) + ndt * velx;
+    T ppy = ((T)y + half) + ndt * vely;
+    int x0, x1, y0, y1;
+    T s, t;
+    face_coord(ppx, zero, D.w, x0, x1, s);
+    face_coord(ppy, half, D.h - 1, y0, y1, t);
+    if constexpr (DIM == 3) {
+        T velz = (v.z[fz_idx(D, z, y, xa)] + v.z[fz_idx(D, z + 1, y, xa)] + v.z[fz_idx(D, z, y, xb)] +
+                  v.z[fz_idx(D, z + 1, y, xb)]) / four;
+        T ppz = ((T)z + half) + ndt * velz;
+        int z0, z1;
+        T u;
+        face_coord(ppz, half, D.d - 1, z0, z1, u);
+        return trilerp(qx[fx_idx(D, z0, y0, x0)], qx[fx_idx(D, z0, y0, x1)], qx[fx_idx(D, z0, y1, x0)],
+                       qx[fx_idx(D, z0, y1, x1)], qx[fx_idx(D, z1, y0, x0)], qx[fx_idx(D, z1, y0, x1)],
+                       qx[fx_idx(D, z1, y1, x0)], qx[fx_idx(D, z1, y1, x1)], s, t, u);
+    } else {
+        return bilerp(qx[fx_idx(D, 0, y0, x0)], qx[fx_idx(D, 0, y0, x1)], qx[fx_idx(D, 0, y1, x0)],
+                      qx[fx_idx(D, 0, y1, x1)], s, t);
+    }
+}
+
+// Y faces at (x+1/2, y, z+1/2).
+template <typename T, int DIM>
+__device__ __forceinline__ T advect_face_y(const Dims& D, const T* qy, T dt, const Faces<const T>& v,
+                                           int z, int y, int x) {
+    const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
+    int ya = imin(y, D.h - 1), yb = y > 0 ? y - 1 : 0;
+    T velx = (v.x[fx_idx(D, z, ya, x)] + v.x[fx_idx(D, z, ya, x + 1)] + v.x[fx_idx(D, z, yb, x)] +
+              v.x[fx_idx(D, z, yb, x + 1)]) / four;
+    T vely = v.y[fy_idx(D, z, y, x)];
+    T ppx = ((T)x + half) + ndt * velx;
+    T ppy = ((T)y + zero) + ndt * vely;
+    int x0, x1, y0, y1;
+    T s, t;
+    face_coord(ppx, half, D.w - 1, x0, x1, s);
+    face_coord(ppy, zero, D.h, y0, y1, t);
+    if constexpr (DIM == 3) {
+        T velz = (v.z[fz_idx(D, z, ya, x)] + v.z[fz_idx(D, z + 1, ya, x)] + v.z[fz_idx(D, z, yb, x)] +
+                  v.z[fz_idx(D, z + 1, yb, x)]) / four;
+        T ppz = ((T)z + half) + ndt * velz;
+        int z0, z1;
+        T u;
+        face_coord(ppz, half, D.d - 1, z0, z1, u);
+        return trilerp(qy[fy_idx(D, z0, y0, x0)], qy[fy_idx(D, z0, y0, x1)], qy[fy_idx(D, z0, y1, x0)],
+                       qy[fy_idx(D, z0, y1, x1)], qy[fy_idx(D, z1, y0, x0)], qy[fy_idx(D, z1, y0, x1)],
+                       qy[fy_idx(D, z1, y1, x0)], qy[fy_idx(D, z1, y1, x1)], s, t, u);
+    } else {
+        return bilerp(qy[fy_idx(D, 0, y0, x0)], qy[fy_idx(D, 0, y0, x1)], qy[fy_idx(D, 0, y1, x0)],
+                      qy[fy_idx(D, 0, y1, x1)], s, t);
+    }
+}
+
+// Z faces at (x+1/2, y+1/2, z) (3D only), plus the smoke buoyancy
+// buoy_c*(rho[z-1] + rho[z]) on interior and open faces.
+template <typename T>
+__device__ __forceinline__ T advect_face_z(const Dims& D, const T* qz, T dt, const Faces<const T>& v,
+                                           int z, int y, int x, T buoy_c, const T* rho,
+                                           const Sphere& S) {
+    const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
+    int za = imin(z, D.d - 1), zb = z > 0 ? z - 1 : 0;
+    T velx = (v.x[fx_idx(D, za, y, x)] + v.x[fx_idx(D, za, y, x + 1)] + v.x[fx_idx(D, zb, y, x)] +
+              v.x[fx_idx(D, zb, y, x + 1)]) / four;
+    T vely = (v.y[fy_idx(D, za, y, x)] + v.y[fy_idx(D, za, y + 1, x)] + v.y[fy_idx(D, zb, y, x)] +
+              v.y[fy_idx(D, zb, y + 1, x)]) / four;
+    T velz = v.z[fz_idx(D, z, y, x)];
+    T ppx = ((T)x + half) + ndt * velx;
+    T ppy = ((T)y + half) + ndt * vely;
+    T ppz = ((T)z + zero) + ndt * velz;
+    int x0, x1, y0, y1, z0, z1;
+    T s, t, u;
+    face_coord(ppx, half, D.w - 1, x0, x1, s);
+    face_coord(ppy, half, D.h - 1, y0, y1, t);
+    face_coord(ppz, zero, D.d, z0, z1, u);
+    T val = trilerp(qz[fz_idx(D, z0, y0, x0)], qz[fz_idx(D, z0, y0, x1)], qz[fz_idx(D, z0, y1, x0)],
+                    qz[fz_idx(D, z0, y1, x1)], qz[fz_idx(D, z1, y0, x0)], qz[fz_idx(D, z1, y0, x1)],
+                    qz[fz_idx(D, z1, y1, x0)], qz[fz_idx(D, z1, y1, x1)], s, t, u);
+    if (buoy_c != zero && rho) {
+        bool open_face = (z > 0 && z < D.d) || (z == 0 && (D.bc & 16)) || (z == D.d && (D.bc & 32));
+        if (open_face) {
+            T rlo = z > 0 ? rho_eff(rho, S, D, z - 1, y, x) : zero;
+            T rhi = z < D.d ? rho_eff(rho, S, D, z, y, x) : zero;
+            val = val + buoy_c * (rlo + rhi);
+        }
+    }
+    return val;
+}
+
+// ---- divergence: hodge_1_dual + derivative_1_primal (grid.rs:223-238, 295-305) ----------
+// div = (vz[z+1]-vz[z]) + (((vy[y+1]-vy[y]) - vx[x]) + vx[x+1]).
+template <typename T, int DIM>
+__device__ __forceinline__ T div_value(const Dims& D, const Faces<const T>& v, int z, int y, int x) {
+    T d2 = ((v.y[fy_idx(D, z, y + 1, x)] - v.y[fy_idx(D, z, y, x)]) - v.x[fx_idx(D, z, y, x)]) +
+           v.x[fx_idx(D, z, y, x + 1)];
+    if constexpr (DIM == 3) return (v.z[fz_idx(D, z + 1, y, x)] - v.z[fz_idx(D, z, y, x)]) + d2;
+    else return d2;
+}
+
+// ---- the pressure operator closure (dec_fluid.rs:100-119) -------------------------------
+// q = dt*((-Zb + Zt) + (((-Yb + Yt) - Xl) + Xr)) with primal face values
+// X = p[-] - p[+], Y/Z = -(-(p[+] - p[-])) (derivative_0_dual then
+// hodge_1_dual), open sides using p = 0 in the ghost cell and closed walls
+// the untouched zero (+0 for x, -0 for y/z after the hodge sign flip).
+template <typename T, int DIM, int L>
+__device__ __forceinline__ T poisson_value(const Dims& D, const T* p, T dt, int z, int y, int x) {
+    const T zero = (T)0;
+    T pc = p[lidx<L>(D, z, y, x)];
+    T xl, xr, yt, yb;
+    if (x > 0) xl = p[lidx<L>(D, z, y, x - 1)] - pc;
+    else xl = (D.bc & 1) ? zero - pc : zero;
+    if (x < D.w - 1) xr = pc - p[lidx<L>(D, z, y, x + 1)];
+    else xr = (D.bc & 2) ? pc - zero : zero;
+    if (y > 0) yt = -(-(pc - p[lidx<L>(D, z, y - 1, x)]));
+    else yt = (D.bc & 4) ? -(-(pc - zero)) : -zero;
+    if (y < D.h - 1) yb = -(-(p[lidx<L>(D, z, y + 1, x)] - pc));
+    else yb = (D.bc & 8) ? -(-(zero - pc)) : -zero;
+    T q2 = ((-yb + yt) - xl) + xr;
+    if constexpr (DIM == 3) {
+        T zt, zb;
+        if (z > 0) zt = -(-(pc - p[lidx<L>(D, z - 1, y, x)]));
+        else zt = (D.bc & 16) ? -(-(pc - zero)) : -zero;
+        if (z < D.d - 1) zb = -(-(p[lidx<L>(D, z + 1, y, x)] - pc));
+        else zb = (D.bc & 32) ? -(-(zero - pc)) : -zero;
+        q2 = (-zb + zt) + q2;
+    }
+    return q2 * dt;
+}
+
+// Number of non-wall neighbours: the diagonal of the operator is dt*nc.
+template <int DIM>
+__device__ __forceinline__ int ncount(const Dims& D, int z, int y, int x) {
+    int nc = 0;
+    nc += (x > 0) ? 1 : (D.bc & 1);
+    nc += (x < D.w - 1) ? 1 : ((D.bc >> 1) & 1);
+    nc += (y > 0) ? 1 : ((D.bc >> 2) & 1);
+    nc += (y < D.h - 1) ? 1 : ((D.bc >> 3) & 1);
+    if constexpr (DIM == 3) {
+        nc += (z > 0) ? 1 : ((D.bc >> 4) & 1);
+        nc += (z < D.d - 1) ? 1 : ((D.bc >> 5) & 1);
+    }
+    return nc;
+}
+
+// ---- projection (dec_fluid.rs:124-141) ----------------------------------------------------
+// u = u + dt*g, g = derivative_0_dual(p): x faces p[x-1]-p[x], y/z faces
+// -(p[+]-p[-]); closed-wall faces set to zero, open faces use p = 0 outside.
+template <typename T, int L>
+__device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
+    const T zero = (T)0;
+    T g;
+    if (x > 0 && x < D.w) g = p[lidx<L>(D, z, y, x - 1)] - p[lidx<L>(D, z, y, x)];
+    else if (x == 0) {
+        if (!(D.bc & 1)) return zero;
+        g = zero - p[lidx<L>(D, z, y, 0)];
+    } else {
+        if (!(D.bc & 2)) return zero;
+        g = p[lidx<L>(D, z, y, D.w - 1)] - zero;
+    }
+    return u + dt * g;
+}
+template <typename T, int L>
+__device__ __forceinline__ T project_y(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
+    const T zero = (T)0;
+    T g;
+    if (y > 0 && y < D.h) g = -(p[lidx<L>(D, z, y, x)] - p[lidx<L>(D, z, y - 1, x)]);
+    else if (y == 0) {
+        if (!(D.bc & 4)) return zero;
+        g = -(p[lidx<L>(D, z, 0, x)] - zero);
+    } else {
+        if (!(D.bc & 8)) return zero;
+        g = -(zero - p[lidx<L>(D, z, D.h - 1, x)]);
+    }
+    return u + dt * g;
+}
+template <typename T, int L>
+__device__ __forceinline__ T project_z(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
+    const T zero = (T)0;
+    T g;
+    if (z > 0 && z < D.d) g = -(p[lidx<L>(D, z, y, x)] - p[lidx<L>(D, z - 1, y, x)]);
+    else if (z == 0) {
+        if (!(D.bc & 16)) return zero;
+        g = -(p[lidx<L>(D, 0, y, x)] - zero);
+    } else {
+        if (!(D.bc & 32)) return zero;
+        g = -(zero - p[lidx<L>(D, D.d - 1, y, x)]);
+    }
+    return u + dt * g;
+}
+
+// ---- tiled 3D launch: 64 x 4 threads over (x, y) of one z plane --------------------------
+constexpr int kTX = 64, kTY = 4;
+struct Grid3 {
+    int gx, gy, gz, nb;
+};
+inline Grid3 grid3(int64_t W, int64_t H, int64_t Dz) {
+    Grid3 g;
+    g.gx = (int)((W + kTX - 1) / kTX);
+    g.gy = (int)((H + kTY - 1) / kTY);
+    g.gz = (int)Dz;
+    g.nb = g.gx * g.gy * g.gz;
+    return g;
+}
+__device__ __forceinline__ void tile_xyz(const Grid3& g, int& x, int& y, int& z, int& L) {
+    L = xcd_remap(blockIdx.x, g.nb);
+    int bx = L % g.gx;
+    int t = L / g.gx;
+    int by = t % g.gy;
+    z = t / g.gy;
+    x = bx * kTX + threadIdx.x;
+    y = by * kTY + threadIdx.y;
+}
+
+}  // namespace pano

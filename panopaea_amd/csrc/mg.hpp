@@ -1,0 +1,37 @@
+// Geometric multigrid V-cycle used as the PCG preconditioner z = M r.
+// No counterpart in the reference (it plugs into the Preconditioner seam of
+// panopaea/src/pcg.rs:4-6). Level l has dims >> l (z stays 1 in 2D) and
+// operator A_l = s_l * (nc*z - sum(neighbours)), s_l = dt / 4^l.
+#pragma once
+
+#include "pano_internal.hpp"
+
+namespace pano {
+
+template <typename T>
+struct MG {
+    int levels = 0, nu = 2;
+    Dims dims[16];
+    double s[16];
+    T* z[16];   // level >= 1 (level 0 uses the caller's z)
+    T* r[16];   // level >= 1
+    T* res = nullptr;         // residual scratch, sized for level 0
+    double* fdm_tmp[2] = {nullptr, nullptr};
+    double* fdm_v[3] = {nullptr, nullptr, nullptr};     // eigenvectors (column k = mode k), x/y/z
+    double* fdm_lam[3] = {nullptr, nullptr, nullptr};   // eigenvalues
+    int coarse_n[3] = {1, 1, 1};                         // coarsest extents x, y, z
+};
+
+// Validates dims/levels for the V-cycle; returns PANO_ERR_BAD_SHAPE with a
+// message when unsupported.
+pano_status mg_check(const Dims& D, int levels);
+
+template <typename T>
+pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>* m);
+
+// z0 = M r0 (one V-cycle from zero), both in red-black layout at level 0.
+// Every kernel returns at once when st->done is set.
+template <typename T>
+pano_status mg_vcycle(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st);
+
+}  // namespace pano

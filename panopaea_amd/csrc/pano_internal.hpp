@@ -1,0 +1,240 @@
+// Internal declarations shared by the HIP translation units of libpano.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pano.h"
+
+namespace pano {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const char* fmt, ...);
+pano_status fail(pano_status s, const char* fmt, ...);
+
+#define PANO_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::pano::fail(PANO_ERR_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, \
+                                hipGetErrorString(e_));                                    \
+    } while (0)
+
+#define PANO_CHECK(...)                \
+    do {                               \
+        pano_status s_ = (__VA_ARGS__);\
+        if (s_ < 0) return s_;         \
+    } while (0)
+
+// ---- device buffers cached per context -------------------------------------
+struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+// Solver scalars kept on the device so the CG loop never waits on the host.
+struct SolverState {
+    double sigma;      // z.r
+    double alpha;
+    double beta;
+    double pq;         // p.q
+    double rnorm;      // ||r||_inf after the last update
+    double threshold;  // absolute stop threshold
+    double b_linf;
+    double mean;       // mean of b (removed when requested)
+    int32_t iter;      // operator applications done
+    int32_t max_iter;
+    int32_t done;      // kernels of the loop return at once when set
+    int32_t converged;
+    int32_t nonfinite;
+    int32_t pad[3];
+};
+
+// Times every launch of one named kernel with HIP events on the context's
+// stream. Launches inside a solve are tagged with the number of convergence
+// checks enqueued before them so that launches the converged flag turned
+// into no-ops are left out of the average.
+struct KernelTimer {
+    std::string name;                 // kernel being timed ("" = off)
+    std::vector<hipEvent_t> ev;       // start/end pairs
+    std::vector<int> tag_solve, tag_checks;
+    std::vector<int> solve_iters;     // final iteration count per solve
+    size_t used = 0;
+    int solve = -1;                   // current solve id (-1: outside any solve)
+    int checks = 0;                   // convergence checks enqueued in this solve
+    double bytes = 0.0;               // algorithmic bytes per launch (set by the kernel site)
+};
+
+struct Ctx {
+    int device = 0;
+    KernelTimer timer;
+    hipStream_t stream = nullptr;
+    std::map<std::string, Buffer> pool;  // named workspace buffers
+    std::map<std::string, void*> consts; // immutable device tables (filled once)
+    SolverState* host_state = nullptr;   // pinned mirror of the device state
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int num_cu = 256;
+    // Returns a device buffer of at least `bytes` for `name`, (re)allocated
+    // on growth; contents are unspecified.
+    pano_status workspace(const char* name, size_t bytes, void** out);
+};
+
+Ctx* ctx_of(pano_ctx c);
+bool timer_on(Ctx* c, const char* name);
+void timer_mark(Ctx* c);  // records one event (call before and after the launch)
+
+// Wraps a launch statement with timer events when `name` is being timed.
+#define PANO_TIMED(c, name, bytes_, ...)             \
+    do {                                             \
+        bool t_ = ::pano::timer_on((c), (name));      \
+        if (t_) {                                    \
+            (c)->timer.bytes = (bytes_);             \
+            ::pano::timer_mark(c);                    \
+        }                                            \
+        __VA_ARGS__;                                 \
+        if (t_) ::pano::timer_mark(c);                \
+    } while (0)
+pano_status check_grid(const pano_grid* g);
+size_t dtype_size(int32_t dtype);
+
+// ---- launch helpers ---------------------------------------------------------
+constexpr int kBlock = 256;
+constexpr int kMaxPartials = 16384;
+
+inline unsigned blocks_for(int64_t n, int per_block) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
+// (workgroup b runs on XCD b % 8). Remap so each XCD owns a contiguous range
+// of logical blocks: neighbouring rows/planes share one XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int nx = 8;
+    int q = nb / nx, rem = nb % nx;
+    int x = b % nx, k = b / nx;
+    int start = x * q + (x < rem ? x : rem);
+    return start + k;
+}
+
+// ---- wavefront / block reductions (64-wide wavefronts) -----------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// Block of blockDim.x*blockDim.y*blockDim.z threads (a multiple of 64, <= 1024).
+// Fixed butterfly + fixed LDS order: deterministic. Result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red_s[16];
+    int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    int nthr = blockDim.x * blockDim.y * blockDim.z;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((tid & 63) == 0) red_s[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < (nthr >> 6); ++i) s += red_s[i];
+    return s;
+}
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double redm_s[16];
+    int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    int nthr = blockDim.x * blockDim.y * blockDim.z;
+    v = wave_max(v);
+    __syncthreads();
+    if ((tid & 63) == 0) redm_s[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < (nthr >> 6); ++i) s = fmax(s, redm_s[i]);
+    return s;
+}
+
+// NaN-propagating |x| max for norms: a NaN anywhere makes the norm NaN.
+__device__ __forceinline__ double absmax_acc(double m, double x) {
+    double a = fabs(x);
+    return (a > m || a != a) ? a : m;
+}
+
+// ---- level geometry -----------------------------------------------------------
+struct Dims {
+    int d, h, w;  // cells (d == 1 in 2D)
+    int bc;       // bit i set: side i open (x-, x+, y-, y+, z-, z+)
+};
+
+inline int bc_mask(const pano_grid* g) {
+    int m = 0;
+    for (int i = 0; i < 6; ++i)
+        if (g->bc[i]) m |= 1 << i;
+    if (g->ndim == 2) m &= 0xF;
+    return m;
+}
+inline Dims dims_of(const pano_grid* g) {
+    return Dims{(int)(g->ndim == 3 ? g->nz : 1), (int)g->ny, (int)g->nx, bc_mask(g)};
+}
+inline bool closed_box(const Dims& D) { return D.bc == 0; }
+
+// ---- kernels launched from several translation units ---------------------------
+pano_status fill_zero(Ctx* c, void* dst, size_t bytes);
+
+// Fluid-step operator launches (fluid.hip), templated on T in {float,double}.
+template <typename T>
+pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces);
+template <typename T>
+pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t* mask, T* q,
+                                 const T* p);
+template <typename T>
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p);
+template <typename T>
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p);
+template <typename T>
+pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
+                          const double* src_sphere /* cz,cy,cx,r or null */);
+template <typename T>
+pano_status launch_advect_mac(Ctx* c, const Dims& D, T* dst, const T* src, double dt,
+                              const T* vel, double buoy_c, const T* rho,
+                              const double* src_sphere);
+
+// Face-array component offsets in the library layout.
+struct FaceLayout {
+    int64_t nz, ny, nx;  // element counts of vz, vy, vx
+    int64_t oz, oy, ox;  // offsets
+    int64_t total;
+};
+inline FaceLayout face_layout(const Dims& D) {
+    FaceLayout f;
+    int64_t d = D.d, h = D.h, w = D.w;
+    if (d == 1) {
+        f.nz = 0;
+        f.ny = (h + 1) * w;
+        f.nx = h * (w + 1);
+    } else {
+        f.nz = (d + 1) * h * w;
+        f.ny = d * (h + 1) * w;
+        f.nx = d * h * (w + 1);
+    }
+    f.oz = 0;
+    f.oy = f.nz;
+    f.ox = f.nz + f.ny;
+    f.total = f.nz + f.ny + f.nx;
+    return f;
+}
+
+// PCG driver (pcg.hip).
+template <typename T>
+pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, const T* b,
+                      T* residual, T* auxiliary, T* search, pano_pcg_info* info,
+                      bool b_in_residual_rb);
+
+}  // namespace pano

@@ -1,0 +1,742 @@
+// Preconditioned conjugate gradient on the GPU, panopaea/src/pcg.rs:14-82.
+//
+// The iteration keeps the reference's order and stop test:
+//   x = 0; early out if ||b||_inf < thr; r = b; z = M r; p = z; sigma = z.r
+//   loop: q = A p; alpha = sigma/(q.p); x += alpha p; r -= alpha q;
+//         stop if ||r||_inf < thr; z = M r; sigma' = z.r; beta = sigma'/sigma;
+//         p = z + beta p
+// The scalars live on the device (SolverState): each reduction kernel writes
+// one FP64 partial per workgroup, a one-workgroup finalize kernel reduces
+// them in a fixed order (deterministic, identical alone or batched) and
+// updates the state; every loop kernel returns at once when state.done is
+// set, so the host can enqueue several iterations ahead and poll the state
+// asynchronously (two chunks in flight) without changing the result.
+//
+// Arithmetic: fields in T, dot products accumulated in FP64, alpha/beta in
+// FP64, x/r/p updates formed in FP64 and rounded once to T.
+#include "fluid_device.hpp"
+#include "mg.hpp"
+
+namespace pano {
+
+constexpr int kItems = 8;  // elements per thread in the flat vector kernels
+constexpr int kFlatPer = kBlock * kItems;
+
+enum FinOp { FIN_STATS = 0, FIN_THRESH, FIN_SIGMA0, FIN_ALPHA, FIN_RNORM, FIN_BETA };
+
+struct SolveCfg {
+    int32_t tol_kind;
+    int32_t remove_mean;
+    double tol;
+    int64_t n;
+};
+
+__global__ __launch_bounds__(1024) void k_finalize(int op, const double* part, int np, SolverState* st,
+                                                   SolveCfg cfg, int use_max) {
+    if (op >= FIN_SIGMA0 && st->done) return;  // uniform: every thread returns
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        double v = part[i];
+        acc = use_max ? absmax_acc(acc, v) : acc + v;
+    }
+    double tot = use_max ? block_max(acc) : block_sum(acc);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_STATS:  // tot = sum(b): the mean to remove
+            st->mean = tot / (double)cfg.n;
+            break;
+        case FIN_THRESH: {  // tot = ||b||_inf (after mean removal)
+            st->b_linf = tot;
+            st->rnorm = tot;
+            double thr = cfg.tol_kind == PANO_TOL_ABS_LINF ? cfg.tol : cfg.tol * tot;
+            st->threshold = thr;
+            st->iter = 0;
+            st->done = 0;
+            st->converged = 0;
+            st->nonfinite = 0;
+            if (!(tot == tot) || tot > 1.7e308) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;  // the early-out test runs after rounding thr to T
+        }
+        case FIN_SIGMA0:
+            st->sigma = tot;
+            break;
+        case FIN_ALPHA:
+            st->pq = tot;
+            st->alpha = st->sigma / tot;
+            if (!(st->alpha == st->alpha) || fabs(st->alpha) > 1.7e308) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;
+        case FIN_RNORM:
+            st->rnorm = tot;
+            st->iter += 1;
+            if (!(tot == tot)) {
+                st->nonfinite = 1;
+                st->done = 1;
+            } else if (tot < st->threshold) {  // pcg.rs:58-63
+                st->converged = 1;
+                st->done = 1;
+            } else if (st->iter >= st->max_iter) {
+                st->done = 1;
+            }
+            break;
+        case FIN_BETA: {
+            double sn = tot;
+            st->beta = sn / st->sigma;
+            st->sigma = sn;
+            if (!(st->beta == st->beta)) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;
+        }
+    }
+}
+
+// Threshold rounding: the stop test compares T values, so the threshold is
+// rounded to T like the reference's `threshold: T`.
+template <typename T>
+__global__ void k_round_threshold(SolverState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->threshold = (double)(T)st->threshold;
+        if (!st->done && st->b_linf < st->threshold) {  // pcg.rs:35-38
+            st->converged = 1;
+            st->done = 1;
+        }
+    }
+}
+
+__global__ void k_state_init(SolverState* st, int max_iter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        SolverState s = {};
+        s.max_iter = max_iter;
+        *st = s;
+    }
+}
+
+// partial sum and max|.| of r (the right-hand side)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats(int64_t n, const T* r, double* psum, double* pmax) {
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double s = 0.0, m = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            double v = (double)r[i];
+            s += v;
+            m = absmax_acc(m, v);
+        }
+    }
+    double ts = block_sum(s);
+    double tm = block_max(m);
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = ts;
+        pmax[blockIdx.x] = tm;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_mean(int64_t n, T* r, const SolverState* st, double* pmax) {
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double mean = st->mean, m = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            T v = (T)((double)r[i] - mean);
+            r[i] = v;
+            m = absmax_acc(m, (double)v);
+        }
+    }
+    double tm = block_max(m);
+    if (threadIdx.x == 0) pmax[blockIdx.x] = tm;
+}
+
+// x = 0
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(int64_t n, T* x) {
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) x[i] = (T)0;
+    }
+}
+
+// p = z ; partial z.r
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_dot(int64_t n, T* p, const T* z, const T* r, double* part,
+                                                  const SolverState* st) {
+    if (st->done) return;
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            T zv = z[i];
+            p[i] = zv;
+            s += (double)zv * (double)r[i];
+        }
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// partial z.r
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(int64_t n, const T* a, const T* b, double* part,
+                                             const SolverState* st) {
+    if (st->done) return;
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) s += (double)a[i] * (double)b[i];
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||_inf (and r.r for the identity
+// preconditioner, where z aliases r).
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(int64_t n, T* x, T* r, const T* p, const T* q,
+                                                double* pmax, double* prr, const SolverState* st) {
+    if (st->done) return;
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double alpha = st->alpha;
+    double m = 0.0, s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+            T rv = (T)((double)r[i] - alpha * (double)q[i]);
+            r[i] = rv;
+            m = absmax_acc(m, (double)rv);
+            s += (double)rv * (double)rv;
+        }
+    }
+    double tm = block_max(m);
+    if (prr) {
+        double ts = block_sum(s);
+        if (threadIdx.x == 0) prr[blockIdx.x] = ts;
+    }
+    if (threadIdx.x == 0) pmax[blockIdx.x] = tm;
+}
+
+// p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void k_search(int64_t n, T* p, const T* z, const SolverState* st) {
+    if (st->done) return;
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double beta = st->beta;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+    }
+}
+
+// q = A p with partial p.q ; natural layout, optional 2D obstacle mask.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_apply_dot_nat(Dims D, Grid3 G, T* q, const T* p, T dt,
+                                                       const uint8_t* mask, double* part,
+                                                       const SolverState* st) {
+    if (st->done) return;
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    double s = 0.0;
+    if (x < D.w && y < D.h) {
+        T v;
+        if (DIM == 2 && mask) {
+            const int h = D.h, w = D.w;
+            const T zero = (T)0;
+            const uint8_t* my = mask;
+            const uint8_t* mx = mask + (int64_t)(h + 1) * w;
+            T pc = p[(int64_t)y * w + x];
+            T ey_t = (y >= 1 && my[(int64_t)y * w + x]) ? -(pc - p[(int64_t)(y - 1) * w + x]) : zero;
+            T ey_b = (y + 1 <= h - 1 && my[(int64_t)(y + 1) * w + x]) ? -(p[(int64_t)(y + 1) * w + x] - pc) : zero;
+            T ex_l = (x >= 1 && mx[(int64_t)y * (w + 1) + x]) ? p[(int64_t)y * w + x - 1] - pc : zero;
+            T ex_r = (x + 1 <= w - 1 && mx[(int64_t)y * (w + 1) + x + 1]) ? pc - p[(int64_t)y * w + x + 1] : zero;
+            T top = -ey_t, bottom = -ey_b;
+            v = (-bottom + top - ex_l + ex_r) * dt;
+        } else {
+            v = poisson_value<T, DIM, NATURAL>(D, p, dt, z, y, x);
+        }
+        int64_t i = cell_index(D, z, y, x);
+        q[i] = v;
+        s = (double)p[i] * (double)v;
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
+}
+
+// q = A p with partial p.q ; red-black layout (thread = x-pair).
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T dt,
+                                                      double* part, const SolverState* st) {
+    if (st->done) return;
+    int m, y, z, L;
+    tile_xyz(G, m, y, z, L);
+    double s = 0.0;
+    if (m < (D.w >> 1) && y < D.h) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int x = 2 * m + k;
+            T v = poisson_value<T, DIM, REDBLACK>(D, p, dt, z, y, x);
+            int64_t i = rb_index(D, z, y, x);
+            q[i] = v;
+            s += (double)p[i] * (double)v;
+        }
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
+}
+
+// Jacobi z = r / (dt*nc) with partial z.r ; natural layout.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_jacobi_dot(Dims D, Grid3 G, T* zv, const T* r, T dt,
+                                                    const uint8_t* mask, double* part,
+                                                    const SolverState* st) {
+    if (st->done) return;
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    double s = 0.0;
+    if (x < D.w && y < D.h) {
+        int nc;
+        if (DIM == 2 && mask) {
+            const int h = D.h, w = D.w;
+            const uint8_t* my = mask;
+            const uint8_t* mx = mask + (int64_t)(h + 1) * w;
+            nc = (y >= 1 && my[(int64_t)y * w + x]) + (y + 1 <= h - 1 && my[(int64_t)(y + 1) * w + x]) +
+                 (x >= 1 && mx[(int64_t)y * (w + 1) + x]) + (x + 1 <= w - 1 && mx[(int64_t)y * (w + 1) + x + 1]);
+        } else {
+            nc = ncount<DIM>(D, z, y, x);
+        }
+        int64_t i = cell_index(D, z, y, x);
+        T rv = r[i];
+        T v = nc ? rv / (dt * (T)nc) : (T)0;
+        zv[i] = v;
+        s = (double)v * (double)rv;
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
+}
+
+// natural <-> red-black permutation
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_rb(Dims D, Grid3 G, T* dst, const T* src) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    dst[rb_index(D, z, y, x)] = src[cell_index(D, z, y, x)];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_from_rb(Dims D, Grid3 G, T* dst, const T* src) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    dst[cell_index(D, z, y, x)] = src[rb_index(D, z, y, x)];
+}
+
+#define L3(kern, G, ...)                                                                         \
+    do {                                                                                          \
+        if ((G).nb > 0)                                                                           \
+            hipLaunchKernelGGL(kern, dim3((G).nb), dim3(kTX, kTY), 0, c->stream, __VA_ARGS__);   \
+    } while (0)
+
+// The solver plan shared by pano_precond_conjugate_gradient and the fused step.
+template <typename T>
+struct Solver {
+    Ctx* c;
+    Dims D;
+    int64_t n;
+    int precond;
+    bool rb;             // red-black layout (multigrid)
+    const uint8_t* mask;
+    T dt;
+    SolverState* st;
+    double* part0;       // partials
+    double* part1;
+    int nflat, ntile;
+    Grid3 G;
+    MG<T> mg;
+    SolveCfg cfg;
+    // vectors (layout per rb)
+    T *x, *r, *aux, *p;
+};
+
+template <typename T>
+static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P, Solver<T>* S) {
+    S->c = c;
+    S->D = D;
+    S->n = (int64_t)D.d * D.h * D.w;
+    S->precond = P->precond;
+    S->rb = P->precond == PANO_PRECOND_MULTIGRID;
+    S->mask = P->face_mask;
+    S->dt = (T)P->dt;
+    if (P->precond < 0 || P->precond > 2) return fail(PANO_ERR_INVALID, "unknown preconditioner %d", P->precond);
+    if (P->tol_kind != PANO_TOL_ABS_LINF && P->tol_kind != PANO_TOL_REL_LINF)
+        return fail(PANO_ERR_INVALID, "unknown tol_kind %d", P->tol_kind);
+    if (P->max_iterations < 0) return fail(PANO_ERR_INVALID, "max_iterations < 0");
+    if (S->mask && (D.d != 1 || D.bc != 0 || S->rb))
+        return fail(PANO_ERR_BAD_SHAPE, "face_mask needs a 2D closed grid and the identity or Jacobi preconditioner");
+    if (S->rb) {
+        if (P->mg_nu1 != P->mg_nu2 || P->mg_nu1 < 1)
+            return fail(PANO_ERR_INVALID, "multigrid needs mg_nu1 == mg_nu2 >= 1 (a symmetric V-cycle)");
+        PANO_CHECK(mg_setup<T>(c, D, P->mg_levels, P->mg_nu1, P->dt, &S->mg));
+    }
+    S->nflat = (int)blocks_for(S->n, kFlatPer);
+    S->G = S->rb ? grid3(D.w >> 1, D.h, D.d) : grid3(D.w, D.h, D.d);
+    S->ntile = S->G.nb;
+    int np = S->nflat > S->ntile ? S->nflat : S->ntile;
+    void *st, *p0, *p1;
+    PANO_CHECK(c->workspace("pcg_state", sizeof(SolverState), &st));
+    PANO_CHECK(c->workspace("pcg_part0", (size_t)np * sizeof(double), &p0));
+    PANO_CHECK(c->workspace("pcg_part1", (size_t)np * sizeof(double), &p1));
+    S->st = (SolverState*)st;
+    S->part0 = (double*)p0;
+    S->part1 = (double*)p1;
+    S->cfg.tol_kind = P->tol_kind;
+    S->cfg.tol = P->threshold;
+    S->cfg.n = S->n;
+    int rm = P->remove_mean;
+    S->cfg.remove_mean = rm < 0 ? (D.bc == 0) : rm;
+    return PANO_OK;
+}
+
+template <typename T>
+static void fin(Solver<T>* S, int op, double* part, int np, int use_max) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, S->c->stream, op, part, np, S->st, S->cfg, use_max);
+    if (op == FIN_RNORM) S->c->timer.checks += 1;
+}
+
+// z = M r (z == r for the identity: nothing to do) followed by the partial z.r
+template <typename T>
+static pano_status precond_dot(Solver<T>* S, T* z, const T* r, double* part, int* np) {
+    Ctx* c = S->c;
+    const Dims& D = S->D;
+    if (S->precond == PANO_PRECOND_JACOBI) {
+        if (D.d > 1) L3((k_jacobi_dot<T, 3>), S->G, D, S->G, z, r, S->dt, S->mask, part, S->st);
+        else L3((k_jacobi_dot<T, 2>), S->G, D, S->G, z, r, S->dt, S->mask, part, S->st);
+        *np = S->ntile;
+    } else {
+        if (S->precond == PANO_PRECOND_MULTIGRID) PANO_CHECK(mg_vcycle<T>(c, &S->mg, z, r, S->st));
+        PANO_TIMED(c, "dot", 2.0 * S->n * sizeof(T),
+                   hipLaunchKernelGGL((k_dot<T>), dim3(S->nflat), dim3(kBlock), 0, c->stream, S->n, z, r, part,
+                                      S->st));
+        *np = S->nflat;
+    }
+    return PANO_OK;
+}
+
+// Initialise from b already in S->r (in the solver's layout).
+template <typename T>
+static pano_status solver_init(Solver<T>* S, int max_iterations) {
+    Ctx* c = S->c;
+    hipStream_t st = c->stream;
+    c->timer.solve = (int)c->timer.solve_iters.size();
+    c->timer.checks = 0;
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, st, S->st, max_iterations);
+    hipLaunchKernelGGL((k_zero<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x);  // pcg.rs:32
+    hipLaunchKernelGGL((k_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->part0, S->part1);
+    if (S->cfg.remove_mean) {
+        fin(S, FIN_STATS, S->part0, S->nflat, 0);
+        hipLaunchKernelGGL((k_sub_mean<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->st, S->part1);
+    }
+    fin(S, FIN_THRESH, S->part1, S->nflat, 1);
+    hipLaunchKernelGGL((k_round_threshold<T>), dim3(1), dim3(64), 0, st, S->st);
+    // z = M r ; p = z ; sigma = z.r   (pcg.rs:41-46)
+    T* z = S->precond == PANO_PRECOND_IDENTITY ? S->r : S->aux;
+    int np = 0;
+    if (S->precond == PANO_PRECOND_IDENTITY) {
+        hipLaunchKernelGGL((k_copy_dot<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, S->r, S->r, S->part0,
+                           S->st);
+        np = S->nflat;
+    } else {
+        PANO_CHECK(precond_dot<T>(S, z, S->r, S->part0, &np));
+        hipLaunchKernelGGL((k_copy_dot<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, z, S->r, S->part0,
+                           S->st);
+        np = S->nflat;
+    }
+    fin(S, FIN_SIGMA0, S->part0, np, 0);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+template <typename T>
+static pano_status solver_iteration(Solver<T>* S) {
+    Ctx* c = S->c;
+    const Dims& D = S->D;
+    hipStream_t st = c->stream;
+    // q = A p ; alpha   (pcg.rs:51-53)  -- q lives in aux
+    const double unit = (double)S->n * sizeof(T);  // one field pass
+    PANO_TIMED(c, "apply_dot", 2 * unit, {
+        if (S->rb) {
+            if (D.d > 1) L3((k_apply_dot_rb<T, 3>), S->G, D, S->G, S->aux, S->p, S->dt, S->part0, S->st);
+            else L3((k_apply_dot_rb<T, 2>), S->G, D, S->G, S->aux, S->p, S->dt, S->part0, S->st);
+        } else {
+            if (D.d > 1) L3((k_apply_dot_nat<T, 3>), S->G, D, S->G, S->aux, S->p, S->dt, S->mask, S->part0, S->st);
+            else L3((k_apply_dot_nat<T, 2>), S->G, D, S->G, S->aux, S->p, S->dt, S->mask, S->part0, S->st);
+        }
+    });
+    fin(S, FIN_ALPHA, S->part0, S->ntile, 0);
+    // x += alpha p ; r -= alpha q ; ||r||_inf   (pcg.rs:55-63)
+    bool ident = S->precond == PANO_PRECOND_IDENTITY;
+    PANO_TIMED(c, "update", 6 * unit,
+               hipLaunchKernelGGL((k_update<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x, S->r, S->p,
+                                  S->aux, S->part1, ident ? S->part0 : nullptr, S->st));
+    fin(S, FIN_RNORM, S->part1, S->nflat, 1);
+    // z = M r ; sigma' = z.r ; beta   (pcg.rs:65-68)
+    T* z;
+    if (ident) {
+        z = S->r;
+        fin(S, FIN_BETA, S->part0, S->nflat, 0);
+    } else {
+        z = S->aux;
+        int np = 0;
+        PANO_CHECK(precond_dot<T>(S, z, S->r, S->part0, &np));
+        fin(S, FIN_BETA, S->part0, np, 0);
+    }
+    // p = z + beta p   (pcg.rs:72-77)
+    PANO_TIMED(c, "search", 3 * unit,
+               hipLaunchKernelGGL((k_search<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, z, S->st));
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+// Runs the loop with the state polled every `every` iterations, two chunks in
+// flight. Leaves the final state in c->host_state[0].
+template <typename T>
+static pano_status solver_loop(Solver<T>* S, int max_iterations, int every) {
+    Ctx* c = S->c;
+    SolverState* hs = c->host_state;
+    if (every <= 0) every = S->n >= (1 << 20) ? 4 : 8;
+    int launched = 0, chunk = 0;
+    bool done = false;
+    // The init already decided early-out; check it before enqueuing work.
+    PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    if (hs[0].done) {
+        c->timer.solve_iters.push_back(hs[0].iter);
+        c->timer.solve = -1;
+        return PANO_OK;
+    }
+    while (!done && launched < max_iterations) {
+        int k = every;
+        if (launched + k > max_iterations) k = max_iterations - launched;
+        for (int i = 0; i < k; ++i) PANO_CHECK(solver_iteration<T>(S));
+        launched += k;
+        int slot = chunk & 1;
+        PANO_HIP(hipMemcpyAsync(&hs[slot], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+        PANO_HIP(hipEventRecord(c->ev[slot], c->stream));
+        if (chunk > 0) {
+            int prev = slot ^ 1;
+            PANO_HIP(hipEventSynchronize(c->ev[prev]));
+            if (hs[prev].done) done = true;
+        }
+        ++chunk;
+    }
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    PANO_HIP(hipMemcpy(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost));
+    c->timer.solve_iters.push_back(hs[0].iter);
+    c->timer.solve = -1;
+    return PANO_OK;
+}
+
+static pano_status finish_info(Ctx* c, pano_pcg_info* info) {
+    SolverState& s = c->host_state[0];
+    if (info) {
+        info->iterations = s.iter;
+        info->converged = s.converged;
+        info->b_linf = s.b_linf;
+        info->r_linf = s.rnorm;
+        info->threshold = s.threshold;
+    }
+    if (s.nonfinite) return fail(PANO_ERR_NONFINITE, "non-finite value in the right-hand side or a CG scalar");
+    return s.converged ? PANO_OK : PANO_NOT_CONVERGED;
+}
+
+// Generic solve used by the C ABI: x, b natural layout; residual/auxiliary/
+// search caller scratch (natural for identity/Jacobi; used in red-black order
+// internally for multigrid, with x accumulated in a workspace copy).
+template <typename T>
+pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, const T* b, T* residual,
+                      T* auxiliary, T* search, pano_pcg_info* info, bool) {
+    Solver<T> S;
+    PANO_CHECK(solver_setup<T>(c, D, P, &S));
+    S.r = residual;
+    S.aux = auxiliary;
+    S.p = search;
+    Grid3 Gn = grid3(D.w, D.h, D.d);
+    if (S.rb) {
+        void* xr;
+        PANO_CHECK(c->workspace("pcg_x_rb", (size_t)S.n * sizeof(T), &xr));
+        S.x = (T*)xr;
+        L3((k_to_rb<T>), Gn, D, Gn, S.r, b);
+    } else {
+        S.x = x;
+        PANO_HIP(hipMemcpyAsync(S.r, b, (size_t)S.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    }
+    PANO_CHECK(solver_init<T>(&S, P->max_iterations));
+    PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every));
+    if (S.rb) {
+        L3((k_from_rb<T>), Gn, D, Gn, x, S.x);
+        PANO_HIP(hipStreamSynchronize(c->stream));
+    }
+    return finish_info(c, info);
+}
+
+template pano_status pcg_solve<float>(Ctx*, const Dims&, const pano_pcg_params*, float*, const float*, float*,
+                                      float*, float*, pano_pcg_info*, bool);
+template pano_status pcg_solve<double>(Ctx*, const Dims&, const pano_pcg_params*, double*, const double*,
+                                       double*, double*, double*, pano_pcg_info*, bool);
+
+// ---- the fused fluid step ----------------------------------------------------------------
+
+// b = -div u written straight into the solver's layout (natural or red-black).
+template <typename T, int DIM, int L_>
+__global__ __launch_bounds__(256) void k_neg_div_to(Dims D, Grid3 G, T* b, Faces<const T> v) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    b[lidx<L_>(D, z, y, x)] = -div_value<T, DIM>(D, v, z, y, x);
+}
+
+template <typename T>
+static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, const T* rho_in,
+                              const T* vel_in, T* rho_out, T* vel_out, T* pressure, pano_pcg_info* info) {
+    pano_pcg_params pp = P->pcg;
+    pp.dt = P->dt;
+    Solver<T> S;
+    PANO_CHECK(solver_setup<T>(c, D, &pp, &S));
+    void *r, *a, *s, *xv;
+    size_t nb = (size_t)S.n * sizeof(T);
+    PANO_CHECK(c->workspace("step_r", nb, &r));
+    PANO_CHECK(c->workspace("step_aux", nb, &a));
+    PANO_CHECK(c->workspace("step_p", nb, &s));
+    PANO_CHECK(c->workspace("step_x", nb, &xv));
+    S.r = (T*)r;
+    S.aux = (T*)a;
+    S.p = (T*)s;
+    S.x = (T*)xv;
+    double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
+    const double* sp = P->src_radius > 0 ? sphere : nullptr;
+    // source -> advection of density and velocity from the old velocity
+    PANO_CHECK(launch_advect<T>(c, D, rho_out, rho_in, P->dt, vel_in, sp));
+    double buoy_c = (double)(T)(P->dt * P->buoyancy * 0.5);
+    PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
+    // b = -div u
+    Faces<const T> v = split_faces(D, (const T*)vel_out);
+    Grid3 Gn = grid3(D.w, D.h, D.d);
+    if (S.rb) {
+        if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gn, D, Gn, S.r, v);
+        else L3((k_neg_div_to<T, 2, REDBLACK>), Gn, D, Gn, S.r, v);
+    } else {
+        if (D.d > 1) L3((k_neg_div_to<T, 3, NATURAL>), Gn, D, Gn, S.r, v);
+        else L3((k_neg_div_to<T, 2, NATURAL>), Gn, D, Gn, S.r, v);
+    }
+    PANO_HIP(hipGetLastError());
+    PANO_CHECK(solver_init<T>(&S, pp.max_iterations));
+    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every));
+    // u += dt d0_dual(p), closed walls zeroed
+    if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x));
+    else PANO_CHECK(launch_project<T>(c, D, P->dt, vel_out, S.x));
+    if (pressure) {
+        if (S.rb) L3((k_from_rb<T>), Gn, D, Gn, pressure, S.x);
+        else PANO_HIP(hipMemcpyAsync(pressure, S.x, nb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    return finish_info(c, info);
+}
+
+template <typename T>
+static pano_status precond_apply(Ctx* c, const Dims& D, const pano_pcg_params* P, T* dst, const T* src) {
+    Solver<T> S;
+    PANO_CHECK(solver_setup<T>(c, D, P, &S));
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, c->stream, S.st, 0);
+    size_t nb = (size_t)S.n * sizeof(T);
+    if (P->precond == PANO_PRECOND_IDENTITY) {
+        PANO_HIP(hipMemcpyAsync(dst, src, nb, hipMemcpyDeviceToDevice, c->stream));
+    } else if (P->precond == PANO_PRECOND_JACOBI) {
+        if (D.d > 1) L3((k_jacobi_dot<T, 3>), S.G, D, S.G, dst, src, S.dt, S.mask, S.part0, S.st);
+        else L3((k_jacobi_dot<T, 2>), S.G, D, S.G, dst, src, S.dt, S.mask, S.part0, S.st);
+    } else {
+        void *rr, *zr;
+        PANO_CHECK(c->workspace("papply_r", nb, &rr));
+        PANO_CHECK(c->workspace("papply_z", nb, &zr));
+        Grid3 Gn = grid3(D.w, D.h, D.d);
+        L3((k_to_rb<T>), Gn, D, Gn, (T*)rr, src);
+        PANO_CHECK(mg_vcycle<T>(c, &S.mg, (T*)zr, (const T*)rr, S.st));
+        L3((k_from_rb<T>), Gn, D, Gn, dst, (const T*)zr);
+    }
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+}  // namespace pano
+
+using namespace pano;
+
+extern "C" {
+
+pano_status pano_precond_conjugate_gradient(pano_ctx ctx, const pano_grid* g, const pano_pcg_params* P,
+                                            void* x, const void* b, void* residual, void* auxiliary,
+                                            void* search, pano_pcg_info* info) {
+    Ctx* c = ctx_of(ctx);
+    if (!c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    if (!P || !x || !b || !residual || !auxiliary || !search) return fail(PANO_ERR_INVALID, "null argument");
+    const void* ptrs[5] = {x, b, residual, auxiliary, search};
+    for (int i = 0; i < 5; ++i)
+        for (int j = i + 1; j < 5; ++j)
+            if (ptrs[i] == ptrs[j]) return fail(PANO_ERR_INVALID, "solver buffers must be distinct");
+    PANO_HIP(hipSetDevice(c->device));
+    Dims D = dims_of(g);
+    if (g->dtype == PANO_F64)
+        return pcg_solve<double>(c, D, P, (double*)x, (const double*)b, (double*)residual, (double*)auxiliary,
+                                 (double*)search, info, false);
+    return pcg_solve<float>(c, D, P, (float*)x, (const float*)b, (float*)residual, (float*)auxiliary,
+                            (float*)search, info, false);
+}
+
+pano_status pano_precond_apply(pano_ctx ctx, const pano_grid* g, const pano_pcg_params* P, void* dst,
+                               const void* src) {
+    Ctx* c = ctx_of(ctx);
+    if (!c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    if (!P || !dst || !src || dst == src) return fail(PANO_ERR_INVALID, "null or aliased argument");
+    PANO_HIP(hipSetDevice(c->device));
+    Dims D = dims_of(g);
+    if (g->dtype == PANO_F64) return precond_apply<double>(c, D, P, (double*)dst, (const double*)src);
+    return precond_apply<float>(c, D, P, (float*)dst, (const float*)src);
+}
+
+pano_status pano_fluid_step(pano_ctx ctx, const pano_grid* g, const pano_step_params* P, const void* density_in,
+                            const void* velocity_in, void* density_out, void* velocity_out, void* pressure,
+                            pano_pcg_info* info) {
+    Ctx* c = ctx_of(ctx);
+    if (!c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    if (!P || !density_in || !velocity_in || !density_out || !velocity_out)
+        return fail(PANO_ERR_INVALID, "null argument");
+    if (density_in == density_out || velocity_in == velocity_out || density_out == velocity_out)
+        return fail(PANO_ERR_INVALID, "step outputs must not alias its inputs");
+    PANO_HIP(hipSetDevice(c->device));
+    Dims D = dims_of(g);
+    if (g->dtype == PANO_F64)
+        return fluid_step<double>(c, D, P, (const double*)density_in, (const double*)velocity_in,
+                                  (double*)density_out, (double*)velocity_out, (double*)pressure, info);
+    return fluid_step<float>(c, D, P, (const float*)density_in, (const float*)velocity_in, (float*)density_out,
+                             (float*)velocity_out, (float*)pressure, info);
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+"""The fluid-step workloads: sizes, boundary conditions, solver settings and
+seeded inputs, named by what they are.
+
+Resolutions of details the workload statements leave open (stated once):
+- velocity modes: each face component gets its own n_modes modes, amplitude
+  1 per mode; integer wavevectors drawn uniformly from [-k, k]^D and
+  rejected unless 1 <= |k| <= k_max (SplitMix64 from the seed);
+- "up" is +z; the smoke source is a sphere of radius 0.08 N centred at
+  (z, y, x) = (0.1 N, N/2, N/2), reset to density 1 every step; buoyancy
+  w += dt * beta * (rho[z-1] + rho[z]) / 2 on z faces, from the density at
+  the start of the step;
+- the Poisson-only 512^3 problem has closed walls (the mean of b removed).
+"""
+import numpy as np
+
+from . import Grid2d, Grid3d, Jacobi, Multigrid, StepConfig, gen_bandlimited_faces, gen_gaussian_cells
+
+# 2D, closed box, Jacobi-PCG: 128^2 f32, 10 steps at dt = 0.5/128.
+DEC2D_128 = dict(
+    name="dec2d_128_jacobi", dims=(128, 128), steps=10, dt=0.5 / 128, seed=0, n_modes=16, k_max=8.0,
+    sigma_frac=0.1, bc=(0, 0, 0, 0, 0, 0), precond="jacobi", tol=1e-6,
+)
+
+# 3D smoke plume, open top, MG-PCG: 256^3 f32, 5 steps at dt = 0.5/256.
+SMOKE3D_256 = dict(
+    name="smoke3d_256_mgpcg", dims=(256, 256, 256), steps=5, dt=0.5 / 256, seed=0, n_modes=32, k_max=16.0,
+    bc=(0, 0, 0, 0, 0, 1), precond="mg", mg_levels=4, mg_nu=2, tol=1e-6, buoyancy=1.0,
+    src_radius_frac=0.08, src_z_frac=0.1,
+)
+
+
+def make_grid(wl, dims=None, dtype=np.float32, ctx=None):
+    dims = tuple(dims or wl["dims"])
+    if len(dims) == 2:
+        return Grid2d(dims, dtype, bc=wl["bc"], ctx=ctx)
+    return Grid3d(dims, dtype, bc=wl["bc"], ctx=ctx)
+
+
+def step_config(wl, dims=None, max_iterations=500):
+    dims = tuple(dims or wl["dims"])
+    if wl["precond"] == "mg":
+        pre = Multigrid(wl.get("mg_levels", 4), wl.get("mg_nu", 2))
+    else:
+        pre = Jacobi()
+    src_center, src_radius = (0.0, 0.0, 0.0), 0.0
+    if "src_radius_frac" in wl:
+        n = dims[-1]
+        src_radius = wl["src_radius_frac"] * n
+        src_center = (wl["src_z_frac"] * dims[0], dims[1] / 2.0, dims[2] / 2.0)
+    return StepConfig(dt=wl["dt"] if dims == tuple(wl["dims"]) else 0.5 / dims[-1], preconditioner=pre,
+                      tol=wl["tol"], tol_kind="rel", max_iterations=max_iterations,
+                      buoyancy=wl.get("buoyancy", 0.0), src_center=src_center, src_radius=src_radius)
+
+
+def initial_state(wl, grid):
+    """Host arrays (density, faces) of the workload's seeded initial state."""
+    faces = gen_bandlimited_faces(grid, wl["seed"], wl["n_modes"], wl["k_max"], 1.0)
+    if "sigma_frac" in wl:
+        h, w = grid.dim()[-2:]
+        center = (0.0, h / 2.0, w / 2.0)
+        rho = gen_gaussian_cells(grid, center, wl["sigma_frac"] * w)
+    else:
+        rho = np.zeros(grid.num_elem_2(), grid.dtype)
+    return rho, faces
