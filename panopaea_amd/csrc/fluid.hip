@@ -4,6 +4,8 @@
 // and the projection (dec_fluid.rs:124-141). All are HBM-bound gathers or
 // stencils; threads are tiled 64 (x, coalesced) x 4 (y) per z plane and the
 // workgroup order is remapped so each XCD sweeps a contiguous slab.
+#include <cmath>
+
 #include "fluid_device.hpp"
 
 namespace pano {
@@ -111,13 +113,18 @@ __global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const
 }
 
 static Sphere make_sphere(const double* s) {
-    Sphere S{0, 0, 0, 0, 0};
+    Sphere S{0, 0, 0, 0, 0, {0, 0, 0}, {-1, -1, -1}};
     if (s && s[3] > 0) {
         S.cz = s[0];
         S.cy = s[1];
         S.cx = s[2];
         S.r2 = s[3] * s[3];
         S.on = 1;
+        // cells whose centre c + 0.5 can lie within r of the centre, with a margin
+        for (int a = 0; a < 3; ++a) {
+            S.lo[a] = (int)std::floor(s[a] - s[3] - 0.5) - 1;
+            S.hi[a] = (int)std::ceil(s[a] + s[3] - 0.5) + 1;
+        }
     }
     return S;
 }
@@ -137,10 +144,12 @@ pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt
     Sphere S = make_sphere(src_sphere);
     const double cells = (double)D.d * D.h * D.w * sizeof(T);
     // read rho and every velocity component once, write rho'
-    PANO_TIMED(c, "advect", (D.d > 1 ? 5.0 : 4.0) * cells, {
-        if (D.d > 1) LAUNCH3((k_advect_cell<T, 3>), G, D, G, dst, src, (T)dt, v, S);
-        else LAUNCH3((k_advect_cell<T, 2>), G, D, G, dst, src, (T)dt, v, S);
-    });
+    const dim3 B(kTX, kTY);
+    if (D.d > 1)
+        klaunch(c, "advect", 5.0 * cells, k_advect_cell<T, 3>, dim3(G.nb), B, D, G, dst, src, (T)dt, v, S);
+    else
+        klaunch(c, "advect", 4.0 * cells, k_advect_cell<T, 2>, dim3(G.nb), B, D, G, dst, src, (T)dt, v, S);
+    PANO_HIP(hipGetLastError());
     return PANO_OK;
 }
 

@@ -86,8 +86,11 @@ __device__ __forceinline__ int64_t fx_idx(const Dims& D, int z, int y, int x) {
 struct Sphere {
     double cz, cy, cx, r2;
     int on;
+    int lo[3], hi[3];  // (z, y, x) cell bounding box of the sphere (inclusive)
 };
 __device__ __forceinline__ bool in_sphere(const Sphere& S, const Dims& D, int z, int y, int x) {
+    if ((D.d > 1 && (z < S.lo[0] || z > S.hi[0])) || y < S.lo[1] || y > S.hi[1] || x < S.lo[2] || x > S.hi[2])
+        return false;
     double dz = (double)z + 0.5 - S.cz;
     double dy = (double)y + 0.5 - S.cy;
     double dx = (double)x + 0.5 - S.cx;
@@ -275,44 +278,123 @@ __device__ __forceinline__ T div_value(const Dims& D, const Faces<const T>& v, i
 // X = p[-] - p[+], Y/Z = -(-(p[+] - p[-])) (derivative_0_dual then
 // hodge_1_dual), open sides using p = 0 in the ghost cell and closed walls
 // the untouched zero (+0 for x, -0 for y/z after the hodge sign flip).
-template <typename T, int DIM, int L>
-__device__ __forceinline__ T poisson_value(const Dims& D, const T* p, T dt, int z, int y, int x) {
+// Neighbour values a..g = x-, x+, y-, y+, z-, z+; `has` bit i set when
+// neighbour i exists (is inside the grid).
+template <typename T, int DIM>
+__device__ __forceinline__ T poisson_from(T pc, T a, T b, T c, T e, T f, T g, int has, int bc, T dt) {
     const T zero = (T)0;
-    T pc = p[lidx<L>(D, z, y, x)];
-    T xl, xr, yt, yb;
-    if (x > 0) xl = p[lidx<L>(D, z, y, x - 1)] - pc;
-    else xl = (D.bc & 1) ? zero - pc : zero;
-    if (x < D.w - 1) xr = pc - p[lidx<L>(D, z, y, x + 1)];
-    else xr = (D.bc & 2) ? pc - zero : zero;
-    if (y > 0) yt = -(-(pc - p[lidx<L>(D, z, y - 1, x)]));
-    else yt = (D.bc & 4) ? -(-(pc - zero)) : -zero;
-    if (y < D.h - 1) yb = -(-(p[lidx<L>(D, z, y + 1, x)] - pc));
-    else yb = (D.bc & 8) ? -(-(zero - pc)) : -zero;
+    T xl = (has & 1) ? a - pc : ((bc & 1) ? zero - pc : zero);
+    T xr = (has & 2) ? pc - b : ((bc & 2) ? pc - zero : zero);
+    T yt = (has & 4) ? -(-(pc - c)) : ((bc & 4) ? -(-(pc - zero)) : -zero);
+    T yb = (has & 8) ? -(-(e - pc)) : ((bc & 8) ? -(-(zero - pc)) : -zero);
     T q2 = ((-yb + yt) - xl) + xr;
     if constexpr (DIM == 3) {
-        T zt, zb;
-        if (z > 0) zt = -(-(pc - p[lidx<L>(D, z - 1, y, x)]));
-        else zt = (D.bc & 16) ? -(-(pc - zero)) : -zero;
-        if (z < D.d - 1) zb = -(-(p[lidx<L>(D, z + 1, y, x)] - pc));
-        else zb = (D.bc & 32) ? -(-(zero - pc)) : -zero;
+        T zt = (has & 16) ? -(-(pc - f)) : ((bc & 16) ? -(-(pc - zero)) : -zero);
+        T zb = (has & 32) ? -(-(g - pc)) : ((bc & 32) ? -(-(zero - pc)) : -zero);
         q2 = (-zb + zt) + q2;
     }
     return q2 * dt;
 }
 
-// Number of non-wall neighbours: the diagonal of the operator is dt*nc.
+// Existence bits of the six neighbours of (z, y, x).
+template <int DIM>
+__device__ __forceinline__ int nbr_bits(const Dims& D, int z, int y, int x) {
+    int has = (x > 0) | ((x < D.w - 1) << 1) | ((y > 0) << 2) | ((y < D.h - 1) << 3);
+    if constexpr (DIM == 3) has |= ((z > 0) << 4) | ((z < D.d - 1) << 5);
+    return has;
+}
+
+template <typename T, int DIM, int L>
+__device__ __forceinline__ T poisson_value(const Dims& D, const T* p, T dt, int z, int y, int x) {
+    const T zero = (T)0;
+    int has = nbr_bits<DIM>(D, z, y, x);
+    T pc = p[lidx<L>(D, z, y, x)];
+    // loads from clamped (valid) coordinates, then selects: no branches between loads
+    T a = p[lidx<L>(D, z, y, (has & 1) ? x - 1 : x)];
+    T b = p[lidx<L>(D, z, y, (has & 2) ? x + 1 : x)];
+    T c = p[lidx<L>(D, z, (has & 4) ? y - 1 : y, x)];
+    T e = p[lidx<L>(D, z, (has & 8) ? y + 1 : y, x)];
+    a = (has & 1) ? a : zero;
+    b = (has & 2) ? b : zero;
+    c = (has & 4) ? c : zero;
+    e = (has & 8) ? e : zero;
+    T f = zero, g = zero;
+    if constexpr (DIM == 3) {
+        f = p[lidx<L>(D, (has & 16) ? z - 1 : z, y, x)];
+        g = p[lidx<L>(D, (has & 32) ? z + 1 : z, y, x)];
+        f = (has & 16) ? f : zero;
+        g = (has & 32) ? g : zero;
+    }
+    return poisson_from<T, DIM>(pc, a, b, c, e, f, g, has, D.bc, dt);
+}
+
+// ---- red-black x-pair stencil ------------------------------------------------------------
+// A thread owning compact index m of row (z, y) holds the x-pair (2m, 2m+1):
+// cell x0 = 2m of colour c0 = (y+z)&1 at i0 = c0*n2 + row + m and cell
+// x1 = 2m+1 at i1 = (1-c0)*n2 + row + m. Neighbours of x0 lie in plane 1-c0
+// (x-1 at i1-1, x+1 at i1, y/z at i1 +- stride), neighbours of x1 in plane c0
+// (x-1 at i0, x+1 at i0+1, y/z at i0 +- stride). 32-bit offsets (cell counts
+// stay below 2^31).
+struct RBRow {
+    int i0, i1;     // indices of x0 and x1
+    int wh, hw;     // y and z strides in a colour plane
+    int has0, has1; // neighbour existence bits of x0 and x1
+};
+template <int DIM>
+__device__ __forceinline__ RBRow rb_row(const Dims& D, int z, int y, int m) {
+    RBRow R;
+    R.wh = D.w >> 1;
+    R.hw = D.h * R.wh;
+    int n2 = D.d * R.hw;
+    int row = z * R.hw + y * R.wh;
+    int c0 = (y + z) & 1;
+    R.i0 = c0 * n2 + row + m;
+    R.i1 = (1 - c0) * n2 + row + m;
+    int common = ((y > 0) << 2) | ((y < D.h - 1) << 3);
+    if constexpr (DIM == 3) common |= ((z > 0) << 4) | ((z < D.d - 1) << 5);
+    R.has0 = common | (m > 0) | 2;
+    R.has1 = common | 1 | ((m < R.wh - 1) << 1);
+    return R;
+}
+// Indices of the six neighbours (x-, x+, y-, y+, z-, z+) of x0 (k = 0) or
+// x1 (k = 1). A missing neighbour gets the cell's own opposite-colour base
+// index (always valid), so every load can be issued unconditionally and the
+// value discarded with a select: no branches between the loads.
+template <int DIM>
+__device__ __forceinline__ void rb_nbr_idx(const RBRow& R, int k, int* j) {
+    int o = k == 0 ? R.i1 : R.i0;  // the opposite-colour base
+    int has = k == 0 ? R.has0 : R.has1;
+    j[0] = (has & 1) ? (k == 0 ? o - 1 : o) : o;
+    j[1] = (has & 2) ? (k == 0 ? o : o + 1) : o;
+    j[2] = (has & 4) ? o - R.wh : o;
+    j[3] = (has & 8) ? o + R.wh : o;
+    j[4] = (DIM == 3 && (has & 16)) ? o - R.hw : o;
+    j[5] = (DIM == 3 && (has & 32)) ? o + R.hw : o;
+}
+// The six neighbour values of x0 or x1; missing ones read as zero.
+template <typename T, int DIM>
+__device__ __forceinline__ void rb_nbrs(const T* v, const RBRow& R, int k, T* n) {
+    int j[6];
+    rb_nbr_idx<DIM>(R, k, j);
+    int has = k == 0 ? R.has0 : R.has1;
+    T raw[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) raw[q] = (q < 4 || DIM == 3) ? v[j[q]] : (T)0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) n[q] = ((has >> q) & 1) ? raw[q] : (T)0;
+}
+
+// Number of non-wall neighbours from the existence bits: an existing
+// neighbour or an open side counts. The diagonal of the operator is dt*nc.
+template <int DIM>
+__device__ __forceinline__ int ncount_bits(int has, int bc) {
+    int m = has | bc;
+    if constexpr (DIM == 2) m &= 0xF;
+    return __popc(m & 0x3F);
+}
 template <int DIM>
 __device__ __forceinline__ int ncount(const Dims& D, int z, int y, int x) {
-    int nc = 0;
-    nc += (x > 0) ? 1 : (D.bc & 1);
-    nc += (x < D.w - 1) ? 1 : ((D.bc >> 1) & 1);
-    nc += (y > 0) ? 1 : ((D.bc >> 2) & 1);
-    nc += (y < D.h - 1) ? 1 : ((D.bc >> 3) & 1);
-    if constexpr (DIM == 3) {
-        nc += (z > 0) ? 1 : ((D.bc >> 4) & 1);
-        nc += (z < D.d - 1) ? 1 : ((D.bc >> 5) & 1);
-    }
-    return nc;
+    return ncount_bits<DIM>(nbr_bits<DIM>(D, z, y, x), D.bc);
 }
 
 // ---- projection (dec_fluid.rs:124-141) ----------------------------------------------------

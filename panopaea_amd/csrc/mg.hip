@@ -20,93 +20,112 @@
 
 namespace pano {
 
-template <typename T, int DIM, int L>
-__device__ __forceinline__ T nbr_sum(const Dims& D, const T* zv, int z, int y, int x) {
-    const T zero = (T)0;
-    T a = x > 0 ? zv[lidx<L>(D, z, y, x - 1)] : zero;
-    T b = x < D.w - 1 ? zv[lidx<L>(D, z, y, x + 1)] : zero;
-    T c = y > 0 ? zv[lidx<L>(D, z, y - 1, x)] : zero;
-    T e = y < D.h - 1 ? zv[lidx<L>(D, z, y + 1, x)] : zero;
-    T s = ((a + b) + c) + e;
-    if constexpr (DIM == 3) {
-        T f = z > 0 ? zv[lidx<L>(D, z - 1, y, x)] : zero;
-        T g = z < D.d - 1 ? zv[lidx<L>(D, z + 1, y, x)] : zero;
-        s = (s + f) + g;
-    }
+// Reciprocals 1/(s*nc) of the smoother's diagonal for nc = 0..6 (0 for nc = 0),
+// formed in double from the T-valued s and rounded once to T.
+template <typename T>
+struct RInv {
+    T v[8];
+};
+template <typename T>
+static RInv<T> make_rinv(T s) {
+    RInv<T> R;
+    for (int k = 0; k < 8; ++k) R.v[k] = k == 0 || k > 6 ? (T)0 : (T)(1.0 / ((double)s * (double)k));
+    return R;
+}
+
+// sum of the six neighbours in the fixed order ((x- + x+) + y-) + y+ (+ z-) + z+
+template <typename T, int DIM>
+__device__ __forceinline__ T nsum(const T* n) {
+    T s = ((n[0] + n[1]) + n[2]) + n[3];
+    if constexpr (DIM == 3) s = (s + n[4]) + n[5];
     return s;
 }
 
-// Red value of the first pre-smoothing half-sweep (z = 0 before it).
-template <typename T, int DIM>
-__device__ __forceinline__ T first_red(const Dims& D, const T* r, T s, int z, int y, int x) {
-    int nc = ncount<DIM>(D, z, y, x);
-    T zero = (T)0;
-    return nc ? (r[rb_index(D, z, y, x)] + s * zero) / (s * (T)nc) : zero;
+// Smoothed value z = (r + s*sum) * rinv[nc].
+template <typename T>
+__device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RInv<T>& ri) {
+    return nc ? (r + s * sum) * ri.v[nc] : (T)0;
 }
 
-// First full sweep from z = 0: red z = (r + s*0)/(s*nc); black uses those
-// red neighbours, recomputed in place from r (no read of z at all).
+// First full sweep from z = 0: red z = (r + s*0)*rinv[nc]; the black update
+// recomputes its six red neighbours from r (no read of z).
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void k_smooth_first(Dims D, Grid3 G, T* zv, const T* r, T s,
+__global__ __launch_bounds__(256) void k_smooth_first(Dims D, Grid3 G, T* zv, const T* r, T s, RInv<T> ri,
                                                       const SolverState* st) {
     if (st->done) return;
     int m, y, z, L;
     tile_xyz(G, m, y, z, L);
     if (m >= (D.w >> 1) || y >= D.h) return;
     const T zero = (T)0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        int x = 2 * m + k;
-        int64_t i = rb_index(D, z, y, x);
-        int nc = ncount<DIM>(D, z, y, x);
-        if (((x + y + z) & 1) == 0) {
-            zv[i] = nc ? (r[i] + s * zero) / (s * (T)nc) : zero;
-        } else {
-            T a = x > 0 ? first_red<T, DIM>(D, r, s, z, y, x - 1) : zero;
-            T b = x < D.w - 1 ? first_red<T, DIM>(D, r, s, z, y, x + 1) : zero;
-            T c = y > 0 ? first_red<T, DIM>(D, r, s, z, y - 1, x) : zero;
-            T e = y < D.h - 1 ? first_red<T, DIM>(D, r, s, z, y + 1, x) : zero;
-            T sum = ((a + b) + c) + e;
-            if constexpr (DIM == 3) {
-                T f = z > 0 ? first_red<T, DIM>(D, r, s, z - 1, y, x) : zero;
-                T g = z < D.d - 1 ? first_red<T, DIM>(D, r, s, z + 1, y, x) : zero;
-                sum = (sum + f) + g;
-            }
-            zv[i] = nc ? (r[i] + s * sum) / (s * (T)nc) : zero;
-        }
+    RBRow R = rb_row<DIM>(D, z, y, m);
+    int c0 = (y + z) & 1;
+    int kr = c0;          // the red cell of the pair: x0 when c0 == 0, else x1
+    int kb = 1 - c0;
+    int ir = kr == 0 ? R.i0 : R.i1, ib = kb == 0 ? R.i0 : R.i1;
+    int hr = kr == 0 ? R.has0 : R.has1, hb = kb == 0 ? R.has0 : R.has1;
+    zv[ir] = gs_value(r[ir], s, zero, ncount_bits<DIM>(hr, D.bc), ri);
+    // red neighbours of the black cell: their r values and their nc
+    int xb = 2 * m + kb;
+    T rn[6];
+    rb_nbrs<T, DIM>(r, R, kb, rn);
+    T n[6];
+    n[0] = (hb & 1) ? gs_value(rn[0], s, zero, ncount<DIM>(D, z, y, xb - 1), ri) : zero;
+    n[1] = (hb & 2) ? gs_value(rn[1], s, zero, ncount<DIM>(D, z, y, xb + 1), ri) : zero;
+    n[2] = (hb & 4) ? gs_value(rn[2], s, zero, ncount<DIM>(D, z, y - 1, xb), ri) : zero;
+    n[3] = (hb & 8) ? gs_value(rn[3], s, zero, ncount<DIM>(D, z, y + 1, xb), ri) : zero;
+    if constexpr (DIM == 3) {
+        n[4] = (hb & 16) ? gs_value(rn[4], s, zero, ncount<DIM>(D, z - 1, y, xb), ri) : zero;
+        n[5] = (hb & 32) ? gs_value(rn[5], s, zero, ncount<DIM>(D, z + 1, y, xb), ri) : zero;
+    } else {
+        n[4] = zero;
+        n[5] = zero;
     }
+    zv[ib] = gs_value(r[ib], s, nsum<T, DIM>(n), ncount_bits<DIM>(hb, D.bc), ri);
 }
 
-// One colour of a red-black Gauss-Seidel sweep.
+// One colour of a red-black Gauss-Seidel sweep. With part, also the
+// workgroup's partial sum of z.r over the updated cells.
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void k_smooth_half(Dims D, Grid3 G, T* zv, const T* r, T s,
-                                                     int color, const SolverState* st) {
+__global__ __launch_bounds__(256) void k_smooth_half(Dims D, Grid3 G, T* zv, const T* r, T s, RInv<T> ri,
+                                                     int color, double* part, const SolverState* st) {
     if (st->done) return;
     int m, y, z, L;
     tile_xyz(G, m, y, z, L);
-    if (m >= (D.w >> 1) || y >= D.h) return;
-    int x = 2 * m + ((color + y + z) & 1);
-    int64_t i = rb_index(D, z, y, x);
-    int nc = ncount<DIM>(D, z, y, x);
-    T sum = nbr_sum<T, DIM, REDBLACK>(D, zv, z, y, x);
-    zv[i] = nc ? (r[i] + s * sum) / (s * (T)nc) : (T)0;
+    double acc = 0.0;
+    if (m < (D.w >> 1) && y < D.h) {
+        RBRow R = rb_row<DIM>(D, z, y, m);
+        int k = (color + y + z) & 1;  // parity of the updated cell's x
+        int i = k == 0 ? R.i0 : R.i1;
+        int has = k == 0 ? R.has0 : R.has1;
+        T n[6];
+        rb_nbrs<T, DIM>(zv, R, k, n);
+        T rv = r[i];
+        T v = gs_value(rv, s, nsum<T, DIM>(n), ncount_bits<DIM>(has, D.bc), ri);
+        zv[i] = v;
+        acc = (double)v * (double)rv;
+    }
+    if (part) {  // uniform across the workgroup
+        double t = block_sum(acc);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
+    }
 }
 
 // res = r - s*(nc*z - sum)
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void k_mg_residual(Dims D, Grid3 G, T* res, const T* zv,
-                                                     const T* r, T s, const SolverState* st) {
+__global__ __launch_bounds__(256) void k_mg_residual(Dims D, Grid3 G, T* res, const T* zv, const T* r, T s,
+                                                     const SolverState* st) {
     if (st->done) return;
     int m, y, z, L;
     tile_xyz(G, m, y, z, L);
     if (m >= (D.w >> 1) || y >= D.h) return;
+    RBRow R = rb_row<DIM>(D, z, y, m);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        int x = 2 * m + k;
-        int64_t i = rb_index(D, z, y, x);
-        int nc = ncount<DIM>(D, z, y, x);
-        T sum = nbr_sum<T, DIM, REDBLACK>(D, zv, z, y, x);
-        T az = s * ((T)nc * zv[i] - sum);
+        int i = k == 0 ? R.i0 : R.i1;
+        int has = k == 0 ? R.has0 : R.has1;
+        T n[6];
+        rb_nbrs<T, DIM>(zv, R, k, n);
+        T az = s * ((T)ncount_bits<DIM>(has, D.bc) * zv[i] - nsum<T, DIM>(n));
         res[i] = r[i] - az;
     }
 }
@@ -134,11 +153,18 @@ __global__ __launch_bounds__(256) void k_restrict(Dims Df, Dims Dc, Grid3 G, T* 
     const T zero = (T)0;
     bool hmI = I > 0, hpI = I < Dc.w - 1;
     bool hmJ = J > 0, hpJ = J < Dc.h - 1;
+    const int wh = Df.w >> 1, hw = Df.h * wh, n2 = Df.d * hw;
+    // fine x = 2I-1, 2I, 2I+1, 2I+2 of row (z, y): colours 1-ce, ce, 1-ce, ce at
+    // compact I-1, I, I, I+1 with ce = (y+z)&1
     auto tx = [&](int z, int y) -> T {
-        T vm = hmI ? res[rb_index(Df, z, y, 2 * I - 1)] : zero;
-        T v0 = res[rb_index(Df, z, y, 2 * I)];
-        T v1 = res[rb_index(Df, z, y, 2 * I + 1)];
-        T vp = hpI ? res[rb_index(Df, z, y, 2 * I + 2)] : zero;
+        int row = z * hw + y * wh + I;
+        int ce = (y + z) & 1;
+        const T* pe = res + ce * n2 + row;
+        const T* po = res + (1 - ce) * n2 + row;
+        T vm = hmI ? po[-1] : zero;
+        T v0 = pe[0];
+        T v1 = po[0];
+        T vp = hpI ? pe[1] : zero;
         return r1(vm, v0, v1, vp, hmI, hpI);
     };
     auto ty = [&](int z) -> T {
@@ -171,7 +197,19 @@ __device__ __forceinline__ void p1(int i, int n, int& I0, int& I1, bool& edge) {
     I1 = edge ? I : J;
 }
 
-// z_f += P z_c (nested: x, then y, then z interpolation).
+// Coarse-level index in natural or red-black layout.
+template <int LC>
+__device__ __forceinline__ int cidx(const Dims& Dc, int K, int J, int I) {
+    if constexpr (LC == REDBLACK) {
+        int wh = Dc.w >> 1, hw = Dc.h * wh;
+        return ((I + J + K) & 1) * (Dc.d * hw) + K * hw + J * wh + (I >> 1);
+    } else {
+        return (K * Dc.h + J) * Dc.w + I;
+    }
+}
+
+// z_f += P z_c (nested: x, then y, then z interpolation) for the x-pair
+// (2m, 2m+1): coarse column m with m-1 (for 2m) and m+1 (for 2m+1).
 template <typename T, int DIM, int LC>
 __global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* zf, const T* zc,
                                                  const SolverState* st) {
@@ -190,23 +228,36 @@ __global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* z
         wz0 = ez ? one : w34;
         wz1 = ez ? zero : w14;
     }
+    // x weights: x0 = 2m pairs with m-1, x1 = 2m+1 with m+1 (reflected at the ends)
+    bool el = m == 0, er = m + 1 >= Dc.w;
+    int Il = el ? m : m - 1, Ir = er ? m : m + 1;
+    T wl0 = el ? one : w34, wl1 = el ? zero : w14;
+    T wr0 = er ? one : w34, wr1 = er ? zero : w14;
+    T px0[2][2], px1[2][2];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        int x = 2 * m + k;
-        int I0, I1;
-        bool ex;
-        p1(x, Dc.w, I0, I1, ex);
-        T wx0 = ex ? one : w34, wx1 = ex ? zero : w14;
-        auto PX = [&](int kk, int jj) -> T {
-            return wx0 * zc[lidx<LC>(Dc, kk, jj, I0)] + wx1 * zc[lidx<LC>(Dc, kk, jj, I1)];
-        };
-        auto PY = [&](int kk) -> T { return wy0 * PX(kk, J0) + wy1 * PX(kk, J1); };
-        T v;
-        if constexpr (DIM == 3) v = wz0 * PY(K0) + wz1 * PY(K1);
-        else v = PY(0);
-        int64_t i = rb_index(Df, z, y, x);
-        zf[i] = zf[i] + v;
+    for (int a = 0; a < (DIM == 3 ? 2 : 1); ++a) {
+        int kk = a == 0 ? K0 : K1;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            int jj = b == 0 ? J0 : J1;
+            T cm = zc[cidx<LC>(Dc, kk, jj, m)];
+            T cl = zc[cidx<LC>(Dc, kk, jj, Il)];
+            T cr = zc[cidx<LC>(Dc, kk, jj, Ir)];
+            px0[a][b] = wl0 * cm + wl1 * cl;
+            px1[a][b] = wr0 * cm + wr1 * cr;
+        }
     }
+    T v0, v1;
+    if constexpr (DIM == 3) {
+        v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
+        v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
+    } else {
+        v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
+        v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
+    }
+    RBRow R = rb_row<DIM>(Df, z, y, m);
+    zf[R.i0] = zf[R.i0] + v0;
+    zf[R.i1] = zf[R.i1] + v1;
 }
 
 // ---- coarsest level: fast diagonalisation in FP64 (natural layout) ---------------------
@@ -380,49 +431,164 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
     return PANO_OK;
 }
 
-#define L3(kern, G, ...)                                                                         \
-    do {                                                                                          \
-        if ((G).nb > 0)                                                                           \
-            hipLaunchKernelGGL(kern, dim3((G).nb), dim3(kTX, kTY), 0, c->stream, __VA_ARGS__);   \
-    } while (0)
+// ---- coarsest level in LDS: whole planes of at most kFdmPlane cells -------------------------
+constexpr int kFdmPlane = 4096;  // 32 KiB of doubles per buffer
+
+// Transform the plane in `a` (ny rows of nx) along x (axis 0) or y (axis 1),
+// forward (coefficients = V^T v) or inverse (values = V c), into `b`.
+__device__ __forceinline__ void lds_axis(const double* a, double* b, int nx, int ny, int axis, int inverse,
+                                         const double* V) {
+    int P = nx * ny;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int j = e / nx, i = e - j * nx;
+        double acc = 0.0;
+        if (axis == 0) {
+            if (inverse)
+                for (int k = 0; k < nx; ++k) acc += V[i * nx + k] * a[j * nx + k];
+            else
+                for (int q = 0; q < nx; ++q) acc += V[q * nx + i] * a[j * nx + q];
+        } else {
+            if (inverse)
+                for (int k = 0; k < ny; ++k) acc += V[j * ny + k] * a[k * nx + i];
+            else
+                for (int q = 0; q < ny; ++q) acc += V[q * ny + j] * a[q * nx + i];
+        }
+        b[e] = acc;
+    }
+}
+
+// One z-plane (blockIdx.x) of the coarsest level:
+//  mode 0: forward x then y (input T, output double)
+//  mode 1: inverse y then x (input double, output T)
+//  mode 2: the whole 2D solve: forward x, y, divide by s*lambda, inverse y, x
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(256) void k_fdm_plane(int nx, int ny, int mode, const Tin* in, Tout* out,
+                                                   const double* Vx, const double* Vy, const double* lx,
+                                                   const double* ly, double s, double eps,
+                                                   const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    double* a = fdm_lds;
+    double* b = fdm_lds + nx * ny;
+    int P = nx * ny;
+    const Tin* src = in + (int64_t)blockIdx.x * P;
+    Tout* dst = out + (int64_t)blockIdx.x * P;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = (double)src[e];
+    __syncthreads();
+    if (mode == 1) {
+        lds_axis(a, b, nx, ny, 1, 1, Vy);
+        __syncthreads();
+        lds_axis(b, a, nx, ny, 0, 1, Vx);
+    } else {
+        lds_axis(a, b, nx, ny, 0, 0, Vx);
+        __syncthreads();
+        lds_axis(b, a, nx, ny, 1, 0, Vy);
+        if (mode == 2) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < P; e += blockDim.x) {
+                int j = e / nx, i = e - j * nx;
+                double lam = lx[i] + ly[j];
+                a[e] = lam > eps ? a[e] / (s * lam) : 0.0;
+            }
+            __syncthreads();
+            lds_axis(a, b, nx, ny, 1, 1, Vy);
+            __syncthreads();
+            lds_axis(b, a, nx, ny, 0, 1, Vx);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) dst[e] = (Tout)a[e];
+}
+
+// One y-row (blockIdx.x) of the 3D coarsest level: forward z, divide by
+// s*(lx+ly+lz), inverse z, in place on the (nz x nx) slice.
+__global__ __launch_bounds__(256) void k_fdm_zcol(int nx, int ny, int nz, double* data, const double* Vz,
+                                                  const double* lx, const double* ly, const double* lz, double s,
+                                                  double eps, const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    double* a = fdm_lds;
+    double* b = fdm_lds + nx * nz;
+    int j = blockIdx.x;
+    int P = nx * nz;
+    int64_t zs = (int64_t)nx * ny;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        a[e] = data[k * zs + (int64_t)j * nx + i];
+    }
+    __syncthreads();
+    // slice rows are z: the "y" axis of lds_axis
+    lds_axis(a, b, nx, nz, 1, 0, Vz);
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        double lam = lx[i] + ly[j] + lz[k];
+        b[e] = lam > eps ? b[e] / (s * lam) : 0.0;
+    }
+    __syncthreads();
+    lds_axis(b, a, nx, nz, 1, 1, Vz);
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        data[k * zs + (int64_t)j * nx + i] = a[e];
+    }
+}
 
 template <typename T, int DIM>
 static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const SolverState* st) {
     const Dims& C = m->dims[m->levels - 1];
     int nx = C.w, ny = C.h, nz = C.d;
     int64_t n = (int64_t)nx * ny * nz;
-    unsigned nb = blocks_for(n, kBlock);
     double s = m->s[m->levels - 1];
     double* t0 = m->fdm_tmp[0];
     double* t1 = m->fdm_tmp[1];
     const double eps = 1e-10;
     hipStream_t S = c->stream;
-    // forward x, y (, z) ; scale on the last forward axis ; inverse in reverse order
-    hipLaunchKernelGGL((k_fdm_axis<T, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 0, 0, r, t0,
-                       m->fdm_v[0], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
-    if (DIM == 3) {
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 0, t0, t1,
-                           m->fdm_v[1], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 2, 0, t1, t0,
-                           m->fdm_v[2], 1, m->fdm_lam[0], m->fdm_lam[1], m->fdm_lam[2], s, eps, st);
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 2, 1, t0, t1,
-                           m->fdm_v[2], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 1, t1, t0,
-                           m->fdm_v[1], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
-    } else {
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 0, t0, t1,
-                           m->fdm_v[1], 1, m->fdm_lam[0], m->fdm_lam[1], nullptr, s, eps, st);
-        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 1, t1, t0,
-                           m->fdm_v[1], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+    const double *Vx = m->fdm_v[0], *Vy = m->fdm_v[1], *Vz = m->fdm_v[2];
+    const double *lx = m->fdm_lam[0], *ly = m->fdm_lam[1], *lz = m->fdm_lam[2];
+    if (nx * ny <= kFdmPlane && (DIM == 2 || nx * nz <= kFdmPlane)) {
+        size_t lds = 2 * sizeof(double) * (size_t)kFdmPlane;
+        if (DIM == 2) {
+            hipLaunchKernelGGL((k_fdm_plane<T, T>), dim3(1), dim3(256), lds, S, nx, ny, 2, r, z, Vx, Vy, lx, ly, s,
+                               eps, st);
+        } else {
+            hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, Vy,
+                               (const double*)nullptr, (const double*)nullptr, s, eps, st);
+            hipLaunchKernelGGL(k_fdm_zcol, dim3(ny), dim3(256), lds, S, nx, ny, nz, t0, Vz, lx, ly, lz, s, eps, st);
+            hipLaunchKernelGGL((k_fdm_plane<double, T>), dim3(nz), dim3(256), lds, S, nx, ny, 1, t0, z, Vx, Vy,
+                               (const double*)nullptr, (const double*)nullptr, s, eps, st);
+        }
+        PANO_HIP(hipGetLastError());
+        return PANO_OK;
     }
-    hipLaunchKernelGGL((k_fdm_axis<double, T>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 0, 1, t0, z,
-                       m->fdm_v[0], 0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+    // large coarsest planes: one global-memory pass per axis
+    unsigned nb = blocks_for(n, kBlock);
+    hipLaunchKernelGGL((k_fdm_axis<T, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 0, 0, r, t0, Vx, 0,
+                       nullptr, nullptr, nullptr, 0.0, 0.0, st);
+    if (DIM == 3) {
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 0, t0, t1, Vy,
+                           0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 2, 0, t1, t0, Vz,
+                           1, lx, ly, lz, s, eps, st);
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 2, 1, t0, t1, Vz,
+                           0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 1, t1, t0, Vy,
+                           0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+    } else {
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 0, t0, t1, Vy,
+                           1, lx, ly, nullptr, s, eps, st);
+        hipLaunchKernelGGL((k_fdm_axis<double, double>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 1, 1, t1, t0, Vy,
+                           0, nullptr, nullptr, nullptr, 0.0, 0.0, st);
+    }
+    hipLaunchKernelGGL((k_fdm_axis<double, T>), dim3(nb), dim3(kBlock), 0, S, nx, ny, nz, 0, 1, t0, z, Vx, 0,
+                       nullptr, nullptr, nullptr, 0.0, 0.0, st);
     PANO_HIP(hipGetLastError());
     return PANO_OK;
 }
 
 template <typename T, int DIM>
-static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st) {
+static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st, double* dot_part,
+                              int* dot_np) {
     int Lc = m->levels - 1;
     T* zl[16];
     const T* rl[16];
@@ -432,44 +598,59 @@ static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const Solver
         zl[l] = m->z[l];
         rl[l] = m->r[l];
     }
+    const dim3 B(kTX, kTY);
+    const double cfrac = DIM == 3 ? 0.125 : 0.25;
     for (int l = 0; l < Lc; ++l) {
         const Dims& D = m->dims[l];
         const Dims& Dc = m->dims[l + 1];
         T s = (T)m->s[l];
+        RInv<T> ri = make_rinv<T>(s);
         Grid3 G = grid3(D.w >> 1, D.h, D.d);
         const double unit = (double)D.d * D.h * D.w * sizeof(T);
-        const char* nf = l == 0 ? "smooth_first_l0" : "";
         const char* nh = l == 0 ? "smooth_l0" : "";
-        PANO_TIMED(c, nf, 2 * unit, L3((k_smooth_first<T, DIM>), G, D, G, zl[l], rl[l], s, st));
+        klaunch(c, l == 0 ? "smooth_first_l0" : "", 2 * unit, k_smooth_first<T, DIM>, dim3(G.nb), B, D, G, zl[l],
+                rl[l], s, ri, st);
         for (int k = 1; k < m->nu; ++k) {
-            PANO_TIMED(c, nh, 1.5 * unit, L3((k_smooth_half<T, DIM>), G, D, G, zl[l], rl[l], s, 0, st));
-            PANO_TIMED(c, nh, 1.5 * unit, L3((k_smooth_half<T, DIM>), G, D, G, zl[l], rl[l], s, 1, st));
+            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0,
+                    (double*)nullptr, st);
+            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
+                    (double*)nullptr, st);
         }
-        PANO_TIMED(c, l == 0 ? "residual_l0" : "", 3 * unit,
-                   L3((k_mg_residual<T, DIM>), G, D, G, m->res, zl[l], rl[l], s, st));
+        klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM>, dim3(G.nb), B, D, G, m->res, zl[l],
+                rl[l], s, st);
         Grid3 Gc = grid3(Dc.w, Dc.h, Dc.d);
-        double cfrac = DIM == 3 ? 0.125 : 0.25;
-        PANO_TIMED(c, l == 0 ? "restrict_l0" : "", (1 + cfrac) * unit, {
-            if (l + 1 == Lc) L3((k_restrict<T, DIM, NATURAL>), Gc, D, Dc, Gc, m->r[l + 1], m->res, st);
-            else L3((k_restrict<T, DIM, REDBLACK>), Gc, D, Dc, Gc, m->r[l + 1], m->res, st);
-        });
+        const char* nr = l == 0 ? "restrict_l0" : "";
+        if (l + 1 == Lc)
+            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, NATURAL>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                    (const T*)m->res, st);
+        else
+            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, REDBLACK>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                    (const T*)m->res, st);
     }
     PANO_CHECK(coarse_solve<T, DIM>(c, m, m->z[Lc], m->r[Lc], st));
     for (int l = Lc - 1; l >= 0; --l) {
         const Dims& D = m->dims[l];
         const Dims& Dc = m->dims[l + 1];
         T s = (T)m->s[l];
+        RInv<T> ri = make_rinv<T>(s);
         Grid3 G = grid3(D.w >> 1, D.h, D.d);
         const double unit = (double)D.d * D.h * D.w * sizeof(T);
-        double cfrac = DIM == 3 ? 0.125 : 0.25;
         const char* nh = l == 0 ? "smooth_l0" : "";
-        PANO_TIMED(c, l == 0 ? "prolong_l0" : "", (2 + cfrac) * unit, {
-            if (l + 1 == Lc) L3((k_prolong<T, DIM, NATURAL>), G, D, Dc, G, zl[l], m->z[l + 1], st);
-            else L3((k_prolong<T, DIM, REDBLACK>), G, D, Dc, G, zl[l], m->z[l + 1], st);
-        });
+        const char* np = l == 0 ? "prolong_l0" : "";
+        if (l + 1 == Lc)
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL>, dim3(G.nb), B, D, Dc, G, zl[l],
+                    (const T*)m->z[l + 1], st);
+        else
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, REDBLACK>, dim3(G.nb), B, D, Dc, G, zl[l],
+                    (const T*)m->z[l + 1], st);
         for (int k = 0; k < m->nu; ++k) {
-            PANO_TIMED(c, nh, 1.5 * unit, L3((k_smooth_half<T, DIM>), G, D, G, zl[l], rl[l], s, 1, st));
-            PANO_TIMED(c, nh, 1.5 * unit, L3((k_smooth_half<T, DIM>), G, D, G, zl[l], rl[l], s, 0, st));
+            // the final (black, red) pair at level 0 also yields the partials of z.r
+            bool last = l == 0 && k == m->nu - 1 && dot_part;
+            double* pb = last ? dot_part : nullptr;
+            double* pr = last ? dot_part + G.nb : nullptr;
+            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1, pb, st);
+            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0, pr, st);
+            if (last) *dot_np = 2 * G.nb;
         }
     }
     PANO_HIP(hipGetLastError());
@@ -477,14 +658,16 @@ static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const Solver
 }
 
 template <typename T>
-pano_status mg_vcycle(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st) {
-    if (m->dims[0].d > 1) return vcycle_dim<T, 3>(c, m, z0, r0, st);
-    return vcycle_dim<T, 2>(c, m, z0, r0, st);
+pano_status mg_vcycle(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st, double* dot_part,
+                      int* dot_np) {
+    if (m->dims[0].d > 1) return vcycle_dim<T, 3>(c, m, z0, r0, st, dot_part, dot_np);
+    return vcycle_dim<T, 2>(c, m, z0, r0, st, dot_part, dot_np);
 }
 
 template pano_status mg_setup<float>(Ctx*, const Dims&, int, int, double, MG<float>*);
 template pano_status mg_setup<double>(Ctx*, const Dims&, int, int, double, MG<double>*);
-template pano_status mg_vcycle<float>(Ctx*, MG<float>*, float*, const float*, const SolverState*);
-template pano_status mg_vcycle<double>(Ctx*, MG<double>*, double*, const double*, const SolverState*);
+template pano_status mg_vcycle<float>(Ctx*, MG<float>*, float*, const float*, const SolverState*, double*, int*);
+template pano_status mg_vcycle<double>(Ctx*, MG<double>*, double*, const double*, const SolverState*, double*,
+                                        int*);
 
 }  // namespace pano

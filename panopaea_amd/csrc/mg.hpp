@@ -31,7 +31,10 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
 
 // z0 = M r0 (one V-cycle from zero), both in red-black layout at level 0.
 // Every kernel returns at once when st->done is set.
+// When dot_part is not null, the last two level-0 half-sweeps also write the
+// partial sums of z.r per workgroup into dot_part (*dot_np entries).
 template <typename T>
-pano_status mg_vcycle(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st);
+pano_status mg_vcycle(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st, double* dot_part,
+                      int* dot_np);
 
 }  // namespace pano

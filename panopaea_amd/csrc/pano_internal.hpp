@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP translation units of libpano.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -76,6 +77,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     std::map<std::string, Buffer> pool;  // named workspace buffers
     std::map<std::string, void*> consts; // immutable device tables (filled once)
+    std::map<std::string, int> iter_hint; // CG iterations of the last solve per problem
     SolverState* host_state = nullptr;   // pinned mirror of the device state
     hipEvent_t ev[2] = {nullptr, nullptr};
     int num_cu = 256;
@@ -86,19 +88,21 @@ struct Ctx {
 
 Ctx* ctx_of(pano_ctx c);
 bool timer_on(Ctx* c, const char* name);
-void timer_mark(Ctx* c);  // records one event (call before and after the launch)
+// Returns the start/stop event pair for the next timed launch (tagged for the
+// no-op filter) and records the kernel's algorithmic bytes per launch.
+bool timer_pair(Ctx* c, double bytes, hipEvent_t* e0, hipEvent_t* e1);
 
-// Wraps a launch statement with timer events when `name` is being timed.
-#define PANO_TIMED(c, name, bytes_, ...)             \
-    do {                                             \
-        bool t_ = ::pano::timer_on((c), (name));      \
-        if (t_) {                                    \
-            (c)->timer.bytes = (bytes_);             \
-            ::pano::timer_mark(c);                    \
-        }                                            \
-        __VA_ARGS__;                                 \
-        if (t_) ::pano::timer_mark(c);                \
-    } while (0)
+// Launches `kernel` on the context's stream. When `tname` is the kernel being
+// timed, the launch carries start/stop events (hipExtLaunchKernelGGL): they
+// take the dispatch's own begin/end timestamps, with no extra packet between.
+template <typename F, typename... Args>
+inline void klaunch(Ctx* c, const char* tname, double bytes, F kernel, dim3 grid, dim3 block, Args... args) {
+    hipEvent_t e0, e1;
+    if (tname && tname[0] && timer_on(c, tname) && timer_pair(c, bytes, &e0, &e1))
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, args...);
+}
 pano_status check_grid(const pano_grid* g);
 size_t dtype_size(int32_t dtype);
 

@@ -34,11 +34,20 @@ struct SolveCfg {
 __global__ __launch_bounds__(1024) void k_finalize(int op, const double* part, int np, SolverState* st,
                                                    SolveCfg cfg, int use_max) {
     if (op >= FIN_SIGMA0 && st->done) return;  // uniform: every thread returns
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-        double v = part[i];
-        acc = use_max ? absmax_acc(acc, v) : acc + v;
+    // eight independent accumulators per thread (loads in flight), combined in a fixed order
+    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int stride = blockDim.x * 8;
+    for (int base = threadIdx.x; base < np; base += stride) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int i = base + u * blockDim.x;
+            double v = i < np ? part[i] : 0.0;
+            a8[u] = use_max ? absmax_acc(a8[u], v) : a8[u] + v;
+        }
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = use_max ? absmax_acc(acc, a8[u]) : acc + a8[u];
     double tot = use_max ? block_max(acc) : block_sum(acc);
     if (threadIdx.x != 0) return;
     switch (op) {
@@ -279,22 +288,46 @@ __global__ __launch_bounds__(256) void k_apply_dot_nat(Dims D, Grid3 G, T* q, co
     if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
 }
 
-// q = A p with partial p.q ; red-black layout (thread = x-pair).
-template <typename T, int DIM>
-__global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T dt,
-                                                      double* part, const SolverState* st) {
+// q = A p with partial p.q ; red-black layout (thread = x-pair). With FUSE
+// the search update of the previous iteration is folded in: every p value is
+// formed as (T)(z + beta*p_old) (the same rounding as a separate
+// p = z + beta p pass), the pair's own values are stored to pnew (a second
+// buffer: neighbours still read p_old) and q = A pnew.
+template <typename T, int DIM, bool FUSE>
+__global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T* pnew, const T* zv,
+                                                      T dt, double* part, const SolverState* st) {
     if (st->done) return;
     int m, y, z, L;
     tile_xyz(G, m, y, z, L);
     double s = 0.0;
+    const double beta = FUSE ? st->beta : 0.0;
+    auto P = [&](int j) -> T {
+        if constexpr (FUSE) return (T)((double)zv[j] + beta * (double)p[j]);
+        else return p[j];
+    };
     if (m < (D.w >> 1) && y < D.h) {
+        RBRow R = rb_row<DIM>(D, z, y, m);
+        const T zero = (T)0;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            int x = 2 * m + k;
-            T v = poisson_value<T, DIM, REDBLACK>(D, p, dt, z, y, x);
-            int64_t i = rb_index(D, z, y, x);
+            int i = k == 0 ? R.i0 : R.i1;
+            int has = k == 0 ? R.has0 : R.has1;
+            int j[6];
+            rb_nbr_idx<DIM>(R, k, j);
+            T raw[6];
+#pragma unroll
+            for (int qq = 0; qq < 6; ++qq) raw[qq] = (qq < 4 || DIM == 3) ? P(j[qq]) : zero;
+            T a = (has & 1) ? raw[0] : zero;
+            T b = (has & 2) ? raw[1] : zero;
+            T c = (has & 4) ? raw[2] : zero;
+            T e = (has & 8) ? raw[3] : zero;
+            T f = (DIM == 3 && (has & 16)) ? raw[4] : zero;
+            T g = (DIM == 3 && (has & 32)) ? raw[5] : zero;
+            T pc = P(i);
+            if constexpr (FUSE) pnew[i] = pc;
+            T v = poisson_from<T, DIM>(pc, a, b, c, e, f, g, has, D.bc, dt);
             q[i] = v;
-            s += (double)p[i] * (double)v;
+            s += (double)pc * (double)v;
         }
     }
     double t = block_sum(s);
@@ -370,8 +403,9 @@ struct Solver {
     Grid3 G;
     MG<T> mg;
     SolveCfg cfg;
-    // vectors (layout per rb)
-    T *x, *r, *aux, *p;
+    // vectors (layout per rb); q aliases aux except on the fused red-black path
+    T *x, *r, *aux, *p, *q, *p2;
+    int it;  // iterations enqueued in this solve
 };
 
 template <typename T>
@@ -397,7 +431,7 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     S->nflat = (int)blocks_for(S->n, kFlatPer);
     S->G = S->rb ? grid3(D.w >> 1, D.h, D.d) : grid3(D.w, D.h, D.d);
     S->ntile = S->G.nb;
-    int np = S->nflat > S->ntile ? S->nflat : S->ntile;
+    int np = S->nflat > 2 * S->ntile ? S->nflat : 2 * S->ntile;
     void *st, *p0, *p1;
     PANO_CHECK(c->workspace("pcg_state", sizeof(SolverState), &st));
     PANO_CHECK(c->workspace("pcg_part0", (size_t)np * sizeof(double), &p0));
@@ -428,11 +462,11 @@ static pano_status precond_dot(Solver<T>* S, T* z, const T* r, double* part, int
         if (D.d > 1) L3((k_jacobi_dot<T, 3>), S->G, D, S->G, z, r, S->dt, S->mask, part, S->st);
         else L3((k_jacobi_dot<T, 2>), S->G, D, S->G, z, r, S->dt, S->mask, part, S->st);
         *np = S->ntile;
+    } else if (S->precond == PANO_PRECOND_MULTIGRID) {
+        PANO_CHECK(mg_vcycle<T>(c, &S->mg, z, r, S->st, part, np));
     } else {
-        if (S->precond == PANO_PRECOND_MULTIGRID) PANO_CHECK(mg_vcycle<T>(c, &S->mg, z, r, S->st));
-        PANO_TIMED(c, "dot", 2.0 * S->n * sizeof(T),
-                   hipLaunchKernelGGL((k_dot<T>), dim3(S->nflat), dim3(kBlock), 0, c->stream, S->n, z, r, part,
-                                      S->st));
+        klaunch(c, "dot", 2.0 * S->n * sizeof(T), k_dot<T>, dim3(S->nflat), dim3(kBlock), S->n, (const T*)z, r,
+                part, (const SolverState*)S->st);
         *np = S->nflat;
     }
     return PANO_OK;
@@ -445,6 +479,8 @@ static pano_status solver_init(Solver<T>* S, int max_iterations) {
     hipStream_t st = c->stream;
     c->timer.solve = (int)c->timer.solve_iters.size();
     c->timer.checks = 0;
+    S->it = 0;
+    if (!S->q) S->q = S->aux;
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, st, S->st, max_iterations);
     hipLaunchKernelGGL((k_zero<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x);  // pcg.rs:32
     hipLaunchKernelGGL((k_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->part0, S->part1);
@@ -476,24 +512,45 @@ template <typename T>
 static pano_status solver_iteration(Solver<T>* S) {
     Ctx* c = S->c;
     const Dims& D = S->D;
-    hipStream_t st = c->stream;
-    // q = A p ; alpha   (pcg.rs:51-53)  -- q lives in aux
     const double unit = (double)S->n * sizeof(T);  // one field pass
-    PANO_TIMED(c, "apply_dot", 2 * unit, {
-        if (S->rb) {
-            if (D.d > 1) L3((k_apply_dot_rb<T, 3>), S->G, D, S->G, S->aux, S->p, S->dt, S->part0, S->st);
-            else L3((k_apply_dot_rb<T, 2>), S->G, D, S->G, S->aux, S->p, S->dt, S->part0, S->st);
+    const dim3 B(kTX, kTY), Gd(S->G.nb);
+    const SolverState* cst = S->st;
+    // Folding p = z + beta p into q = A p (FUSE) doubles the loads per stencil
+    // point; measured slower on gfx950 than the two passes, so it is off.
+    const bool fuse = false;
+    // q = A p ; alpha   (pcg.rs:51-53)
+    if (S->rb) {
+        if (fuse) {
+            if (D.d > 1)
+                klaunch(c, "apply_dot", 4 * unit, k_apply_dot_rb<T, 3, true>, Gd, B, D, S->G, S->q, (const T*)S->p,
+                        S->p2, (const T*)S->aux, S->dt, S->part0, cst);
+            else
+                klaunch(c, "apply_dot", 4 * unit, k_apply_dot_rb<T, 2, true>, Gd, B, D, S->G, S->q, (const T*)S->p,
+                        S->p2, (const T*)S->aux, S->dt, S->part0, cst);
+            T* t = S->p;
+            S->p = S->p2;
+            S->p2 = t;
         } else {
-            if (D.d > 1) L3((k_apply_dot_nat<T, 3>), S->G, D, S->G, S->aux, S->p, S->dt, S->mask, S->part0, S->st);
-            else L3((k_apply_dot_nat<T, 2>), S->G, D, S->G, S->aux, S->p, S->dt, S->mask, S->part0, S->st);
+            if (D.d > 1)
+                klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, false>, Gd, B, D, S->G, S->q, (const T*)S->p,
+                        (T*)nullptr, (const T*)nullptr, S->dt, S->part0, cst);
+            else
+                klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2, false>, Gd, B, D, S->G, S->q, (const T*)S->p,
+                        (T*)nullptr, (const T*)nullptr, S->dt, S->part0, cst);
         }
-    });
+    } else {
+        if (D.d > 1)
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_nat<T, 3>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->mask, S->part0, cst);
+        else
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_nat<T, 2>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->mask, S->part0, cst);
+    }
     fin(S, FIN_ALPHA, S->part0, S->ntile, 0);
     // x += alpha p ; r -= alpha q ; ||r||_inf   (pcg.rs:55-63)
     bool ident = S->precond == PANO_PRECOND_IDENTITY;
-    PANO_TIMED(c, "update", 6 * unit,
-               hipLaunchKernelGGL((k_update<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x, S->r, S->p,
-                                  S->aux, S->part1, ident ? S->part0 : nullptr, S->st));
+    klaunch(c, "update", 6 * unit, k_update<T>, dim3(S->nflat), dim3(kBlock), S->n, S->x, S->r, (const T*)S->p,
+            (const T*)S->q, S->part1, ident ? S->part0 : (double*)nullptr, cst);
     fin(S, FIN_RNORM, S->part1, S->nflat, 1);
     // z = M r ; sigma' = z.r ; beta   (pcg.rs:65-68)
     T* z;
@@ -507,49 +564,70 @@ static pano_status solver_iteration(Solver<T>* S) {
         fin(S, FIN_BETA, S->part0, np, 0);
     }
     // p = z + beta p   (pcg.rs:72-77)
-    PANO_TIMED(c, "search", 3 * unit,
-               hipLaunchKernelGGL((k_search<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, z, S->st));
+    if (!fuse)
+        klaunch(c, "search", 3 * unit, k_search<T>, dim3(S->nflat), dim3(kBlock), S->n, S->p, (const T*)z, cst);
+    S->it += 1;
     PANO_HIP(hipGetLastError());
     return PANO_OK;
 }
 
-// Runs the loop with the state polled every `every` iterations, two chunks in
-// flight. Leaves the final state in c->host_state[0].
+// Runs the CG loop. Every loop kernel returns at once after convergence, but
+// a launch past convergence still dispatches all its workgroups, so the loop
+// avoids them: with check_every == 0 it enqueues as many iterations as the
+// previous solve of the same problem needed (a hint kept in the context; 4
+// without one), then polls the state after each further iteration. With
+// check_every > 0 it enqueues chunks of that size, two in flight. Leaves the
+// final state in c->host_state[0]. The result does not depend on the hint.
 template <typename T>
-static pano_status solver_loop(Solver<T>* S, int max_iterations, int every) {
+static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string& key) {
     Ctx* c = S->c;
     SolverState* hs = c->host_state;
-    if (every <= 0) every = S->n >= (1 << 20) ? 4 : 8;
-    int launched = 0, chunk = 0;
-    bool done = false;
-    // The init already decided early-out; check it before enqueuing work.
-    PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-    PANO_HIP(hipStreamSynchronize(c->stream));
-    if (hs[0].done) {
-        c->timer.solve_iters.push_back(hs[0].iter);
-        c->timer.solve = -1;
-        return PANO_OK;
-    }
-    while (!done && launched < max_iterations) {
-        int k = every;
+    int launched = 0;
+    auto enqueue = [&](int k) -> pano_status {
         if (launched + k > max_iterations) k = max_iterations - launched;
         for (int i = 0; i < k; ++i) PANO_CHECK(solver_iteration<T>(S));
         launched += k;
-        int slot = chunk & 1;
-        PANO_HIP(hipMemcpyAsync(&hs[slot], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-        PANO_HIP(hipEventRecord(c->ev[slot], c->stream));
-        if (chunk > 0) {
-            int prev = slot ^ 1;
-            PANO_HIP(hipEventSynchronize(c->ev[prev]));
-            if (hs[prev].done) done = true;
+        return PANO_OK;
+    };
+    if (every > 0) {
+        int chunk = 0;
+        bool done = false;
+        while (!done && launched < max_iterations) {
+            PANO_CHECK(enqueue(every));
+            int slot = chunk & 1;
+            PANO_HIP(hipMemcpyAsync(&hs[slot], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            PANO_HIP(hipEventRecord(c->ev[slot], c->stream));
+            if (chunk > 0) {
+                PANO_HIP(hipEventSynchronize(c->ev[slot ^ 1]));
+                if (hs[slot ^ 1].done) done = true;
+            }
+            ++chunk;
         }
-        ++chunk;
+    } else {
+        auto it = c->iter_hint.find(key);
+        int first = it != c->iter_hint.end() ? it->second : 4;
+        PANO_CHECK(enqueue(first < 1 ? 1 : first));
+        for (;;) {
+            PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+            PANO_HIP(hipStreamSynchronize(c->stream));
+            if (hs[0].done || launched >= max_iterations) break;
+            PANO_CHECK(enqueue(it != c->iter_hint.end() ? 1 : 2));
+        }
     }
     PANO_HIP(hipStreamSynchronize(c->stream));
     PANO_HIP(hipMemcpy(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost));
+    c->iter_hint[key] = hs[0].iter;
     c->timer.solve_iters.push_back(hs[0].iter);
     c->timer.solve = -1;
     return PANO_OK;
+}
+
+// Key of the iteration-count hint: the problem shape and solver settings.
+static std::string hint_key(const Dims& D, const pano_pcg_params* P, int dtype_bytes) {
+    char k[160];
+    snprintf(k, sizeof(k), "%d:%d:%d:%d:%d:%d:%d:%d:%d:%.3e:%.3e", D.d, D.h, D.w, D.bc, P->precond, P->mg_levels,
+             P->mg_nu1, P->tol_kind, dtype_bytes, P->threshold, P->dt);
+    return k;
 }
 
 static pano_status finish_info(Ctx* c, pano_pcg_info* info) {
@@ -576,6 +654,15 @@ pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, con
     S.r = residual;
     S.aux = auxiliary;
     S.p = search;
+    S.q = nullptr;
+    S.p2 = nullptr;
+    if (S.rb) {
+        void *qv, *p2;
+        PANO_CHECK(c->workspace("pcg_q", (size_t)S.n * sizeof(T), &qv));
+        PANO_CHECK(c->workspace("pcg_p2", (size_t)S.n * sizeof(T), &p2));
+        S.q = (T*)qv;
+        S.p2 = (T*)p2;
+    }
     Grid3 Gn = grid3(D.w, D.h, D.d);
     if (S.rb) {
         void* xr;
@@ -587,7 +674,7 @@ pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, con
         PANO_HIP(hipMemcpyAsync(S.r, b, (size_t)S.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     }
     PANO_CHECK(solver_init<T>(&S, P->max_iterations));
-    PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every));
+    PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every, hint_key(D, P, sizeof(T))));
     if (S.rb) {
         L3((k_from_rb<T>), Gn, D, Gn, x, S.x);
         PANO_HIP(hipStreamSynchronize(c->stream));
@@ -628,6 +715,15 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     S.aux = (T*)a;
     S.p = (T*)s;
     S.x = (T*)xv;
+    S.q = nullptr;
+    S.p2 = nullptr;
+    if (S.rb) {
+        void *qv, *p2;
+        PANO_CHECK(c->workspace("step_q", nb, &qv));
+        PANO_CHECK(c->workspace("step_p2", nb, &p2));
+        S.q = (T*)qv;
+        S.p2 = (T*)p2;
+    }
     double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
     const double* sp = P->src_radius > 0 ? sphere : nullptr;
     // source -> advection of density and velocity from the old velocity
@@ -646,7 +742,7 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     }
     PANO_HIP(hipGetLastError());
     PANO_CHECK(solver_init<T>(&S, pp.max_iterations));
-    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every));
+    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, hint_key(D, &pp, sizeof(T))));
     // u += dt d0_dual(p), closed walls zeroed
     if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x));
     else PANO_CHECK(launch_project<T>(c, D, P->dt, vel_out, S.x));
@@ -675,7 +771,7 @@ static pano_status precond_apply(Ctx* c, const Dims& D, const pano_pcg_params* P
         PANO_CHECK(c->workspace("papply_z", nb, &zr));
         Grid3 Gn = grid3(D.w, D.h, D.d);
         L3((k_to_rb<T>), Gn, D, Gn, (T*)rr, src);
-        PANO_CHECK(mg_vcycle<T>(c, &S.mg, (T*)zr, (const T*)rr, S.st));
+        PANO_CHECK(mg_vcycle<T>(c, &S.mg, (T*)zr, (const T*)rr, S.st, nullptr, nullptr));
         L3((k_from_rb<T>), Gn, D, Gn, dst, (const T*)zr);
     }
     PANO_HIP(hipGetLastError());

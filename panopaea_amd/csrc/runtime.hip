@@ -27,18 +27,19 @@ Ctx* ctx_of(pano_ctx c) { return reinterpret_cast<Ctx*>(c); }
 
 bool timer_on(Ctx* c, const char* name) { return !c->timer.name.empty() && c->timer.name == name; }
 
-void timer_mark(Ctx* c) {
+bool timer_pair(Ctx* c, double bytes, hipEvent_t* e0, hipEvent_t* e1) {
     KernelTimer& t = c->timer;
-    if (t.used == t.ev.size()) {
+    while (t.used + 2 > t.ev.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreate(&e) != hipSuccess) return false;
         t.ev.push_back(e);
     }
-    if ((t.used & 1) == 0) {
-        t.tag_solve.push_back(t.solve);
-        t.tag_checks.push_back(t.checks);
-    }
-    (void)hipEventRecord(t.ev[t.used++], c->stream);
+    t.tag_solve.push_back(t.solve);
+    t.tag_checks.push_back(t.checks);
+    t.bytes = bytes;
+    *e0 = t.ev[t.used++];
+    *e1 = t.ev[t.used++];
+    return true;
 }
 
 size_t dtype_size(int32_t dtype) { return dtype == PANO_F64 ? 8 : 4; }

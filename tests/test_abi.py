@@ -93,3 +93,10 @@ def test_generator_3d_and_gaussian():
     g2 = pa.Grid2d((16, 16))
     rho = pa.gen_gaussian_cells(g2, (0.0, 8.0, 8.0), 1.6).reshape(16, 16)
     assert rho.max() <= 1.0 and np.allclose(rho, rho[::-1, ::-1])
+
+
+def test_library_has_no_unresolved_internal_symbols():
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--undefined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    internal = [ln for ln in out.splitlines() if "_ZN4pano" in ln or " pano_" in ln]
+    assert not internal, internal
