@@ -37,19 +37,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kernel", default="smooth_l0", help="kernel whose roofline is reported")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
-    ap.add_argument("--dims", default="256,256,256")
+    ap.add_argument("--dims", default=None)
+    ap.add_argument("--workload", default="smoke3d_256",
+                    choices=["smoke3d_256", "poisson512_mixed"])
+    ap.add_argument("--cpu-check", action="store_true",
+                    help="poisson512_mixed: also solve in pure FP64 with the CPU oracle (minutes at 512^3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the fluid step runs on a single MI355X); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "poisson512_mixed":
+        return bench_mixed(args)
+
     import panopaea_amd as pa
     from panopaea_amd import _lib
     from panopaea_amd import workloads as W
 
     wl = W.SMOKE3D_256
-    dims = tuple(int(v) for v in args.dims.split(","))
+    dims = tuple(int(v) for v in (args.dims or "256,256,256").split(","))
     g = W.make_grid(wl, dims)
     cfg = W.step_config(wl, dims)
     n = g.num_elem_2()
@@ -132,6 +139,99 @@ def main():
                    "cg_iterations_per_step": iters},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_mixed(args):
+    """512^3 Poisson: FP64 iterative refinement around the FP32 MG-PCG against
+    the pure-FP64 MG-PCG (same FP64-verified residual) and the plain FP32
+    MG-PCG (its attainable accuracy, measured in FP64)."""
+    import panopaea_amd as pa
+    from panopaea_amd import workloads as W
+    wl = W.POISSON512_MIXED
+    dims = tuple(int(v) for v in (args.dims or "512,512,512").split(","))
+    n = int(np.prod(dims))
+    g64 = pa.Grid3d(dims, np.float64, bc=wl["bc"])
+    g32 = pa.Grid3d(dims, np.float32, bc=wl["bc"], ctx=g64._ctx())
+    ctx = g64._ctx()
+    faces = g64.new_simplex_1()
+    pa.gen_uniform_faces_device(faces, wl["seed"])  # 24-bit values: the same in f32 and f64
+    b64 = g64.new_simplex_2()
+    pa.neg_divergence(b64, faces)
+    faces.free()
+    mg = pa.Multigrid(wl["mg_levels"], wl["mg_nu"])
+
+    def timed(fn, reps):
+        ts, out = [], None
+        for _ in range(reps):
+            ctx.synchronize()
+            t = time.perf_counter()
+            out = fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t)
+        return out, ts
+
+    # mixed precision
+    x_mx = g64.new_simplex_2()
+    for _ in range(args.warmup):
+        pa.poisson_solve_mixed(x_mx, b64, dt=wl["dt"], preconditioner=mg, inner_tol=wl["inner_tol"],
+                               outer_tol=wl["outer_tol"])
+    info_mx, t_mx = timed(lambda: pa.poisson_solve_mixed(x_mx, b64, dt=wl["dt"], preconditioner=mg,
+                                                         inner_tol=wl["inner_tol"], outer_tol=wl["outer_tol"]),
+                          args.steps)
+    print(f"bench: mixed {info_mx.as_dict()} times {[round(t * 1e3, 1) for t in t_mx]} ms", file=sys.stderr,
+          flush=True)
+    # pure FP64 MG-PCG to the same relative residual
+    x64, r, a, s = (g64.new_simplex_2() for _ in range(4))
+    op64 = pa.PoissonOperator(g64, wl["dt"])
+    info_64, t_64 = timed(lambda: pa.precond_conjugate_gradient(mg, x64, b64, 1000, wl["outer_tol"], r, a, s, op64,
+                                                                tol_kind="rel"), max(1, args.steps // 2))
+    print(f"bench: fp64 pcg iterations={info_64.iterations} converged={info_64.converged} "
+          f"times {[round(t * 1e3, 1) for t in t_64]} ms", file=sys.stderr, flush=True)
+    # plain FP32 MG-PCG to 1e-6 (FP32 right-hand side)
+    bh = b64.download()
+    b32 = g32.new_simplex_2().upload(bh.astype(np.float32))
+    x32, r32, a32, s32 = (g32.new_simplex_2() for _ in range(4))
+    op32 = pa.PoissonOperator(g32, wl["dt"])
+    info_32, t_32 = timed(lambda: pa.precond_conjugate_gradient(mg, x32, b32, 1000, 1e-6, r32, a32, s32, op32,
+                                                                tol_kind="rel"), max(1, args.steps // 2))
+    # accuracy, measured in FP64 on the device: ||b - A x||_inf / ||b||_inf and the error vs the FP64 solve
+    def rel_res(xd):
+        q = g64.new_simplex_2()
+        op64(q, xd)
+        return float(np.abs(bh - q.download()).max() / np.abs(bh).max())
+
+    x32_64 = g64.new_simplex_2().upload(x32.download().astype(np.float64))
+    xm, xf, xs = x_mx.download(), x64.download(), x32_64.download()
+    ref = xf - xf.mean()
+    err = lambda v: float(np.abs((v - v.mean()) - ref).max() / np.abs(ref).max())
+    acc = {"mixed_rel_residual": rel_res(x_mx), "fp64_rel_residual": rel_res(x64), "fp32_rel_residual": rel_res(x32_64),
+           "mixed_err_vs_fp64_gpu": err(xm), "fp32_err_vs_fp64_gpu": err(xs)}
+    if args.cpu_check:
+        from oracle import oracle as o
+        P = o.params(dims, bc=wl["bc"], dt=wl["dt"], precond="mg", levels=wl["mg_levels"], tol=1e-12,
+                     max_iterations=2000)
+        t = time.perf_counter()
+        xo, oinfo, _ = o.solve(P, bh)
+        print(f"bench: cpu fp64 oracle solve {time.perf_counter() - t:.1f}s iterations={oinfo.iterations}",
+              file=sys.stderr, flush=True)
+        ref = xo - xo.mean()
+        acc["mixed_err_vs_fp64_cpu"] = err(xm)
+        acc["fp64_gpu_err_vs_fp64_cpu"] = err(xf)
+    print(f"bench: accuracy {acc}", file=sys.stderr, flush=True)
+    tm = float(np.median(t_mx))
+    out = {
+        "metric": "cells/s 512^3 Poisson to 1e-10 rel residual, FP64 iterative refinement around FP32 MG-PCG",
+        "value": n / tm, "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": tm * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer", "data": "synthetic (seeded i.i.d. uniform[-1,1] faces)",
+        "config": {"workload": wl["name"], "dims": list(dims), "inner_tol": wl["inner_tol"],
+                   "outer_tol": wl["outer_tol"], "precond": "multigrid 4 levels, 2+2 red-black GS",
+                   "outer_iterations": info_mx.outer_iterations, "inner_iterations": info_mx.inner_iterations},
+        "fp64_pcg": {"ms": float(np.median(t_64)) * 1e3, "iterations": info_64.iterations},
+        "fp32_pcg_1e-6": {"ms": float(np.median(t_32)) * 1e3, "iterations": info_32.iterations},
+        "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
 

@@ -168,6 +168,31 @@ pano_status pano_precond_conjugate_gradient(pano_ctx, const pano_grid*, const pa
                                             void* x, const void* b, void* residual,
                                             void* auxiliary, void* search, pano_pcg_info* info);
 
+/* ---- mixed precision: FP64 iterative refinement around an FP32 solve ------- */
+/* No reference counterpart (the reference solves in f64 only, pcg.rs with
+ * f64 fields at dec_fluid.rs:18-24). Outer loop in FP64: r = b - A x
+ * (operator and residual in FP64, L-inf norm in FP64), inner FP32 MG-PCG on
+ * A e = r to inner.threshold (relative), x += e in FP64, until
+ * ||b - A x||_inf <= outer_tol * ||b||_inf. The multigrid coarsest solve and
+ * every CG dot product accumulate in FP64. Fields x, b are FP64 (grid dtype
+ * must be PANO_F64); inner.precond must be PANO_PRECOND_MULTIGRID. */
+typedef struct {
+    pano_pcg_params inner;   /* inner FP32 solve: threshold = relative tolerance (e.g. 1e-4) */
+    double outer_tol;        /* relative L-inf residual target (e.g. 1e-10) */
+    int32_t max_outer;
+} pano_mixed_params;
+
+typedef struct {
+    int32_t outer_iterations;
+    int32_t inner_iterations; /* total FP32 CG iterations */
+    int32_t converged;
+    double b_linf;
+    double r_linf;            /* FP64 true residual at exit */
+} pano_mixed_info;
+
+pano_status pano_poisson_solve_mixed(pano_ctx, const pano_grid*, const pano_mixed_params*, double* x,
+                                     const double* b, pano_mixed_info* info);
+
 /* ---- one fluid step (dec_fluid.rs:46-141) ---------------------------------- */
 typedef struct {
     double dt;
@@ -196,6 +221,11 @@ pano_status pano_gen_bandlimited_faces(const pano_grid* g, uint64_t seed, int32_
 pano_status pano_gen_bandlimited_faces_device(pano_ctx, const pano_grid* g, uint64_t seed,
                                               int32_t n_modes, double k_max, double amplitude,
                                               void* faces);
+/* I.i.d. uniform face values in [lo, hi) (the i-th value from SplitMix64's
+ * i-th output of seed, top 24 bits), closed-wall faces zero; host and device. */
+pano_status pano_gen_uniform_faces(const pano_grid* g, uint64_t seed, double lo, double hi, void* host_faces);
+pano_status pano_gen_uniform_faces_device(pano_ctx, const pano_grid* g, uint64_t seed, double lo, double hi,
+                                          void* faces);
 /* Gaussian blob exp(-r^2 / (2 sigma^2)) at cell centres around center (z,y,x). */
 pano_status pano_gen_gaussian_cells(const pano_grid* g, const double center[3], double sigma,
                                     void* host_cells);

@@ -339,6 +339,23 @@ def project(faces, p, dt):
     _lib.check(_lib.lib().pano_project(g._ctx().handle, ctypes.byref(g.desc), float(dt), faces.ptr, p.ptr))
 
 
+def poisson_solve_mixed(x, b, dt=1.0, preconditioner=None, inner_tol=1e-4, outer_tol=1e-10, max_outer=20,
+                        max_inner=500, remove_mean=-1):
+    """FP64 iterative refinement around an FP32 multigrid-PCG solve of A x = b
+    (x, b FP64 device fields on an FP64 grid). Returns MixedInfo."""
+    g = x.grid
+    M = _lib.MixedParams()
+    M.inner = _pcg_params(preconditioner or Multigrid(), max_inner, inner_tol, PoissonOperator(g, dt), "rel", 0,
+                          remove_mean)
+    M.outer_tol = outer_tol
+    M.max_outer = max_outer
+    info = _lib.MixedInfo()
+    _lib.check(_lib.lib().pano_poisson_solve_mixed(g._ctx().handle, ctypes.byref(g.desc), ctypes.byref(M), x.ptr,
+                                                   b.ptr, ctypes.byref(info)),
+               allow=(PANO_OK, PANO_NOT_CONVERGED))
+    return info
+
+
 class StepConfig:
     """Parameters of one fluid step (source, advection, projection)."""
 
@@ -391,6 +408,20 @@ def gen_bandlimited_faces_device(field, seed, n_modes, k_max, amplitude=1.0):
     g = field.grid
     _lib.check(_lib.lib().pano_gen_bandlimited_faces_device(g._ctx().handle, ctypes.byref(g.desc), seed, n_modes,
                                                             k_max, amplitude, field.ptr))
+    return field
+
+
+def gen_uniform_faces(grid, seed, lo=-1.0, hi=1.0):
+    """Seeded i.i.d. uniform face values (host array, library layout)."""
+    out = np.empty(grid.num_elem_1(), grid.dtype)
+    _lib.check(_lib.lib().pano_gen_uniform_faces(ctypes.byref(grid.desc), seed, lo, hi, out.ctypes.data))
+    return out
+
+
+def gen_uniform_faces_device(field, seed, lo=-1.0, hi=1.0):
+    g = field.grid
+    _lib.check(_lib.lib().pano_gen_uniform_faces_device(g._ctx().handle, ctypes.byref(g.desc), seed, lo, hi,
+                                                        field.ptr))
     return field
 
 

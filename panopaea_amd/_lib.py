@@ -53,6 +53,18 @@ class PcgInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MixedParams(ctypes.Structure):
+    _fields_ = [("inner", PcgParams), ("outer_tol", c_f64), ("max_outer", c_i32)]
+
+
+class MixedInfo(ctypes.Structure):
+    _fields_ = [("outer_iterations", c_i32), ("inner_iterations", c_i32), ("converged", c_i32), ("b_linf", c_f64),
+                ("r_linf", c_f64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", c_f64), ("pcg", PcgParams), ("buoyancy", c_f64), ("src_center", c_f64 * 3),
                 ("src_radius", c_f64)]
@@ -88,11 +100,15 @@ _SIGS = {
     "pano_precond_apply": ([c_void_p, _P(GridDesc), _P(PcgParams), c_void_p, c_void_p], c_i32),
     "pano_precond_conjugate_gradient": ([c_void_p, _P(GridDesc), _P(PcgParams), c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, _P(PcgInfo)], c_i32),
+    "pano_poisson_solve_mixed": ([c_void_p, _P(GridDesc), _P(MixedParams), c_void_p, c_void_p, _P(MixedInfo)],
+                                 c_i32),
     "pano_fluid_step": ([c_void_p, _P(GridDesc), _P(StepParams), c_void_p, c_void_p, c_void_p, c_void_p,
                          c_void_p, _P(PcgInfo)], c_i32),
     "pano_gen_bandlimited_faces": ([_P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_bandlimited_faces_device": ([c_void_p, _P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p],
                                           c_i32),
+    "pano_gen_uniform_faces": ([_P(GridDesc), c_u64, c_f64, c_f64, c_void_p], c_i32),
+    "pano_gen_uniform_faces_device": ([c_void_p, _P(GridDesc), c_u64, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_gaussian_cells": ([_P(GridDesc), _P(c_f64), c_f64, c_void_p], c_i32),
 }
 for _name in ("hodge_0_primal", "hodge_2_dual", "hodge_1_primal", "hodge_1_dual", "hodge_2_primal",

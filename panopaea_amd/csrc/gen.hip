@@ -193,6 +193,40 @@ static void host_gen(const Dims& D, int ncomp, const CompGeom* g, const double* 
     }
 }
 
+// I.i.d. uniform face values: element i of the face array takes the i-th
+// SplitMix64 output of the stream started at `seed` (evaluated directly from
+// the counter, so host and device agree), top 24 bits scaled to [0, 1) and
+// mapped to [lo, hi); closed-wall faces are zero.
+__host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline double uniform_face(const Dims& D, const FaceLayout& L, uint64_t seed, double lo, double hi,
+                                               int64_t i) {
+    int axis;
+    int64_t j;
+    if (i < L.oy) { axis = 2; j = i; }
+    else if (i < L.ox) { axis = 1; j = i - L.oy; }
+    else { axis = 0; j = i - L.ox; }
+    int nxc = D.w + (axis == 0), nyc = D.h + (axis == 1);
+    int x = (int)(j % nxc);
+    int64_t r = j / nxc;
+    int y = (int)(r % nyc);
+    int z = (int)(r / nyc);
+    if (wall_face(axis, D.bc, nxc, nyc, 0, x, y, z, D)) return 0.0;
+    double u = (double)(splitmix_at(seed, (uint64_t)i) >> 40) * (1.0 / 16777216.0);
+    return lo + (hi - lo) * u;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gen_uniform(Dims D, FaceLayout L, uint64_t seed, double lo, double hi,
+                                                     T* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L.total) out[i] = (T)uniform_face(D, L, seed, lo, hi, i);
+}
+
 static pano_status gen_args(const pano_grid* g, int32_t n_modes, double k_max) {
     PANO_CHECK(check_grid(g));
     if (n_modes < 0 || n_modes > 4096) return fail(PANO_ERR_INVALID, "n_modes out of range");
@@ -247,6 +281,40 @@ pano_status pano_gen_bandlimited_faces_device(pano_ctx ctx, const pano_grid* g, 
             hipLaunchKernelGGL(k_gen_faces<float>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, D,
                                axis, G[k], (const double*)dtab, n_modes, amplitude, (float*)faces);
     }
+    PANO_HIP(hipGetLastError());
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    return PANO_OK;
+}
+
+pano_status pano_gen_uniform_faces(const pano_grid* g, uint64_t seed, double lo, double hi, void* host_faces) {
+    PANO_CHECK(check_grid(g));
+    if (!host_faces || !(hi >= lo)) return fail(PANO_ERR_INVALID, "bad argument");
+    Dims D = dims_of(g);
+    FaceLayout L = face_layout(D);
+    for (int64_t i = 0; i < L.total; ++i) {
+        double v = uniform_face(D, L, seed, lo, hi, i);
+        if (g->dtype == PANO_F64) ((double*)host_faces)[i] = v;
+        else ((float*)host_faces)[i] = (float)v;
+    }
+    return PANO_OK;
+}
+
+pano_status pano_gen_uniform_faces_device(pano_ctx ctx, const pano_grid* g, uint64_t seed, double lo, double hi,
+                                          void* faces) {
+    Ctx* c = ctx_of(ctx);
+    if (!c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    if (!faces || !(hi >= lo)) return fail(PANO_ERR_INVALID, "bad argument");
+    PANO_HIP(hipSetDevice(c->device));
+    Dims D = dims_of(g);
+    FaceLayout L = face_layout(D);
+    unsigned nb = blocks_for(L.total, kBlock);
+    if (g->dtype == PANO_F64)
+        hipLaunchKernelGGL(k_gen_uniform<double>, dim3(nb), dim3(kBlock), 0, c->stream, D, L, seed, lo, hi,
+                           (double*)faces);
+    else
+        hipLaunchKernelGGL(k_gen_uniform<float>, dim3(nb), dim3(kBlock), 0, c->stream, D, L, seed, lo, hi,
+                           (float*)faces);
     PANO_HIP(hipGetLastError());
     PANO_HIP(hipStreamSynchronize(c->stream));
     return PANO_OK;

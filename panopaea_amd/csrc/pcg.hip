@@ -778,6 +778,133 @@ static pano_status precond_apply(Ctx* c, const Dims& D, const pano_pcg_params* P
     return PANO_OK;
 }
 
+// ---- mixed precision: FP64 iterative refinement around the FP32 MG-PCG ------------------
+
+// r32 = (float)(b - A x) with the operator and the difference in FP64
+// (red-black layout); partial max |b - A x| in FP64.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_resid64_rb(Dims D, Grid3 G, float* r32, const double* x, const double* b,
+                                                    double dt, double* pmax) {
+    int m, y, z, L;
+    tile_xyz(G, m, y, z, L);
+    double mx = 0.0;
+    if (m < (D.w >> 1) && y < D.h) {
+        RBRow R = rb_row<DIM>(D, z, y, m);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int i = k == 0 ? R.i0 : R.i1;
+            int has = k == 0 ? R.has0 : R.has1;
+            double n[6];
+            rb_nbrs<double, DIM>(x, R, k, n);
+            double ax = poisson_from<double, DIM>(x[i], n[0], n[1], n[2], n[3], n[4], n[5], has, D.bc, dt);
+            double r = b[i] - ax;
+            r32[i] = (float)r;
+            mx = absmax_acc(mx, r);
+        }
+    }
+    double t = block_max(mx);
+    if (threadIdx.x == 0 && threadIdx.y == 0) pmax[L] = t;
+}
+
+// x += (double)e
+__global__ __launch_bounds__(256) void k_add_correction(int64_t n, double* x, const float* e) {
+    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) x[i] = x[i] + (double)e[i];
+    }
+}
+
+static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M, double* x, const double* b,
+                               pano_mixed_info* info) {
+    if (M->inner.precond != PANO_PRECOND_MULTIGRID)
+        return fail(PANO_ERR_INVALID, "the mixed-precision solve needs the multigrid inner preconditioner");
+    if (!(M->outer_tol > 0) || M->max_outer < 1) return fail(PANO_ERR_INVALID, "outer_tol > 0 and max_outer >= 1");
+    pano_pcg_params ip = M->inner;
+    ip.tol_kind = PANO_TOL_REL_LINF;
+    Solver<float> S;
+    PANO_CHECK(solver_setup<float>(c, D, &ip, &S));
+    const int64_t n = S.n;
+    void *r32, *aux, *pp, *q, *p2, *e, *b64, *x64, *st2v, *part2;
+    PANO_CHECK(c->workspace("mx_r32", n * sizeof(float), &r32));
+    PANO_CHECK(c->workspace("mx_aux", n * sizeof(float), &aux));
+    PANO_CHECK(c->workspace("mx_p", n * sizeof(float), &pp));
+    PANO_CHECK(c->workspace("mx_q", n * sizeof(float), &q));
+    PANO_CHECK(c->workspace("mx_p2", n * sizeof(float), &p2));
+    PANO_CHECK(c->workspace("mx_e", n * sizeof(float), &e));
+    PANO_CHECK(c->workspace("mx_b64", n * sizeof(double), &b64));
+    PANO_CHECK(c->workspace("mx_x64", n * sizeof(double), &x64));
+    PANO_CHECK(c->workspace("mx_state", sizeof(SolverState), &st2v));
+    int np = S.nflat > S.ntile ? S.nflat : S.ntile;
+    PANO_CHECK(c->workspace("mx_part", (size_t)np * sizeof(double), &part2));
+    S.r = (float*)r32;
+    S.aux = (float*)aux;
+    S.p = (float*)pp;
+    S.q = (float*)q;
+    S.p2 = (float*)p2;
+    S.x = (float*)e;
+    SolverState* st2 = (SolverState*)st2v;
+    double* part = (double*)part2;
+    double* bb = (double*)b64;
+    double* xx = (double*)x64;
+    hipStream_t s = c->stream;
+    SolveCfg cfg2;
+    cfg2.tol_kind = PANO_TOL_REL_LINF;
+    cfg2.tol = M->outer_tol;
+    cfg2.n = n;
+    cfg2.remove_mean = ip.remove_mean < 0 ? (D.bc == 0) : ip.remove_mean;
+    Grid3 Gn = grid3(D.w, D.h, D.d);
+    hipLaunchKernelGGL((k_to_rb<double>), dim3(Gn.nb), dim3(kTX, kTY), 0, s, D, Gn, bb, b);
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st2, M->max_outer + 1);
+    hipLaunchKernelGGL((k_zero<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, xx);
+    hipLaunchKernelGGL((k_stats<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, (const double*)bb, S.part0, part);
+    if (cfg2.remove_mean) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_STATS, (const double*)S.part0, S.nflat,
+                           st2, cfg2, 0);
+        hipLaunchKernelGGL((k_sub_mean<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, bb, (const SolverState*)st2,
+                           part);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_THRESH, (const double*)part, S.nflat, st2,
+                       cfg2, 1);
+    SolverState* hs = c->host_state;
+    const std::string key = hint_key(D, &ip, sizeof(float));
+    int outer = 0, inner_total = 0;
+    for (;;) {
+        // r = b - A x in FP64 (to the FP32 buffer of the inner solve); stop test in FP64
+        if (D.d > 1)
+            hipLaunchKernelGGL(k_resid64_rb<3>, dim3(S.G.nb), dim3(kTX, kTY), 0, s, D, S.G, S.r, (const double*)xx,
+                               (const double*)bb, M->inner.dt, part);
+        else
+            hipLaunchKernelGGL(k_resid64_rb<2>, dim3(S.G.nb), dim3(kTX, kTY), 0, s, D, S.G, S.r, (const double*)xx,
+                               (const double*)bb, M->inner.dt, part);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_RNORM, (const double*)part, S.ntile, st2,
+                           cfg2, 1);
+        PANO_HIP(hipMemcpyAsync(&hs[1], st2, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+        PANO_HIP(hipStreamSynchronize(s));
+        if (hs[1].done) break;
+        // inner FP32 solve A e = r to the inner relative tolerance
+        PANO_CHECK(solver_init<float>(&S, ip.max_iterations));
+        PANO_CHECK(solver_loop<float>(&S, ip.max_iterations, ip.check_every, key));
+        if (hs[0].nonfinite) return fail(PANO_ERR_NONFINITE, "non-finite value in the inner solve");
+        inner_total += hs[0].iter;
+        ++outer;
+        hipLaunchKernelGGL(k_add_correction, dim3(S.nflat), dim3(kBlock), 0, s, n, xx, (const float*)S.x);
+    }
+    hipLaunchKernelGGL((k_from_rb<double>), dim3(Gn.nb), dim3(kTX, kTY), 0, s, D, Gn, x, (const double*)xx);
+    PANO_HIP(hipStreamSynchronize(s));
+    PANO_HIP(hipGetLastError());
+    if (info) {
+        info->outer_iterations = outer;
+        info->inner_iterations = inner_total;
+        info->converged = hs[1].converged;
+        info->b_linf = hs[1].b_linf;
+        info->r_linf = hs[1].rnorm;
+    }
+    if (hs[1].nonfinite) return fail(PANO_ERR_NONFINITE, "non-finite value in the right-hand side");
+    return hs[1].converged ? PANO_OK : PANO_NOT_CONVERGED;
+}
+
 }  // namespace pano
 
 using namespace pano;
@@ -814,6 +941,17 @@ pano_status pano_precond_apply(pano_ctx ctx, const pano_grid* g, const pano_pcg_
     Dims D = dims_of(g);
     if (g->dtype == PANO_F64) return precond_apply<double>(c, D, P, (double*)dst, (const double*)src);
     return precond_apply<float>(c, D, P, (float*)dst, (const float*)src);
+}
+
+pano_status pano_poisson_solve_mixed(pano_ctx ctx, const pano_grid* g, const pano_mixed_params* M, double* x,
+                                     const double* b, pano_mixed_info* info) {
+    Ctx* c = ctx_of(ctx);
+    if (!c) return fail(PANO_ERR_INVALID, "ctx is null");
+    PANO_CHECK(check_grid(g));
+    if (g->dtype != PANO_F64) return fail(PANO_ERR_INVALID, "the mixed-precision solve takes FP64 x and b");
+    if (!M || !x || !b || (const void*)x == (const void*)b) return fail(PANO_ERR_INVALID, "null or aliased argument");
+    PANO_HIP(hipSetDevice(c->device));
+    return mixed_solve(c, dims_of(g), M, x, b, info);
 }
 
 pano_status pano_fluid_step(pano_ctx ctx, const pano_grid* g, const pano_step_params* P, const void* density_in,

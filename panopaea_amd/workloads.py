@@ -29,6 +29,15 @@ SMOKE3D_256 = dict(
 )
 
 
+# Poisson only, 512^3, closed walls: b = -div u of i.i.d. uniform[-1, 1] f32
+# face values; FP32 MG-PCG inner solve to 1e-4 inside FP64 iterative
+# refinement to 1e-10 relative L-inf residual.
+POISSON512_MIXED = dict(
+    name="poisson512_mixed", dims=(512, 512, 512), seed=0, bc=(0, 0, 0, 0, 0, 0), precond="mg", mg_levels=4,
+    mg_nu=2, inner_tol=1e-4, outer_tol=1e-10, dt=1.0,
+)
+
+
 def make_grid(wl, dims=None, dtype=np.float32, ctx=None):
     dims = tuple(dims or wl["dims"])
     if len(dims) == 2:

@@ -100,3 +100,14 @@ def test_library_has_no_unresolved_internal_symbols():
     out = subprocess.run(["nm", "-D", "--undefined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
     internal = [ln for ln in out.splitlines() if "_ZN4pano" in ln or " pano_" in ln]
     assert not internal, internal
+
+
+def test_uniform_generator():
+    g = pa.Grid3d((6, 8, 10), np.float32)
+    a = pa.gen_uniform_faces(g, 11)
+    b = pa.gen_uniform_faces(g, 11)
+    assert np.array_equal(a, b)
+    vz, vy, vx = g.split_faces(a)
+    assert np.all(vz[[0, -1]] == 0) and np.all(vy[:, [0, -1]] == 0) and np.all(vx[:, :, [0, -1]] == 0)
+    inner = vx[:, :, 1:-1].ravel()
+    assert inner.min() >= -1.0 and inner.max() < 1.0 and abs(inner.mean()) < 0.1 and inner.std() > 0.5
