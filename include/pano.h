@@ -210,6 +210,43 @@ pano_status pano_fluid_step(pano_ctx, const pano_grid*, const pano_step_params*,
                             const void* density_in, const void* velocity_in, void* density_out,
                             void* velocity_out, void* pressure, pano_pcg_info* info);
 
+/* ---- batched step: many independent uneven grids per launch ----------------- */
+/* No reference counterpart (the reference steps one grid, dec_fluid.rs:46).
+ * Every launch covers the tiles of all grids; per-grid solver states let
+ * converged grids idle while the others iterate; the step phases are
+ * captured as hipGraphs; tile shapes come from a timing autotuner or are
+ * given. A grid's results are bit-identical to the same grid run alone in a
+ * batch with the same tile shapes. f32 fields, MG-PCG to a relative L-inf
+ * tolerance, no source or buoyancy. */
+typedef struct pano_batch_s* pano_batch;
+typedef struct {
+    int32_t ndim;            /* 2 or 3 */
+    int64_t nz, ny, nx;
+    int32_t bc[6];
+    double dt;
+} pano_batch_grid;
+typedef struct {
+    int32_t mg_levels, mg_nu;
+    double tol;              /* relative L-inf residual */
+    int32_t max_iterations;
+    int32_t use_graph;       /* capture the step phases in hipGraphs */
+    int32_t autotune;        /* time the candidate tile shapes and keep the fastest */
+    int32_t tile_nat[2];     /* threads (x, y) of natural-layout tiles when not autotuning (x*y = 256) */
+    int32_t tile_rb[2];      /* threads (x, y) of red-black pair tiles when not autotuning */
+} pano_batch_params;
+pano_status pano_batch_create(pano_ctx, const pano_batch_grid* grids, int32_t n, const pano_batch_params* params,
+                              pano_batch* out);
+pano_status pano_batch_destroy(pano_batch);
+/* Host f32 density (cells) and velocity (faces) of grid i; either may be NULL. */
+pano_status pano_batch_set_state(pano_batch, int32_t i, const float* density, const float* faces);
+pano_status pano_batch_get_state(pano_batch, int32_t i, float* density, float* faces);
+/* One advect+project step of every grid; iterations (optional, n entries)
+ * receives each grid's CG iteration count. Synchronous. */
+pano_status pano_batch_step(pano_batch, int32_t* iterations);
+/* The tile shapes in use and the autotuner's timings (us, 2 families x 4
+ * candidates: 64x4, 32x8, 128x2, 16x16). */
+pano_status pano_batch_tiles(pano_batch, int32_t* tile_nat, int32_t* tile_rb, double* tune_us);
+
 /* ---- seeded synthetic inputs (host, deterministic) ------------------------- */
 /* Band-limited velocity: each face component is the sum of n_modes
  * sinusoids amplitude*sin(2*pi*(k.X)/N + phi), integer k with

@@ -29,7 +29,7 @@ from ._lib import (PanoError, PcgInfo, PcgParams, StepParams, GridDesc, PANO_OK,
 __all__ = [
     "Context", "default_context", "DeviceField", "Grid2d", "Grid3d", "Identity", "Jacobi", "Multigrid",
     "PoissonOperator", "precond_conjugate_gradient", "advect", "advect_mac", "neg_divergence", "project",
-    "fluid_step", "PanoError", "StepConfig",
+    "fluid_step", "PanoError", "StepConfig", "Batch",
 ]
 
 
@@ -354,6 +354,76 @@ def poisson_solve_mixed(x, b, dt=1.0, preconditioner=None, inner_tol=1e-4, outer
                                                    b.ptr, ctypes.byref(info)),
                allow=(PANO_OK, PANO_NOT_CONVERGED))
     return info
+
+
+class Batch:
+    """Many independent uneven grids advanced by one set of launches
+    (pano_batch_*): grids = [dict(dims=(y, x) or (z, y, x), bc=..., dt=...)].
+    f32 fields, MG-PCG to a relative L-inf tolerance, steps captured as
+    hipGraphs, tile shapes autotuned or given."""
+
+    def __init__(self, grids, mg_levels=4, mg_nu=2, tol=1e-6, max_iterations=500, use_graph=True, autotune=False,
+                 tile_nat=(64, 4), tile_rb=(64, 4), ctx=None):
+        self.ctx = ctx or default_context()
+        self.grids = [dict(g) for g in grids]
+        arr = (_lib.BatchGrid * len(grids))()
+        for k, g in enumerate(grids):
+            dims = tuple(g["dims"])
+            a = arr[k]
+            a.ndim = len(dims)
+            a.nz, a.ny, a.nx = (1,) + dims if len(dims) == 2 else dims
+            bc = tuple(g.get("bc", (0,) * 6))
+            for j in range(6):
+                a.bc[j] = int(bc[j]) if j < len(bc) else 0
+            a.dt = float(g["dt"])
+        P = _lib.BatchParams()
+        P.mg_levels, P.mg_nu, P.tol, P.max_iterations = mg_levels, mg_nu, tol, max_iterations
+        P.use_graph, P.autotune = int(use_graph), int(autotune)
+        P.tile_nat[0], P.tile_nat[1] = tile_nat
+        P.tile_rb[0], P.tile_rb[1] = tile_rb
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().pano_batch_create(self.ctx.handle, arr, len(grids), ctypes.byref(P), ctypes.byref(h)))
+        self.handle = h
+        self._geo = [(_Grid2d_or_3d(g["dims"], g.get("bc"))) for g in grids]
+
+    def set_state(self, i, density=None, faces=None):
+        d = None if density is None else np.ascontiguousarray(density, np.float32)
+        f = None if faces is None else np.ascontiguousarray(faces, np.float32)
+        _lib.check(_lib.lib().pano_batch_set_state(self.handle, i, None if d is None else d.ctypes.data,
+                                                   None if f is None else f.ctypes.data))
+
+    def get_state(self, i):
+        g = self._geo[i]
+        d = np.empty(g.num_elem_2(), np.float32)
+        f = np.empty(g.num_elem_1(), np.float32)
+        _lib.check(_lib.lib().pano_batch_get_state(self.handle, i, d.ctypes.data, f.ctypes.data))
+        return d, f
+
+    def step(self):
+        it = (ctypes.c_int32 * len(self.grids))()
+        _lib.check(_lib.lib().pano_batch_step(self.handle, it), allow=(PANO_OK, PANO_NOT_CONVERGED))
+        return list(it)
+
+    def tiles(self):
+        tn, tr, us = (ctypes.c_int32 * 2)(), (ctypes.c_int32 * 2)(), (ctypes.c_double * 8)()
+        _lib.check(_lib.lib().pano_batch_tiles(self.handle, tn, tr, us))
+        return {"nat": tuple(tn), "rb": tuple(tr), "tune_us": list(us)}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().pano_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _Grid2d_or_3d(dims, bc=None):
+    dims = tuple(dims)
+    return (Grid2d if len(dims) == 2 else Grid3d)(dims, np.float32, bc=bc)
 
 
 class StepConfig:

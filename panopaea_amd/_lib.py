@@ -65,6 +65,15 @@ class MixedInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchGrid(ctypes.Structure):
+    _fields_ = [("ndim", c_i32), ("nz", c_i64), ("ny", c_i64), ("nx", c_i64), ("bc", c_i32 * 6), ("dt", c_f64)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("mg_levels", c_i32), ("mg_nu", c_i32), ("tol", c_f64), ("max_iterations", c_i32),
+                ("use_graph", c_i32), ("autotune", c_i32), ("tile_nat", c_i32 * 2), ("tile_rb", c_i32 * 2)]
+
+
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", c_f64), ("pcg", PcgParams), ("buoyancy", c_f64), ("src_center", c_f64 * 3),
                 ("src_radius", c_f64)]
@@ -104,6 +113,12 @@ _SIGS = {
                                  c_i32),
     "pano_fluid_step": ([c_void_p, _P(GridDesc), _P(StepParams), c_void_p, c_void_p, c_void_p, c_void_p,
                          c_void_p, _P(PcgInfo)], c_i32),
+    "pano_batch_create": ([c_void_p, _P(BatchGrid), c_i32, _P(BatchParams), _P(c_void_p)], c_i32),
+    "pano_batch_destroy": ([c_void_p], c_i32),
+    "pano_batch_set_state": ([c_void_p, c_i32, c_void_p, c_void_p], c_i32),
+    "pano_batch_get_state": ([c_void_p, c_i32, c_void_p, c_void_p], c_i32),
+    "pano_batch_step": ([c_void_p, _P(c_i32)], c_i32),
+    "pano_batch_tiles": ([c_void_p, _P(c_i32), _P(c_i32), _P(c_f64)], c_i32),
     "pano_gen_bandlimited_faces": ([_P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_bandlimited_faces_device": ([c_void_p, _P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p],
                                           c_i32),

@@ -1,0 +1,1116 @@
+// Batched engine: the fluid step of many uneven 2D/3D grids per launch.
+//
+// Every launch covers the tiles of all grids in its entry table; workgroup b
+// (after the XCD remap) belongs to the grid whose block range holds b and
+// runs the same per-element body as the single-grid kernel on the grid's own
+// tile decomposition, so a grid's results are bit-identical alone and in any
+// mix (same tile shapes). Per-grid solver states make converged grids' tiles
+// no-ops while the others iterate; the one-workgroup-per-grid finalize
+// reduces each grid's partials in a fixed order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "batch.hpp"
+
+namespace pano {
+
+// ---- launch decode -------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_entry(const BEntry* e, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (e[mid].start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ void local_xyz(const Grid3& G, int lb, int& x, int& y, int& z) {
+    int bx = lb % G.gx;
+    int t = lb / G.gx;
+    int by = t % G.gy;
+    z = t / G.gy;
+    x = bx * blockDim.x + threadIdx.x;
+    y = by * blockDim.y + threadIdx.y;
+}
+template <typename T>
+__device__ __forceinline__ T* fld(char* arena, int64_t off) {
+    return reinterpret_cast<T*>(arena + off);
+}
+
+#define BATCH_ENTRY(tab, ne, nbt)              \
+    const int b_ = xcd_remap(blockIdx.x, nbt); \
+    const int g_ = find_entry(tab, ne, b_);    \
+    const BEntry& E = tab[g_];                 \
+    const int lb = b_ - E.start;
+
+// ---- step kernels ----------------------------------------------------------------------------
+
+// density advection: o[0]/o[1] = rho ping/pong, o[2]/o[3] = u ping/pong
+__global__ __launch_bounds__(256) void kb_advect_cell(const BEntry* tab, int ne, int nbt, char* A, int ping) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int x, y, z;
+    local_xyz(E.G, lb, x, y, z);
+    const Dims& D = E.D;
+    if (x >= D.w || y >= D.h) return;
+    const float* q = fld<float>(A, E.o[ping]);
+    float* dst = fld<float>(A, E.o[1 - ping]);
+    Faces<const float> v = split_faces(D, (const float*)fld<float>(A, E.o[2 + ping]));
+    Sphere S{0, 0, 0, 0, 0, {0, 0, 0}, {-1, -1, -1}};
+    float dt = (float)E.s;
+    if (D.d > 1) dst[cell_index(D, z, y, x)] = advect_cell_value<float, 3>(D, q, S, dt, v, z, y, x);
+    else dst[cell_index(D, z, y, x)] = advect_cell_value<float, 2>(D, q, S, dt, v, z, y, x);
+}
+
+// one velocity component: axis 0 (x faces), 1 (y faces), 2 (z faces); o[0]/o[1] = u ping/pong
+__global__ __launch_bounds__(256) void kb_advect_face(const BEntry* tab, int ne, int nbt, char* A, int ping,
+                                                      int axis) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int x, y, z;
+    local_xyz(E.G, lb, x, y, z);
+    const Dims& D = E.D;
+    if (x >= D.w + (axis == 0) || y >= D.h + (axis == 1)) return;
+    Faces<const float> v = split_faces(D, (const float*)fld<float>(A, E.o[ping]));
+    Faces<float> o = split_faces(D, fld<float>(A, E.o[1 - ping]));
+    float dt = (float)E.s;
+    Sphere S{0, 0, 0, 0, 0, {0, 0, 0}, {-1, -1, -1}};
+    if (axis == 0) {
+        float r = D.d > 1 ? advect_face_x<float, 3>(D, v.x, dt, v, z, y, x) : advect_face_x<float, 2>(D, v.x, dt, v, z, y, x);
+        o.x[fx_idx(D, z, y, x)] = r;
+    } else if (axis == 1) {
+        float r = D.d > 1 ? advect_face_y<float, 3>(D, v.y, dt, v, z, y, x) : advect_face_y<float, 2>(D, v.y, dt, v, z, y, x);
+        o.y[fy_idx(D, z, y, x)] = r;
+    } else {
+        o.z[fz_idx(D, z, y, x)] = advect_face_z<float>(D, v.z, dt, v, z, y, x, 0.0f, nullptr, S);
+    }
+}
+
+// r = -div u into the red-black layout: o[0] = r, o[1] = u
+__global__ __launch_bounds__(256) void kb_neg_div(const BEntry* tab, int ne, int nbt, char* A) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int x, y, z;
+    local_xyz(E.G, lb, x, y, z);
+    const Dims& D = E.D;
+    if (x >= D.w || y >= D.h) return;
+    Faces<const float> v = split_faces(D, (const float*)fld<float>(A, E.o[1]));
+    float dv = D.d > 1 ? div_value<float, 3>(D, v, z, y, x) : div_value<float, 2>(D, v, z, y, x);
+    fld<float>(A, E.o[0])[rb_index(D, z, y, x)] = -dv;
+}
+
+// u += dt d0_dual(p) with p red-black: o[0] = u, o[1] = p
+__global__ __launch_bounds__(256) void kb_project(const BEntry* tab, int ne, int nbt, char* A, int axis) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int x, y, z;
+    local_xyz(E.G, lb, x, y, z);
+    const Dims& D = E.D;
+    if (x >= D.w + (axis == 0) || y >= D.h + (axis == 1)) return;
+    Faces<float> u = split_faces(D, fld<float>(A, E.o[0]));
+    const float* p = fld<float>(A, E.o[1]);
+    float dt = (float)E.s;
+    if (axis == 0) {
+        int64_t i = fx_idx(D, z, y, x);
+        u.x[i] = project_x<float, REDBLACK>(D, u.x[i], p, dt, z, y, x);
+    } else if (axis == 1) {
+        int64_t i = fy_idx(D, z, y, x);
+        u.y[i] = project_y<float, REDBLACK>(D, u.y[i], p, dt, z, y, x);
+    } else {
+        int64_t i = fz_idx(D, z, y, x);
+        u.z[i] = project_z<float, REDBLACK>(D, u.z[i], p, dt, z, y, x);
+    }
+}
+
+// ---- solver kernels --------------------------------------------------------------------------
+
+__global__ void kb_state_init(const BEntry* tab, SolverState* st, int max_iter) {
+    if (threadIdx.x != 0) return;
+    SolverState s = {};
+    s.max_iter = max_iter;
+    st[tab[blockIdx.x].sidx] = s;
+}
+
+template <typename Tr>
+__global__ void kb_round_threshold(const BEntry* tab, SolverState* st) {
+    if (threadIdx.x != 0) return;
+    SolverState* s = &st[tab[blockIdx.x].sidx];
+    s->threshold = (double)(Tr)s->threshold;
+    if (!s->done && s->b_linf < s->threshold) {  // pcg.rs:35-38
+        s->converged = 1;
+        s->done = 1;
+    }
+}
+
+// One workgroup per grid: reduces the grid's partials of the launch `tab`
+// (blocks [start, start + G.nb)) and updates its state.
+__global__ __launch_bounds__(1024) void kb_finalize(int op, const BEntry* tab, const double* part0,
+                                                    const double* part1, SolverState* st, const SolveCfg* cfg,
+                                                    int use_max) {
+    const BEntry& E = tab[blockIdx.x];
+    finalize_range(op, part0 + E.start, part1 ? part1 + E.start : nullptr, E.G.nb, &st[E.sidx], cfg[E.sidx],
+                   use_max);
+    if (op == FIN_STATS && threadIdx.x == 0 && !cfg[E.sidx].remove_mean) st[E.sidx].mean = 0.0;
+}
+
+// flat vector kernels over the "flat" table: o[0] = x, o[1] = r, o[2] = z (aux, q), o[3] = p
+__global__ __launch_bounds__(256) void kb_zero_stats(const BEntry* tab, int ne, int nbt, char* A, double* psum,
+                                                     double* pmax) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
+    flat_zero(lb, n, fld<float>(A, E.o[0]));
+    double s, m;
+    flat_stats(lb, n, (const float*)fld<float>(A, E.o[1]), &s, &m);
+    if (threadIdx.x == 0) {
+        psum[b_] = s;
+        pmax[b_] = m;
+    }
+}
+__global__ __launch_bounds__(256) void kb_sub_mean(const BEntry* tab, int ne, int nbt, char* A,
+                                                   const SolverState* st, double* pmax) {
+    BATCH_ENTRY(tab, ne, nbt);
+    int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
+    double m = flat_sub_mean(lb, n, fld<float>(A, E.o[1]), st[E.sidx].mean);
+    if (threadIdx.x == 0) pmax[b_] = m;
+}
+__global__ __launch_bounds__(256) void kb_copy_dot(const BEntry* tab, int ne, int nbt, char* A,
+                                                   const SolverState* st, double* part) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
+    double t = flat_copy_dot(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]),
+                             (const float*)fld<float>(A, E.o[1]));
+    if (threadIdx.x == 0) part[b_] = t;
+}
+__global__ __launch_bounds__(256) void kb_update(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st,
+                                                 double* pmax) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
+    double m, s;
+    flat_update(lb, n, fld<float>(A, E.o[0]), fld<float>(A, E.o[1]), (const float*)fld<float>(A, E.o[3]),
+                (const float*)fld<float>(A, E.o[2]), st[E.sidx].alpha, false, &m, &s);
+    if (threadIdx.x == 0) pmax[b_] = m;
+}
+__global__ __launch_bounds__(256) void kb_search(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
+    flat_search(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]), st[E.sidx].beta);
+}
+
+// red-black pair kernels over rb_[l]: o[0] = z_l, o[1] = r_l, o[2] = res, o[3] = p (level 0)
+__global__ __launch_bounds__(256) void kb_apply(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st,
+                                                double* part) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int m, y, z;
+    local_xyz(E.G, lb, m, y, z);
+    const Dims& D = E.D;
+    double s = 0.0;
+    if (m < (D.w >> 1) && y < D.h) {
+        float* q = fld<float>(A, E.o[0]);
+        const float* p = fld<float>(A, E.o[3]);
+        float dt = (float)E.s;
+        s = D.d > 1 ? apply_pair<float, 3>(D, q, p, dt, m, y, z) : apply_pair<float, 2>(D, q, p, dt, m, y, z);
+    }
+    double t = block_sum(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[b_] = t;
+}
+
+__global__ __launch_bounds__(256) void kb_smooth_first(const BEntry* tab, int ne, int nbt, char* A,
+                                                       const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st && st[E.sidx].done) return;
+    int m, y, z;
+    local_xyz(E.G, lb, m, y, z);
+    const Dims& D = E.D;
+    if (m >= (D.w >> 1) || y >= D.h) return;
+    RInv<float> ri;
+    for (int k = 0; k < 8; ++k) ri.v[k] = E.rinv[k];
+    float* zv = fld<float>(A, E.o[0]);
+    const float* r = fld<float>(A, E.o[1]);
+    if (D.d > 1) smooth_first_pair<float, 3>(D, zv, r, (float)E.s, ri, m, y, z);
+    else smooth_first_pair<float, 2>(D, zv, r, (float)E.s, ri, m, y, z);
+}
+
+__global__ __launch_bounds__(256) void kb_smooth_half(const BEntry* tab, int ne, int nbt, char* A, int color,
+                                                      double* part, const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st && st[E.sidx].done) return;
+    int m, y, z;
+    local_xyz(E.G, lb, m, y, z);
+    const Dims& D = E.D;
+    double acc = 0.0;
+    if (m < (D.w >> 1) && y < D.h) {
+        RInv<float> ri;
+        for (int k = 0; k < 8; ++k) ri.v[k] = E.rinv[k];
+        float* zv = fld<float>(A, E.o[0]);
+        const float* r = fld<float>(A, E.o[1]);
+        acc = D.d > 1 ? smooth_half_cell<float, 3>(D, zv, r, (float)E.s, ri, color, m, y, z)
+                      : smooth_half_cell<float, 2>(D, zv, r, (float)E.s, ri, color, m, y, z);
+    }
+    if (part) {
+        double t = block_sum(acc);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part[b_] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_residual(const BEntry* tab, int ne, int nbt, char* A,
+                                                   const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int m, y, z;
+    local_xyz(E.G, lb, m, y, z);
+    const Dims& D = E.D;
+    if (m >= (D.w >> 1) || y >= D.h) return;
+    float* res = fld<float>(A, E.o[2]);
+    const float* zv = fld<float>(A, E.o[0]);
+    const float* r = fld<float>(A, E.o[1]);
+    if (D.d > 1) residual_pair<float, 3>(D, res, zv, r, (float)E.s, m, y, z);
+    else residual_pair<float, 2>(D, res, zv, r, (float)E.s, m, y, z);
+}
+
+// restriction into level l+1 over co_[l]: o[0] = r_{l+1}, o[1] = res (level l); LC = coarse layout
+template <int LC>
+__global__ __launch_bounds__(256) void kb_restrict(const BEntry* tab, int ne, int nbt, char* A,
+                                                   const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int I, J, K;
+    local_xyz(E.G, lb, I, J, K);
+    if (I >= E.Dc.w || J >= E.Dc.h) return;
+    float* rc = fld<float>(A, E.o[0]);
+    const float* res = fld<float>(A, E.o[1]);
+    if (E.D.d > 1) restrict_cell<float, 3, LC>(E.D, E.Dc, rc, res, I, J, K);
+    else restrict_cell<float, 2, LC>(E.D, E.Dc, rc, res, I, J, K);
+}
+
+// prolongation from level l+1 over prl_[l]: o[0] = z_l, o[1] = z_{l+1}
+template <int LC>
+__global__ __launch_bounds__(256) void kb_prolong(const BEntry* tab, int ne, int nbt, char* A,
+                                                  const SolverState* st) {
+    BATCH_ENTRY(tab, ne, nbt);
+    if (st[E.sidx].done) return;
+    int m, y, z;
+    local_xyz(E.G, lb, m, y, z);
+    if (m >= (E.D.w >> 1) || y >= E.D.h) return;
+    float* zf = fld<float>(A, E.o[0]);
+    const float* zc = fld<float>(A, E.o[1]);
+    if (E.D.d > 1) prolong_pair<float, 3, LC>(E.D, E.Dc, zf, zc, m, y, z);
+    else prolong_pair<float, 2, LC>(E.D, E.Dc, zf, zc, m, y, z);
+}
+
+__device__ __forceinline__ int find_fdm(const BFdm* e, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (e[mid].start <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Coarsest level, pass `phase` 0 (forward x,y; a 2D grid solves completely)
+// or 1 (inverse y,x of a 3D grid). One workgroup per z-plane of each grid.
+__global__ __launch_bounds__(256) void kb_fdm_plane(const BFdm* tab, int ne, char* A, int phase,
+                                                    const SolverState* st) {
+    extern __shared__ double fdm_lds[];
+    const BFdm& F = tab[find_fdm(tab, ne, blockIdx.x)];
+    if (st[F.sidx].done) return;
+    int k = blockIdx.x - F.start;
+    int64_t P = (int64_t)F.nx * F.ny;
+    const double eps = 1e-10;
+    if (!F.is3d) {
+        if (phase == 0)
+            fdm_plane_body<float, float>(fdm_lds, F.nx, F.ny, 2, fld<float>(A, F.o_r), fld<float>(A, F.o_z), F.Vx,
+                                         F.Vy, F.lx, F.ly, F.s, eps);
+        return;
+    }
+    if (phase == 0)
+        fdm_plane_body<float, double>(fdm_lds, F.nx, F.ny, 0, fld<float>(A, F.o_r) + k * P,
+                                      fld<double>(A, F.o_t0) + k * P, F.Vx, F.Vy, F.lx, F.ly, F.s, eps);
+    else
+        fdm_plane_body<double, float>(fdm_lds, F.nx, F.ny, 1, fld<double>(A, F.o_t0) + k * P,
+                                      fld<float>(A, F.o_z) + k * P, F.Vx, F.Vy, F.lx, F.ly, F.s, eps);
+}
+
+// Coarsest level of the 3D grids: one workgroup per y-row.
+__global__ __launch_bounds__(256) void kb_fdm_zcol(const BFdm* tab, int ne, char* A, const SolverState* st) {
+    extern __shared__ double fdm_lds[];
+    const BFdm& F = tab[find_fdm(tab, ne, blockIdx.x)];
+    if (st[F.sidx].done) return;
+    fdm_zcol_body(fdm_lds, F.nx, F.ny, F.nz, blockIdx.x - F.start, fld<double>(A, F.o_t0), F.Vz, F.lx, F.ly, F.lz,
+                  F.s, 1e-10);
+}
+
+// ---- host side -------------------------------------------------------------------------------
+
+static inline int64_t align256(int64_t v) { return (v + 255) & ~int64_t(255); }
+
+static Grid3 grid3t(int64_t W, int64_t H, int64_t Dz, int tx, int ty) {
+    Grid3 g;
+    g.gx = (int)((W + tx - 1) / tx);
+    g.gy = (int)((H + ty - 1) / ty);
+    g.gz = (int)Dz;
+    g.nb = g.gx * g.gy * g.gz;
+    return g;
+}
+
+static Dims level_dims(const Dims& D, int l) {
+    Dims L = D;
+    L.w = D.w >> l;
+    L.h = D.h >> l;
+    L.d = D.d > 1 ? D.d >> l : 1;
+    return L;
+}
+
+pano_status hgrid_check(const Dims& D, int levels) {
+    PANO_CHECK(mg_check(D, levels));
+    Dims C = level_dims(D, levels - 1);
+    if (C.w * C.h > kFdmPlane || (D.d > 1 && C.w * C.d > kFdmPlane))
+        return fail(PANO_ERR_BAD_SHAPE, "batched solve: coarsest level %dx%dx%d larger than %d cells per plane", C.d,
+                    C.h, C.w, kFdmPlane);
+    return PANO_OK;
+}
+
+pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base) {
+    const Dims& D = g.D;
+    g.cells = (int64_t)D.d * D.h * D.w;
+    g.faces = face_layout(D).total;
+    int64_t off = base;
+    auto take = [&](int64_t bytes) {
+        int64_t o = off;
+        off = align256(off + bytes);
+        return o;
+    };
+    for (int i = 0; i < 2; ++i) g.o_rho[i] = take(g.cells * 4);
+    for (int i = 0; i < 2; ++i) g.o_u[i] = take(g.faces * 4);
+    g.o_x = take(g.cells * 4);
+    g.o_r = take(g.cells * 4);
+    g.o_z = take(g.cells * 4);
+    g.o_p = take(g.cells * 4);
+    g.o_res = take(g.cells * 4);
+    g.o_zl[0] = g.o_z;
+    g.o_rl[0] = g.o_r;
+    for (int l = 1; l < g.levels; ++l) {
+        Dims L = level_dims(D, l);
+        int64_t n = (int64_t)L.d * L.h * L.w;
+        g.o_zl[l] = take(n * 4);
+        g.o_rl[l] = take(n * 4);
+    }
+    Dims C = level_dims(D, g.levels - 1);
+    g.o_t0 = take((int64_t)C.d * C.h * C.w * 8);
+    g.bytes = off - base;
+    PANO_CHECK(axis_basis(c, C.w, C.bc & 1, (C.bc >> 1) & 1, (double**)&g.Vx, (double**)&g.lx));
+    PANO_CHECK(axis_basis(c, C.h, (C.bc >> 2) & 1, (C.bc >> 3) & 1, (double**)&g.Vy, (double**)&g.ly));
+    if (C.d > 1) {
+        PANO_CHECK(axis_basis(c, C.d, (C.bc >> 4) & 1, (C.bc >> 5) & 1, (double**)&g.Vz, (double**)&g.lz));
+    } else {
+        g.Vz = nullptr;
+        g.lz = nullptr;
+    }
+    return PANO_OK;
+}
+
+// Grows a device buffer owned by the batch (contents not kept). Launches
+// already enqueued that read the old buffer finish first.
+static pano_status grow(Ctx* c, void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return PANO_OK;
+    if (*p) {
+        PANO_HIP(hipStreamSynchronize(c->stream));
+        PANO_HIP(hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(PANO_ERR_OOM, "batch tables: hipMalloc(%zu) failed", bytes);
+    }
+    *cap = bytes;
+    return PANO_OK;
+}
+
+BatchCore::~BatchCore() {
+    if (c) (void)hipStreamSynchronize(c->stream);
+    auto rel = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    rel(nat_.d), rel(fx_.d), rel(fy_.d), rel(fz_.d), rel(flat_.d);
+    for (int p = 0; p < 2; ++p) rel(div_[p].d), rel(prx_[p].d), rel(pry_[p].d), rel(prz_[p].d);
+    for (int l = 0; l < 16; ++l) rel(rb_[l].d), rel(co_[l].d), rel(prl_[l].d);
+    rel(plane_.d), rel(zcol_.d);
+    rel(states), rel(cfgs), rel(part0_), rel(part1_), rel(part2_);
+}
+
+pano_status BatchCore::ensure_slots(int n) {
+    if (n <= nslots) return PANO_OK;
+    PANO_CHECK(grow(c, (void**)&states, &slot_cap_[0], (size_t)n * sizeof(SolverState)));
+    PANO_CHECK(grow(c, (void**)&cfgs, &slot_cap_[1], (size_t)n * sizeof(SolveCfg)));
+    nslots = n;
+    return PANO_OK;
+}
+
+pano_status BatchCore::upload(Table& t) {
+    t.nbt = 0;
+    for (auto& e : t.h) t.nbt = std::max(t.nbt, e.start + e.G.nb);
+    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, std::max<size_t>(1, t.h.size()) * sizeof(BEntry)));
+    if (!t.h.empty())
+        PANO_HIP(hipMemcpyAsync(t.d, t.h.data(), t.h.size() * sizeof(BEntry), hipMemcpyHostToDevice, c->stream));
+    return PANO_OK;
+}
+
+pano_status BatchCore::upload(FdmTable& t) {
+    t.nbt = 0;
+    for (auto& e : t.h) t.nbt = std::max(t.nbt, e.start + e.nblk);
+    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, std::max<size_t>(1, t.h.size()) * sizeof(BFdm)));
+    if (!t.h.empty())
+        PANO_HIP(hipMemcpyAsync(t.d, t.h.data(), t.h.size() * sizeof(BFdm), hipMemcpyHostToDevice, c->stream));
+    return PANO_OK;
+}
+
+pano_status BatchCore::ensure_partials(size_t n) {
+    if (n <= part_cap_) return PANO_OK;
+    PANO_CHECK(grow(c, (void**)&part0_, &part_bytes_[0], n * sizeof(double)));
+    PANO_CHECK(grow(c, (void**)&part1_, &part_bytes_[1], n * sizeof(double)));
+    PANO_CHECK(grow(c, (void**)&part2_, &part_bytes_[2], n * sizeof(double)));
+    part_cap_ = n;
+    return PANO_OK;
+}
+
+pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<int>& ids) {
+    auto reset = [](Table& t) { t.h.clear(); };
+    reset(nat_);
+    reset(fx_);
+    reset(fy_);
+    reset(fz_);
+    reset(flat_);
+    for (int p = 0; p < 2; ++p) {
+        reset(div_[p]);
+        reset(prx_[p]);
+        reset(pry_[p]);
+        reset(prz_[p]);
+    }
+    for (int l = 0; l < 16; ++l) {
+        reset(rb_[l]);
+        reset(co_[l]);
+        reset(prl_[l]);
+    }
+    plane_.h.clear();
+    zcol_.h.clear();
+    levels_ = ids.empty() ? 0 : grids[ids[0]].levels;
+    any3d_ = false;
+    int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
+    size_t maxp = 0;
+    for (int id : ids) {
+        const HGrid& g = grids[id];
+        if (g.levels != levels_) return fail(PANO_ERR_BAD_SHAPE, "all grids of a batch need the same MG levels");
+        const Dims& D = g.D;
+        any3d_ |= D.d > 1;
+        BEntry e;
+        memset(&e, 0, sizeof(e));
+        e.D = D;
+        e.sidx = g.sidx;
+        e.s = g.dt;
+        // natural cells: rho ping/pong, u ping/pong (advection); r, u (divergence)
+        e.G = grid3t(D.w, D.h, D.d, shapes.nat_x, shapes.nat_y);
+        e.start = snat;
+        snat += e.G.nb;
+        e.o[0] = g.o_rho[0];
+        e.o[1] = g.o_rho[1];
+        e.o[2] = g.o_u[0];
+        e.o[3] = g.o_u[1];
+        nat_.h.push_back(e);
+        // faces (u ping/pong for advection; u pong and p for the projection are set per call)
+        BEntry f = e;
+        f.o[0] = g.o_u[0];
+        f.o[1] = g.o_u[1];
+        f.o[2] = g.o_x;
+        f.G = grid3t(D.w + 1, D.h, D.d, shapes.nat_x, shapes.nat_y);
+        f.start = sfx;
+        sfx += f.G.nb;
+        fx_.h.push_back(f);
+        f.G = grid3t(D.w, D.h + 1, D.d, shapes.nat_x, shapes.nat_y);
+        f.start = sfy;
+        sfy += f.G.nb;
+        fy_.h.push_back(f);
+        if (D.d > 1) {
+            f.G = grid3t(D.w, D.h, D.d + 1, shapes.nat_x, shapes.nat_y);
+            f.start = sfz;
+            sfz += f.G.nb;
+            fz_.h.push_back(f);
+        }
+        for (int p = 0; p < 2; ++p) {
+            BEntry dv = e;  // same tiles as the natural table
+            dv.o[0] = g.o_r;
+            dv.o[1] = g.o_u[p];
+            div_[p].h.push_back(dv);
+            BEntry px = fx_.h.back(), py = fy_.h.back();
+            px.o[0] = py.o[0] = g.o_u[p];
+            px.o[1] = py.o[1] = g.o_x;
+            prx_[p].h.push_back(px);
+            pry_[p].h.push_back(py);
+            if (D.d > 1) {
+                BEntry pz = fz_.h.back();
+                pz.o[0] = g.o_u[p];
+                pz.o[1] = g.o_x;
+                prz_[p].h.push_back(pz);
+            }
+        }
+        // flat vectors
+        BEntry v = e;
+        v.G.gx = (int)((g.cells + kFlatPer - 1) / kFlatPer);
+        v.G.gy = v.G.gz = 1;
+        v.G.nb = v.G.gx;
+        v.start = sflat;
+        sflat += v.G.nb;
+        v.o[0] = g.o_x;
+        v.o[1] = g.o_r;
+        v.o[2] = g.o_z;
+        v.o[3] = g.o_p;
+        flat_.h.push_back(v);
+        // multigrid levels
+        double s = g.dt;
+        for (int l = 0; l < g.levels; ++l) {
+            Dims L = level_dims(D, l);
+            if (l < g.levels - 1) {
+                BEntry r = e;
+                r.D = L;
+                r.Dc = level_dims(D, l + 1);
+                r.s = (double)(float)s;
+                RInv<float> ri = make_rinv<float>((float)s);
+                for (int k = 0; k < 8; ++k) r.rinv[k] = ri.v[k];
+                r.G = grid3t(L.w >> 1, L.h, L.d, shapes.rb_x, shapes.rb_y);
+                r.start = srb[l];
+                srb[l] += r.G.nb;
+                r.o[0] = g.o_zl[l];
+                r.o[1] = g.o_rl[l];
+                r.o[2] = g.o_res;
+                r.o[3] = g.o_p;
+                rb_[l].h.push_back(r);
+                BEntry pr = r;
+                pr.o[0] = g.o_zl[l];
+                pr.o[1] = g.o_zl[l + 1];
+                prl_[l].h.push_back(pr);
+                BEntry co = r;
+                co.G = grid3t(r.Dc.w, r.Dc.h, r.Dc.d, shapes.nat_x, shapes.nat_y);
+                co.start = sco[l];
+                sco[l] += co.G.nb;
+                co.o[0] = g.o_rl[l + 1];
+                co.o[1] = g.o_res;
+                co_[l].h.push_back(co);
+            } else {
+                BFdm F;
+                F.sidx = g.sidx;
+                F.is3d = D.d > 1;
+                F.nx = L.w;
+                F.ny = L.h;
+                F.nz = L.d;
+                F.Vx = g.Vx;
+                F.Vy = g.Vy;
+                F.Vz = g.Vz;
+                F.lx = g.lx;
+                F.ly = g.ly;
+                F.lz = g.lz;
+                F.s = s;
+                F.o_r = g.o_rl[l];
+                F.o_z = g.o_zl[l];
+                F.o_t0 = g.o_t0;
+                F.start = splane;
+                F.nblk = L.d;
+                splane += F.nblk;
+                plane_.h.push_back(F);
+                if (D.d > 1) {
+                    BFdm Z = F;
+                    Z.start = szcol;
+                    Z.nblk = L.h;
+                    szcol += Z.nblk;
+                    zcol_.h.push_back(Z);
+                }
+            }
+            s *= 0.25;
+        }
+        maxp = std::max(maxp, (size_t)std::max({snat, sflat, srb[0]}));
+    }
+    fdm_lds_ = 2 * sizeof(double) * (size_t)kFdmPlane;
+    PANO_CHECK(upload(nat_));
+    PANO_CHECK(upload(fx_));
+    PANO_CHECK(upload(fy_));
+    PANO_CHECK(upload(fz_));
+    PANO_CHECK(upload(flat_));
+    for (int p = 0; p < 2; ++p) {
+        PANO_CHECK(upload(div_[p]));
+        PANO_CHECK(upload(prx_[p]));
+        PANO_CHECK(upload(pry_[p]));
+        PANO_CHECK(upload(prz_[p]));
+    }
+    for (int l = 0; l + 1 < levels_; ++l) {
+        PANO_CHECK(upload(rb_[l]));
+        PANO_CHECK(upload(co_[l]));
+        PANO_CHECK(upload(prl_[l]));
+    }
+    PANO_CHECK(upload(plane_));
+    PANO_CHECK(upload(zcol_));
+    PANO_CHECK(ensure_partials(std::max<size_t>(maxp, 1)));
+    return PANO_OK;
+}
+
+#define BL(kern, T_, ...)                                                                            \
+    do {                                                                                              \
+        if ((T_).nbt > 0)                                                                             \
+            hipLaunchKernelGGL(kern, dim3((T_).nbt), dim3(blk.x, blk.y), 0, c->stream, (T_).d,         \
+                               (int)(T_).h.size(), (T_).nbt, __VA_ARGS__);                            \
+    } while (0)
+
+void BatchCore::fin(int op, const double* p0, const double* p1, const Table& t, int use_max) {
+    if (t.h.empty()) return;
+    hipLaunchKernelGGL(kb_finalize, dim3((unsigned)t.h.size()), dim3(1024), 0, c->stream, op, (const BEntry*)t.d,
+                       p0, p1, states, (const SolveCfg*)cfgs, use_max);
+}
+
+pano_status BatchCore::enqueue_advect(int ping) {
+    dim3 blk(shapes.nat_x, shapes.nat_y);
+    BL(kb_advect_cell, nat_, arena, ping);
+    BL(kb_advect_face, fx_, arena, ping, 0);
+    BL(kb_advect_face, fy_, arena, ping, 1);
+    BL(kb_advect_face, fz_, arena, ping, 2);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+pano_status BatchCore::enqueue_divergence(int ping) {
+    dim3 blk(shapes.nat_x, shapes.nat_y);
+    BL(kb_neg_div, div_[ping], arena);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+pano_status BatchCore::enqueue_project(int ping) {
+    dim3 blk(shapes.nat_x, shapes.nat_y);
+    BL(kb_project, prx_[ping], arena, 0);
+    BL(kb_project, pry_[ping], arena, 1);
+    BL(kb_project, prz_[ping], arena, 2);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+// z = M r at level 0 (one V-cycle from zero); with_dot: the last two level-0
+// half-sweeps write the partials of z.r (black into part0_, red into part2_).
+pano_status BatchCore::vcycle(bool with_dot) {
+    const int Lc = levels_ - 1;
+    dim3 blk(shapes.rb_x, shapes.rb_y);
+    for (int l = 0; l < Lc; ++l) {
+        BL(kb_smooth_first, rb_[l], arena, (const SolverState*)states);
+        for (int k = 1; k < nu; ++k) {
+            BL(kb_smooth_half, rb_[l], arena, 0, (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rb_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
+        }
+        BL(kb_residual, rb_[l], arena, (const SolverState*)states);
+        dim3 sblk = blk;
+        blk = dim3(shapes.nat_x, shapes.nat_y);
+        if (l + 1 == Lc) BL(kb_restrict<NATURAL>, co_[l], arena, (const SolverState*)states);
+        else BL(kb_restrict<REDBLACK>, co_[l], arena, (const SolverState*)states);
+        blk = sblk;
+    }
+    if (!plane_.h.empty()) {
+        hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream, (const BFdm*)plane_.d,
+                           (int)plane_.h.size(), arena, 0, (const SolverState*)states);
+        if (!zcol_.h.empty()) {
+            hipLaunchKernelGGL(kb_fdm_zcol, dim3(zcol_.nbt), dim3(256), fdm_lds_, c->stream, (const BFdm*)zcol_.d,
+                               (int)zcol_.h.size(), arena, (const SolverState*)states);
+            hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream,
+                               (const BFdm*)plane_.d, (int)plane_.h.size(), arena, 1, (const SolverState*)states);
+        }
+    }
+    for (int l = Lc - 1; l >= 0; --l) {
+        if (l + 1 == Lc) BL(kb_prolong<NATURAL>, prl_[l], arena, (const SolverState*)states);
+        else BL(kb_prolong<REDBLACK>, prl_[l], arena, (const SolverState*)states);
+        for (int k = 0; k < nu; ++k) {
+            bool last = l == 0 && k == nu - 1 && with_dot;
+            BL(kb_smooth_half, rb_[l], arena, 1, last ? part0_ : (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rb_[l], arena, 0, last ? part2_ : (double*)nullptr, (const SolverState*)states);
+        }
+    }
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+// CG init for r already in place (pcg.rs:32-46): x = 0, mean removal,
+// threshold from ||b||_inf, z = M r, p = z, sigma = z.r.
+pano_status BatchCore::enqueue_init() {
+    if (flat_.h.empty()) return PANO_OK;
+    hipLaunchKernelGGL(kb_state_init, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream, (const BEntry*)flat_.d,
+                       states, max_iterations);
+    dim3 blk(kBlock, 1);
+    BL(kb_zero_stats, flat_, arena, part0_, part1_);
+    fin(FIN_STATS, part0_, nullptr, flat_, 0);
+    BL(kb_sub_mean, flat_, arena, (const SolverState*)states, part1_);
+    fin(FIN_THRESH, part1_, nullptr, flat_, 1);
+    hipLaunchKernelGGL(kb_round_threshold<float>, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream,
+                       (const BEntry*)flat_.d, states);
+    PANO_CHECK(vcycle(false));
+    blk = dim3(kBlock, 1);
+    BL(kb_copy_dot, flat_, arena, (const SolverState*)states, part0_);
+    fin(FIN_SIGMA0, part0_, nullptr, flat_, 0);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+// One CG iteration (pcg.rs:51-77) for every grid not yet converged.
+pano_status BatchCore::enqueue_iteration() {
+    if (flat_.h.empty()) return PANO_OK;
+    dim3 blk(shapes.rb_x, shapes.rb_y);
+    BL(kb_apply, rb_[0], arena, (const SolverState*)states, part0_);
+    fin(FIN_ALPHA, part0_, nullptr, rb_[0], 0);
+    blk = dim3(kBlock, 1);
+    BL(kb_update, flat_, arena, (const SolverState*)states, part1_);
+    fin(FIN_RNORM, part1_, nullptr, flat_, 1);
+    PANO_CHECK(vcycle(true));
+    fin(FIN_BETA, part0_, part2_, rb_[0], 0);
+    blk = dim3(kBlock, 1);
+    BL(kb_search, flat_, arena, (const SolverState*)states);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+pano_status BatchCore::tune_smooth() {
+    dim3 blk(shapes.rb_x, shapes.rb_y);
+    BL(kb_smooth_half, rb_[0], arena, 0, (double*)nullptr, (const SolverState*)nullptr);
+    BL(kb_smooth_half, rb_[0], arena, 1, (double*)nullptr, (const SolverState*)nullptr);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+// ---- the static batch: fixed grids, hipGraph-captured phases, autotuned tiles ----------------
+
+struct Batch {
+    Ctx* c = nullptr;
+    BatchCore core;
+    std::vector<HGrid> grids;
+    std::vector<int> ids;
+    void* arena = nullptr;
+    int64_t arena_bytes = 0;
+    int ping = 0;
+    bool use_graph = true;
+    hipGraphExec_t pre[2] = {nullptr, nullptr}, iter = nullptr, post[2] = {nullptr, nullptr};
+    std::vector<SolverState> hstates;
+    int hint = 0;
+    double tune_us[2][8];
+    int tune_n = 0;
+};
+
+static pano_status capture(Ctx* c, hipGraphExec_t* out, const std::function<pano_status()>& body) {
+    hipGraph_t g = nullptr;
+    PANO_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    pano_status s = body();
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (s < 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    if (e != hipSuccess) return fail(PANO_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(PANO_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return PANO_OK;
+}
+
+static void destroy_graphs(Batch* B) {
+    for (int p = 0; p < 2; ++p) {
+        if (B->pre[p]) (void)hipGraphExecDestroy(B->pre[p]);
+        if (B->post[p]) (void)hipGraphExecDestroy(B->post[p]);
+        B->pre[p] = B->post[p] = nullptr;
+    }
+    if (B->iter) (void)hipGraphExecDestroy(B->iter);
+    B->iter = nullptr;
+}
+
+static pano_status phase_pre(Batch* B, int p) {
+    PANO_CHECK(B->core.enqueue_advect(p));
+    PANO_CHECK(B->core.enqueue_divergence(1 - p));
+    return B->core.enqueue_init();
+}
+
+static pano_status build_graphs(Batch* B) {
+    destroy_graphs(B);
+    if (!B->use_graph) return PANO_OK;
+    for (int p = 0; p < 2; ++p) {
+        PANO_CHECK(capture(B->c, &B->pre[p], [&]() { return phase_pre(B, p); }));
+        PANO_CHECK(capture(B->c, &B->post[p], [&]() { return B->core.enqueue_project(1 - p); }));
+    }
+    PANO_CHECK(capture(B->c, &B->iter, [&]() { return B->core.enqueue_iteration(); }));
+    return PANO_OK;
+}
+
+// Times one launch family for a candidate tile shape: 5 replays of the
+// level-0 half-sweep (red-black family) or of the density advection and
+// projection (natural family), measured with HIP events. The kernels run
+// without solver states, on whatever the fields hold.
+static pano_status time_shape(Batch* B, int family, double* us) {
+    Ctx* c = B->c;
+    PANO_CHECK(B->core.build(B->grids, B->ids));
+    hipEvent_t e0, e1;
+    PANO_HIP(hipEventCreate(&e0));
+    PANO_HIP(hipEventCreate(&e1));
+    BatchCore& K = B->core;
+    auto run = [&]() -> pano_status {
+        if (family == 0) {
+            PANO_CHECK(K.enqueue_advect(0));
+            PANO_CHECK(K.enqueue_project(1));
+        } else {
+            PANO_CHECK(K.tune_smooth());
+        }
+        return PANO_OK;
+    };
+    PANO_CHECK(run());  // warm
+    PANO_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < 5; ++i) PANO_CHECK(run());
+    PANO_HIP(hipEventRecord(e1, c->stream));
+    PANO_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    PANO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *us = ms * 1e3 / 5.0;
+    return PANO_OK;
+}
+
+static const int kShapes[4][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16}};
+
+static pano_status autotune(Batch* B) {
+    B->tune_n = 4;
+    for (int fam = 0; fam < 2; ++fam) {
+        int best = 0;
+        double best_us = 1e300;
+        for (int k = 0; k < 4; ++k) {
+            if (fam == 0) {
+                B->core.shapes.nat_x = kShapes[k][0];
+                B->core.shapes.nat_y = kShapes[k][1];
+            } else {
+                B->core.shapes.rb_x = kShapes[k][0];
+                B->core.shapes.rb_y = kShapes[k][1];
+            }
+            double us;
+            PANO_CHECK(time_shape(B, fam, &us));
+            B->tune_us[fam][k] = us;
+            if (us < best_us) {
+                best_us = us;
+                best = k;
+            }
+        }
+        if (fam == 0) {
+            B->core.shapes.nat_x = kShapes[best][0];
+            B->core.shapes.nat_y = kShapes[best][1];
+        } else {
+            B->core.shapes.rb_x = kShapes[best][0];
+            B->core.shapes.rb_y = kShapes[best][1];
+        }
+    }
+    // the timing ran kernels over the uninitialised fields: clear them
+    PANO_CHECK(fill_zero(B->c, B->arena, (size_t)B->arena_bytes));
+    return PANO_OK;
+}
+
+}  // namespace pano
+
+using namespace pano;
+
+struct pano_batch_s {
+    Batch b;
+};
+
+extern "C" {
+
+pano_status pano_batch_create(pano_ctx ctx, const pano_batch_grid* grids, int32_t n, const pano_batch_params* P,
+                              pano_batch* out) {
+    Ctx* c = ctx_of(ctx);
+    if (!c || !grids || !P || !out || n < 1) return fail(PANO_ERR_INVALID, "null argument or empty batch");
+    *out = nullptr;
+    PANO_HIP(hipSetDevice(c->device));
+    if (P->mg_levels < 2 || P->mg_nu < 1 || !(P->tol > 0) || P->max_iterations < 1)
+        return fail(PANO_ERR_INVALID, "bad batch solver parameters");
+    auto* h = new pano_batch_s();
+    Batch* B = &h->b;
+    B->c = c;
+    B->core.c = c;
+    B->core.nu = P->mg_nu;
+    B->core.max_iterations = P->max_iterations;
+    B->core.tol = P->tol;
+    B->use_graph = P->use_graph != 0;
+    if (!P->autotune) {
+        int a[4] = {P->tile_nat[0], P->tile_nat[1], P->tile_rb[0], P->tile_rb[1]};
+        if (a[0] * a[1] != 256 || a[2] * a[3] != 256 || a[0] < 1 || a[2] < 1) {
+            delete h;
+            return fail(PANO_ERR_INVALID, "tile shapes must have 256 threads");
+        }
+        B->core.shapes.nat_x = a[0];
+        B->core.shapes.nat_y = a[1];
+        B->core.shapes.rb_x = a[2];
+        B->core.shapes.rb_y = a[3];
+    }
+    int64_t off = 0;
+    std::vector<SolveCfg> cfg(n);
+    for (int i = 0; i < n; ++i) {
+        const pano_batch_grid& q = grids[i];
+        pano_grid g;
+        g.ndim = q.ndim;
+        g.dtype = PANO_F32;
+        g.nz = q.ndim == 3 ? q.nz : 1;
+        g.ny = q.ny;
+        g.nx = q.nx;
+        for (int k = 0; k < 6; ++k) g.bc[k] = q.bc[k];
+        pano_status s = check_grid(&g);
+        if (s == PANO_OK) s = hgrid_check(dims_of(&g), P->mg_levels);
+        if (s != PANO_OK || !(q.dt > 0)) {
+            delete h;
+            return s != PANO_OK ? s : fail(PANO_ERR_INVALID, "grid %d: dt must be > 0", i);
+        }
+        HGrid hg;
+        memset(&hg, 0, sizeof(hg));
+        hg.D = dims_of(&g);
+        hg.dt = q.dt;
+        hg.levels = P->mg_levels;
+        hg.sidx = i;
+        s = hgrid_layout(c, hg, off);
+        if (s != PANO_OK) {
+            delete h;
+            return s;
+        }
+        off += hg.bytes;
+        B->grids.push_back(hg);
+        B->ids.push_back(i);
+        cfg[i].tol_kind = PANO_TOL_REL_LINF;
+        cfg[i].tol = P->tol;
+        cfg[i].n = hg.cells;
+        cfg[i].remove_mean = hg.D.bc == 0;
+    }
+    B->arena_bytes = off;
+    if (hipMalloc(&B->arena, (size_t)off) != hipSuccess) {
+        (void)hipGetLastError();
+        delete h;
+        return fail(PANO_ERR_OOM, "batch arena of %lld bytes", (long long)off);
+    }
+    B->core.arena = (char*)B->arena;
+    pano_status s = fill_zero(c, B->arena, (size_t)off);
+    if (s == PANO_OK) s = B->core.ensure_slots(n);
+    if (s == PANO_OK) {
+        hipError_t e = hipMemcpyAsync(B->core.cfgs, cfg.data(), n * sizeof(SolveCfg), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) s = fail(PANO_ERR_DEVICE, "cfg upload: %s", hipGetErrorString(e));
+    }
+    if (s == PANO_OK && P->autotune) s = autotune(B);
+    if (s == PANO_OK) s = B->core.build(B->grids, B->ids);
+    if (s == PANO_OK) s = build_graphs(B);
+    if (s == PANO_OK && hipStreamSynchronize(c->stream) != hipSuccess) s = fail(PANO_ERR_DEVICE, "batch setup");
+    if (s != PANO_OK) {
+        destroy_graphs(B);
+        (void)hipFree(B->arena);
+        delete h;
+        return s;
+    }
+    B->hstates.resize(n);
+    *out = h;
+    return PANO_OK;
+}
+
+pano_status pano_batch_destroy(pano_batch h) {
+    if (!h) return PANO_OK;
+    Batch* B = &h->b;
+    (void)hipStreamSynchronize(B->c->stream);
+    destroy_graphs(B);
+    if (B->arena) (void)hipFree(B->arena);
+    delete h;
+    return PANO_OK;
+}
+
+static pano_status batch_io(pano_batch h, int32_t i, float* rho, float* faces, bool up) {
+    if (!h) return fail(PANO_ERR_INVALID, "batch is null");
+    Batch* B = &h->b;
+    if (i < 0 || i >= (int)B->grids.size()) return fail(PANO_ERR_INVALID, "grid index %d out of range", i);
+    const HGrid& g = B->grids[i];
+    hipStream_t s = B->c->stream;
+    char* A = (char*)B->arena;
+    int cur = B->ping;
+    if (rho) {
+        if (up) PANO_HIP(hipMemcpyAsync(A + g.o_rho[cur], rho, g.cells * 4, hipMemcpyHostToDevice, s));
+        else PANO_HIP(hipMemcpyAsync(rho, A + g.o_rho[cur], g.cells * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (faces) {
+        if (up) PANO_HIP(hipMemcpyAsync(A + g.o_u[cur], faces, g.faces * 4, hipMemcpyHostToDevice, s));
+        else PANO_HIP(hipMemcpyAsync(faces, A + g.o_u[cur], g.faces * 4, hipMemcpyDeviceToHost, s));
+    }
+    PANO_HIP(hipStreamSynchronize(s));
+    return PANO_OK;
+}
+
+pano_status pano_batch_set_state(pano_batch h, int32_t i, const float* density, const float* faces) {
+    return batch_io(h, i, (float*)density, (float*)faces, true);
+}
+
+pano_status pano_batch_get_state(pano_batch h, int32_t i, float* density, float* faces) {
+    return batch_io(h, i, density, faces, false);
+}
+
+pano_status pano_batch_step(pano_batch h, int32_t* iterations) {
+    if (!h) return fail(PANO_ERR_INVALID, "batch is null");
+    Batch* B = &h->b;
+    Ctx* c = B->c;
+    PANO_HIP(hipSetDevice(c->device));
+    const int p = B->ping;
+    const int n = (int)B->grids.size();
+    auto iterate = [&]() -> pano_status {
+        if (B->use_graph) PANO_HIP(hipGraphLaunch(B->iter, c->stream));
+        else PANO_CHECK(B->core.enqueue_iteration());
+        return PANO_OK;
+    };
+    if (B->use_graph) PANO_HIP(hipGraphLaunch(B->pre[p], c->stream));
+    else PANO_CHECK(phase_pre(B, p));
+    int launched = 0, first = B->hint > 0 ? B->hint : 4;
+    for (int k = 0; k < first && launched < B->core.max_iterations; ++k, ++launched) PANO_CHECK(iterate());
+    int maxit = 0;
+    for (;;) {
+        PANO_HIP(hipMemcpyAsync(B->hstates.data(), B->core.states, n * sizeof(SolverState), hipMemcpyDeviceToHost,
+                                c->stream));
+        PANO_HIP(hipStreamSynchronize(c->stream));
+        bool all = true;
+        maxit = 0;
+        for (int i = 0; i < n; ++i) {
+            all = all && B->hstates[i].done;
+            maxit = std::max(maxit, (int)B->hstates[i].iter);
+        }
+        if (all || launched >= B->core.max_iterations) break;
+        PANO_CHECK(iterate());
+        ++launched;
+    }
+    if (B->use_graph) PANO_HIP(hipGraphLaunch(B->post[p], c->stream));
+    else PANO_CHECK(B->core.enqueue_project(1 - p));
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    B->hint = maxit;
+    B->ping = 1 - p;
+    bool bad = false, conv = true;
+    for (int i = 0; i < n; ++i) {
+        if (iterations) iterations[i] = B->hstates[i].iter;
+        bad = bad || B->hstates[i].nonfinite;
+        conv = conv && B->hstates[i].converged;
+    }
+    if (bad) return fail(PANO_ERR_NONFINITE, "non-finite value in a batched solve");
+    return conv ? PANO_OK : PANO_NOT_CONVERGED;
+}
+
+pano_status pano_batch_tiles(pano_batch h, int32_t* tile_nat, int32_t* tile_rb, double* tune_us) {
+    if (!h) return fail(PANO_ERR_INVALID, "batch is null");
+    Batch* B = &h->b;
+    if (tile_nat) {
+        tile_nat[0] = B->core.shapes.nat_x;
+        tile_nat[1] = B->core.shapes.nat_y;
+    }
+    if (tile_rb) {
+        tile_rb[0] = B->core.shapes.rb_x;
+        tile_rb[1] = B->core.shapes.rb_y;
+    }
+    if (tune_us)
+        for (int f = 0; f < 2; ++f)
+            for (int k = 0; k < 4; ++k) tune_us[f * 4 + k] = B->tune_n ? B->tune_us[f][k] : 0.0;
+    return PANO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// Batched engine: many independent grids (2D or 3D, uneven sizes) advanced
+// by one set of launches. Every launch covers the tiles of all grids of the
+// batch; a workgroup finds its grid in the launch's entry table (sorted
+// block ranges), so the per-element arithmetic, the tile decomposition of a
+// grid and the order of its reductions are those of the grid alone: a
+// grid's result does not depend on what else is in the batch.
+//
+// Used by the static batch (pano_batch_*, hipGraph-captured steps, tile
+// shapes from an autotuner) and by the serving loop (pano_server_*, dynamic
+// membership).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "mg.hpp"
+#include "mg_device.hpp"
+#include "solver_device.hpp"
+
+namespace pano {
+
+// One grid's entry in a batched launch.
+struct BEntry {
+    Dims D;          // dims at the launch's level (bc included)
+    Dims Dc;         // coarse dims (restriction / prolongation)
+    Grid3 G;         // this grid's tile grid for the launch (G.nb blocks)
+    int start;       // first global block of this grid in the launch
+    int sidx;        // index of the grid's solver state
+    int64_t o[4];    // byte offsets in the arena of the fields the launch touches
+    double s;        // dt, or the level scale dt/4^l
+    float rinv[8];   // smoother reciprocals 1/(s*nc) at this level
+};
+
+// One grid's coarsest-level solve (fast diagonalisation).
+struct BFdm {
+    int start;       // first global block
+    int nblk;        // blocks of this grid (nz planes or ny rows)
+    int sidx;
+    int is3d;        // 3D grid (its coarsest level may have a single plane)
+    int nx, ny, nz;
+    const double *Vx, *Vy, *Vz, *lx, *ly, *lz;
+    double s;
+    int64_t o_r, o_z, o_t0;  // arena byte offsets: coarse rhs (f32), solution (f32), FP64 temp
+};
+
+// Host description of one grid of the batch and where its fields live.
+struct HGrid {
+    Dims D;
+    double dt;
+    int levels;
+    int sidx;          // state index (slot)
+    int64_t cells;     // level-0 cell count
+    int64_t faces;
+    // arena byte offsets (f32 fields unless noted)
+    int64_t o_rho[2], o_u[2];
+    int64_t o_x, o_r, o_z, o_p, o_res;
+    int64_t o_zl[16], o_rl[16];
+    int64_t o_t0;      // FP64 coarse temp
+    int64_t bytes;     // total bytes of this grid
+    const double *Vx, *Vy, *Vz, *lx, *ly, *lz;  // coarsest eigenbases
+};
+
+// Tile shapes (threads per workgroup along x / y) of the two launch families.
+struct TileShapes {
+    int nat_x = 64, nat_y = 4;  // natural-layout cell and face tiles
+    int rb_x = 64, rb_y = 4;    // red-black x-pair tiles
+};
+
+// Byte size of a grid's fields and their offsets from `base`.
+pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base);
+
+// Checks a grid for the batched solver (f32, multigrid with `levels`).
+pano_status hgrid_check(const Dims& D, int levels);
+
+class BatchCore {
+public:
+    Ctx* c = nullptr;
+    char* arena = nullptr;     // base of all grid fields
+    TileShapes shapes;
+    int nu = 2;
+    int max_iterations = 500;
+    double tol = 1e-6;
+    SolverState* states = nullptr;  // device, one per slot
+    SolveCfg* cfgs = nullptr;       // device, one per slot
+    int nslots = 0;
+
+    BatchCore() = default;
+    BatchCore(const BatchCore&) = delete;
+    BatchCore& operator=(const BatchCore&) = delete;
+    ~BatchCore();
+
+    // Builds the launch tables of the grid subset `ids` (indices into `grids`).
+    pano_status build(const std::vector<HGrid>& grids, const std::vector<int>& ids);
+    // Enqueue the phases for the grids the tables were built for.
+    pano_status enqueue_advect(int ping);         // density and velocity advection (2D/3D, no source)
+    pano_status enqueue_divergence(int ping);     // r = -div u (u = velocity buffer `ping`)
+    pano_status enqueue_init();                   // CG init: x = 0, stats, threshold, z = M r, p = z, sigma
+    pano_status enqueue_iteration();              // one CG iteration
+    pano_status enqueue_project(int ping);        // u += dt d0*(x), closed walls zeroed
+    pano_status ensure_slots(int n);
+    // Level-0 red-black sweeps without solver states (tile-shape timing only).
+    pano_status tune_smooth();
+
+private:
+    struct Table {
+        std::vector<BEntry> h;
+        BEntry* d = nullptr;
+        int nbt = 0;   // total blocks
+        size_t cap = 0;
+    };
+    struct FdmTable {
+        std::vector<BFdm> h;
+        BFdm* d = nullptr;
+        int nbt = 0;
+        size_t cap = 0;
+    };
+    Table nat_;            // level-0 cells, natural tiles (advection)
+    Table fx_, fy_, fz_;   // face tiles per component (advection)
+    Table div_[2];         // natural tiles: o[0] = r, o[1] = u[ping]
+    Table prx_[2], pry_[2], prz_[2];  // face tiles of the projection: o[0] = u[ping], o[1] = x
+    Table flat_;           // flat vector blocks (2048 cells per block)
+    Table rb_[16];         // red-black pair tiles per level
+    Table co_[16];         // coarse-cell tiles of the restriction into level l+1
+    Table prl_[16];        // pair tiles of level l for the prolongation from l+1
+    FdmTable plane_, zcol_;
+    int levels_ = 0;
+    bool any3d_ = false;
+    size_t fdm_lds_ = 0;
+    double* part0_ = nullptr;
+    double* part1_ = nullptr;
+    double* part2_ = nullptr;
+    size_t part_cap_ = 0;
+    size_t slot_cap_[2] = {0, 0};
+    size_t part_bytes_[3] = {0, 0, 0};
+
+    pano_status upload(Table& t);
+    pano_status upload(FdmTable& t);
+    pano_status ensure_partials(size_t n);
+    void fin(int op, const double* p0, const double* p1, const Table& t, int use_max);
+    pano_status vcycle(bool with_dot);
+};
+
+}  // namespace pano

@@ -22,6 +22,10 @@ struct MG {
     int coarse_n[3] = {1, 1, 1};                         // coarsest extents x, y, z
 };
 
+// Device copy of the eigenbasis of the 1D operator along an axis of n cells
+// (lo/hi = 1 for an open side), cached in the context.
+pano_status axis_basis(Ctx* c, int n, int lo, int hi, double** V, double** lam);
+
 // Validates dims/levels for the V-cycle; returns PANO_ERR_BAD_SHAPE with a
 // message when unsupported.
 pano_status mg_check(const Dims& D, int levels);

@@ -1,0 +1,340 @@
+// Per-element bodies of the multigrid V-cycle and of the red-black operator,
+// shared by the single-grid kernels (mg.hip, pcg.hip) and the batched kernels
+// (batch.hip): a grid gives the same bits alone and inside a batch.
+//
+// Red-black layout: plane c holds the cells of colour c = (x+y+z)&1 at
+// (z, y, x>>1); a thread owns the x-pair (2m, 2m+1) of row (z, y).
+#pragma once
+
+#include "fluid_device.hpp"
+
+namespace pano {
+
+// Reciprocals 1/(s*nc) of the smoother's diagonal for nc = 0..6 (0 for nc = 0),
+// formed in double from the T-valued s and rounded once to T.
+template <typename T>
+struct RInv {
+    T v[8];
+};
+template <typename T>
+inline RInv<T> make_rinv(T s) {
+    RInv<T> R;
+    for (int k = 0; k < 8; ++k) R.v[k] = k == 0 || k > 6 ? (T)0 : (T)(1.0 / ((double)s * (double)k));
+    return R;
+}
+
+// sum of the six neighbours in the fixed order ((x- + x+) + y-) + y+ (+ z-) + z+
+template <typename T, int DIM>
+__device__ __forceinline__ T nsum(const T* n) {
+    T s = ((n[0] + n[1]) + n[2]) + n[3];
+    if constexpr (DIM == 3) s = (s + n[4]) + n[5];
+    return s;
+}
+
+// Smoothed value z = (r + s*sum) * rinv[nc].
+template <typename T>
+__device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RInv<T>& ri) {
+    return nc ? (r + s * sum) * ri.v[nc] : (T)0;
+}
+
+// q = A p for the x-pair (m, y, z); returns the pair's p.q in FP64.
+template <typename T, int DIM>
+__device__ __forceinline__ double apply_pair(const Dims& D, T* q, const T* p, T dt, int m, int y, int z) {
+    RBRow R = rb_row<DIM>(D, z, y, m);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int i = k == 0 ? R.i0 : R.i1;
+        int has = k == 0 ? R.has0 : R.has1;
+        T n[6];
+        rb_nbrs<T, DIM>(p, R, k, n);
+        T pc = p[i];
+        T v = poisson_from<T, DIM>(pc, n[0], n[1], n[2], n[3], n[4], n[5], has, D.bc, dt);
+        q[i] = v;
+        s += (double)pc * (double)v;
+    }
+    return s;
+}
+
+// First full sweep from z = 0 for the x-pair: red z = (r + s*0)*rinv[nc];
+// the black update recomputes its six red neighbours from r (no read of z).
+template <typename T, int DIM>
+__device__ __forceinline__ void smooth_first_pair(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri, int m,
+                                                  int y, int z) {
+    const T zero = (T)0;
+    RBRow R = rb_row<DIM>(D, z, y, m);
+    int c0 = (y + z) & 1;
+    int kr = c0;  // the red cell of the pair: x0 when c0 == 0, else x1
+    int kb = 1 - c0;
+    int ir = kr == 0 ? R.i0 : R.i1, ib = kb == 0 ? R.i0 : R.i1;
+    int hr = kr == 0 ? R.has0 : R.has1, hb = kb == 0 ? R.has0 : R.has1;
+    zv[ir] = gs_value(r[ir], s, zero, ncount_bits<DIM>(hr, D.bc), ri);
+    int xb = 2 * m + kb;
+    T rn[6];
+    rb_nbrs<T, DIM>(r, R, kb, rn);
+    T n[6];
+    n[0] = (hb & 1) ? gs_value(rn[0], s, zero, ncount<DIM>(D, z, y, xb - 1), ri) : zero;
+    n[1] = (hb & 2) ? gs_value(rn[1], s, zero, ncount<DIM>(D, z, y, xb + 1), ri) : zero;
+    n[2] = (hb & 4) ? gs_value(rn[2], s, zero, ncount<DIM>(D, z, y - 1, xb), ri) : zero;
+    n[3] = (hb & 8) ? gs_value(rn[3], s, zero, ncount<DIM>(D, z, y + 1, xb), ri) : zero;
+    if constexpr (DIM == 3) {
+        n[4] = (hb & 16) ? gs_value(rn[4], s, zero, ncount<DIM>(D, z - 1, y, xb), ri) : zero;
+        n[5] = (hb & 32) ? gs_value(rn[5], s, zero, ncount<DIM>(D, z + 1, y, xb), ri) : zero;
+    } else {
+        n[4] = zero;
+        n[5] = zero;
+    }
+    zv[ib] = gs_value(r[ib], s, nsum<T, DIM>(n), ncount_bits<DIM>(hb, D.bc), ri);
+}
+
+// One colour of a red-black Gauss-Seidel sweep at (m, y, z); returns z.r of
+// the updated cell in FP64.
+template <typename T, int DIM>
+__device__ __forceinline__ double smooth_half_cell(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri,
+                                                   int color, int m, int y, int z) {
+    RBRow R = rb_row<DIM>(D, z, y, m);
+    int k = (color + y + z) & 1;  // parity of the updated cell's x
+    int i = k == 0 ? R.i0 : R.i1;
+    int has = k == 0 ? R.has0 : R.has1;
+    T n[6];
+    rb_nbrs<T, DIM>(zv, R, k, n);
+    T rv = r[i];
+    T v = gs_value(rv, s, nsum<T, DIM>(n), ncount_bits<DIM>(has, D.bc), ri);
+    zv[i] = v;
+    return (double)v * (double)rv;
+}
+
+// res = r - s*(nc*z - sum) for the x-pair
+template <typename T, int DIM>
+__device__ __forceinline__ void residual_pair(const Dims& D, T* res, const T* zv, const T* r, T s, int m, int y,
+                                              int z) {
+    RBRow R = rb_row<DIM>(D, z, y, m);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int i = k == 0 ? R.i0 : R.i1;
+        int has = k == 0 ? R.has0 : R.has1;
+        T n[6];
+        rb_nbrs<T, DIM>(zv, R, k, n);
+        T az = s * ((T)ncount_bits<DIM>(has, D.bc) * zv[i] - nsum<T, DIM>(n));
+        res[i] = r[i] - az;
+    }
+}
+
+// 1D restriction weights: (2I-1)/8, (2I)*3/8, (2I+1)*3/8, (2I+2)/8, a missing
+// outer term moving its weight onto the boundary cell (3/8 -> 1/2).
+template <typename T>
+__device__ __forceinline__ T r1(T vm, T v0, T v1, T vp, bool hm, bool hp) {
+    const T e8 = (T)0.125, t8 = (T)0.375, h2 = (T)0.5;
+    T a = hm ? vm * e8 : (T)0;
+    T b = v0 * (hm ? t8 : h2);
+    T c = v1 * (hp ? t8 : h2);
+    T e = hp ? vp * e8 : (T)0;
+    return ((a + b) + c) + e;
+}
+
+// Coarse-level index in natural or red-black layout.
+template <int LC>
+__device__ __forceinline__ int cidx(const Dims& Dc, int K, int J, int I) {
+    if constexpr (LC == REDBLACK) {
+        int wh = Dc.w >> 1, hw = Dc.h * wh;
+        return ((I + J + K) & 1) * (Dc.d * hw) + K * hw + J * wh + (I >> 1);
+    } else {
+        return (K * Dc.h + J) * Dc.w + I;
+    }
+}
+
+// r_c(I, J, K) = R res (x, then y, then z restriction of the fine residual in
+// red-black layout).
+template <typename T, int DIM, int LC>
+__device__ __forceinline__ void restrict_cell(const Dims& Df, const Dims& Dc, T* rc, const T* res, int I, int J,
+                                              int K) {
+    const T zero = (T)0;
+    bool hmI = I > 0, hpI = I < Dc.w - 1;
+    bool hmJ = J > 0, hpJ = J < Dc.h - 1;
+    const int wh = Df.w >> 1, hw = Df.h * wh, n2 = Df.d * hw;
+    // fine x = 2I-1, 2I, 2I+1, 2I+2 of row (z, y): colours 1-ce, ce, 1-ce, ce at
+    // compact I-1, I, I, I+1 with ce = (y+z)&1
+    auto tx = [&](int z, int y) -> T {
+        int row = z * hw + y * wh + I;
+        int ce = (y + z) & 1;
+        const T* pe = res + ce * n2 + row;
+        const T* po = res + (1 - ce) * n2 + row;
+        T vm = hmI ? po[-1] : zero;
+        T v0 = pe[0];
+        T v1 = po[0];
+        T vp = hpI ? pe[1] : zero;
+        return r1(vm, v0, v1, vp, hmI, hpI);
+    };
+    auto ty = [&](int z) -> T {
+        T vm = hmJ ? tx(z, 2 * J - 1) : zero;
+        T v0 = tx(z, 2 * J);
+        T v1 = tx(z, 2 * J + 1);
+        T vp = hpJ ? tx(z, 2 * J + 2) : zero;
+        return r1(vm, v0, v1, vp, hmJ, hpJ);
+    };
+    T out;
+    if constexpr (DIM == 3) {
+        bool hmK = K > 0, hpK = K < Dc.d - 1;
+        T vm = hmK ? ty(2 * K - 1) : zero;
+        T v0 = ty(2 * K);
+        T v1 = ty(2 * K + 1);
+        T vp = hpK ? ty(2 * K + 2) : zero;
+        out = r1(vm, v0, v1, vp, hmK, hpK);
+    } else {
+        out = ty(0);
+    }
+    rc[cidx<LC>(Dc, K, J, I)] = out;
+}
+
+// Prolongation weights for fine index i over a coarse axis of n cells.
+__device__ __forceinline__ void p1(int i, int n, int& I0, int& I1, bool& edge) {
+    int I = i >> 1;
+    int J = (i & 1) ? I + 1 : I - 1;
+    I0 = I;
+    edge = (J < 0 || J >= n);
+    I1 = edge ? I : J;
+}
+
+// z_f += P z_c (nested: x, then y, then z interpolation) for the x-pair
+// (2m, 2m+1): coarse column m with m-1 (for 2m) and m+1 (for 2m+1).
+template <typename T, int DIM, int LC>
+__device__ __forceinline__ void prolong_pair(const Dims& Df, const Dims& Dc, T* zf, const T* zc, int m, int y,
+                                             int z) {
+    const T w34 = (T)0.75, w14 = (T)0.25, one = (T)1, zero = (T)0;
+    int J0, J1, K0 = 0, K1 = 0;
+    bool ey, ez = true;
+    p1(y, Dc.h, J0, J1, ey);
+    T wy0 = ey ? one : w34, wy1 = ey ? zero : w14;
+    T wz0 = one, wz1 = zero;
+    if constexpr (DIM == 3) {
+        p1(z, Dc.d, K0, K1, ez);
+        wz0 = ez ? one : w34;
+        wz1 = ez ? zero : w14;
+    }
+    // x weights: x0 = 2m pairs with m-1, x1 = 2m+1 with m+1 (reflected at the ends)
+    bool el = m == 0, er = m + 1 >= Dc.w;
+    int Il = el ? m : m - 1, Ir = er ? m : m + 1;
+    T wl0 = el ? one : w34, wl1 = el ? zero : w14;
+    T wr0 = er ? one : w34, wr1 = er ? zero : w14;
+    T px0[2][2], px1[2][2];
+#pragma unroll
+    for (int a = 0; a < (DIM == 3 ? 2 : 1); ++a) {
+        int kk = a == 0 ? K0 : K1;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            int jj = b == 0 ? J0 : J1;
+            T cm = zc[cidx<LC>(Dc, kk, jj, m)];
+            T cl = zc[cidx<LC>(Dc, kk, jj, Il)];
+            T cr = zc[cidx<LC>(Dc, kk, jj, Ir)];
+            px0[a][b] = wl0 * cm + wl1 * cl;
+            px1[a][b] = wr0 * cm + wr1 * cr;
+        }
+    }
+    T v0, v1;
+    if constexpr (DIM == 3) {
+        v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
+        v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
+    } else {
+        v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
+        v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
+    }
+    RBRow R = rb_row<DIM>(Df, z, y, m);
+    zf[R.i0] = zf[R.i0] + v0;
+    zf[R.i1] = zf[R.i1] + v1;
+}
+
+// ---- coarsest level in LDS: whole planes of at most kFdmPlane cells --------------------
+constexpr int kFdmPlane = 4096;  // 32 KiB of doubles per buffer
+
+// Transform the plane in `a` (ny rows of nx) along x (axis 0) or y (axis 1),
+// forward (coefficients = V^T v) or inverse (values = V c), into `b`.
+__device__ __forceinline__ void lds_axis(const double* a, double* b, int nx, int ny, int axis, int inverse,
+                                         const double* V) {
+    int P = nx * ny;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int j = e / nx, i = e - j * nx;
+        double acc = 0.0;
+        if (axis == 0) {
+            if (inverse)
+                for (int k = 0; k < nx; ++k) acc += V[i * nx + k] * a[j * nx + k];
+            else
+                for (int q = 0; q < nx; ++q) acc += V[q * nx + i] * a[j * nx + q];
+        } else {
+            if (inverse)
+                for (int k = 0; k < ny; ++k) acc += V[j * ny + k] * a[k * nx + i];
+            else
+                for (int q = 0; q < ny; ++q) acc += V[q * ny + j] * a[q * nx + i];
+        }
+        b[e] = acc;
+    }
+}
+
+// One z-plane of the coarsest level (lds: 2*nx*ny doubles):
+//  mode 0: forward x then y (input Tin, output double)
+//  mode 1: inverse y then x (input double, output Tout)
+//  mode 2: the whole 2D solve: forward x, y, divide by s*lambda, inverse y, x
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int mode, const Tin* src, Tout* dst,
+                                               const double* Vx, const double* Vy, const double* lx,
+                                               const double* ly, double s, double eps) {
+    double* a = lds;
+    double* b = lds + nx * ny;
+    int P = nx * ny;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = (double)src[e];
+    __syncthreads();
+    if (mode == 1) {
+        lds_axis(a, b, nx, ny, 1, 1, Vy);
+        __syncthreads();
+        lds_axis(b, a, nx, ny, 0, 1, Vx);
+    } else {
+        lds_axis(a, b, nx, ny, 0, 0, Vx);
+        __syncthreads();
+        lds_axis(b, a, nx, ny, 1, 0, Vy);
+        if (mode == 2) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < P; e += blockDim.x) {
+                int j = e / nx, i = e - j * nx;
+                double lam = lx[i] + ly[j];
+                a[e] = lam > eps ? a[e] / (s * lam) : 0.0;
+            }
+            __syncthreads();
+            lds_axis(a, b, nx, ny, 1, 1, Vy);
+            __syncthreads();
+            lds_axis(b, a, nx, ny, 0, 1, Vx);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) dst[e] = (Tout)a[e];
+}
+
+// One y-row j of the 3D coarsest level: forward z, divide by s*(lx+ly+lz),
+// inverse z, in place on the (nz x nx) slice of data (lds: 2*nx*nz doubles).
+__device__ __forceinline__ void fdm_zcol_body(double* lds, int nx, int ny, int nz, int j, double* data,
+                                              const double* Vz, const double* lx, const double* ly, const double* lz,
+                                              double s, double eps) {
+    double* a = lds;
+    double* b = lds + nx * nz;
+    int P = nx * nz;
+    int64_t zs = (int64_t)nx * ny;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        a[e] = data[k * zs + (int64_t)j * nx + i];
+    }
+    __syncthreads();
+    lds_axis(a, b, nx, nz, 1, 0, Vz);  // slice rows are z: the "y" axis of lds_axis
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        double lam = lx[i] + ly[j] + lz[k];
+        b[e] = lam > eps ? b[e] / (s * lam) : 0.0;
+    }
+    __syncthreads();
+    lds_axis(b, a, nx, nz, 1, 1, Vz);
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int k = e / nx, i = e - k * nx;
+        data[k * zs + (int64_t)j * nx + i] = a[e];
+    }
+}
+
+}  // namespace pano

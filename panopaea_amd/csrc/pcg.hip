@@ -14,244 +14,15 @@
 //
 // Arithmetic: fields in T, dot products accumulated in FP64, alpha/beta in
 // FP64, x/r/p updates formed in FP64 and rounded once to T.
-#include "fluid_device.hpp"
 #include "mg.hpp"
+#include "mg_device.hpp"
+#include "solver_device.hpp"
 
 namespace pano {
 
-constexpr int kItems = 8;  // elements per thread in the flat vector kernels
-constexpr int kFlatPer = kBlock * kItems;
-
-enum FinOp { FIN_STATS = 0, FIN_THRESH, FIN_SIGMA0, FIN_ALPHA, FIN_RNORM, FIN_BETA };
-
-struct SolveCfg {
-    int32_t tol_kind;
-    int32_t remove_mean;
-    double tol;
-    int64_t n;
-};
-
-__global__ __launch_bounds__(1024) void k_finalize(int op, const double* part, int np, SolverState* st,
-                                                   SolveCfg cfg, int use_max) {
-    if (op >= FIN_SIGMA0 && st->done) return;  // uniform: every thread returns
-    // eight independent accumulators per thread (loads in flight), combined in a fixed order
-    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int stride = blockDim.x * 8;
-    for (int base = threadIdx.x; base < np; base += stride) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            int i = base + u * blockDim.x;
-            double v = i < np ? part[i] : 0.0;
-            a8[u] = use_max ? absmax_acc(a8[u], v) : a8[u] + v;
-        }
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc = use_max ? absmax_acc(acc, a8[u]) : acc + a8[u];
-    double tot = use_max ? block_max(acc) : block_sum(acc);
-    if (threadIdx.x != 0) return;
-    switch (op) {
-        case FIN_STATS:  // tot = sum(b): the mean to remove
-            st->mean = tot / (double)cfg.n;
-            break;
-        case FIN_THRESH: {  // tot = ||b||_inf (after mean removal)
-            st->b_linf = tot;
-            st->rnorm = tot;
-            double thr = cfg.tol_kind == PANO_TOL_ABS_LINF ? cfg.tol : cfg.tol * tot;
-            st->threshold = thr;
-            st->iter = 0;
-            st->done = 0;
-            st->converged = 0;
-            st->nonfinite = 0;
-            if (!(tot == tot) || tot > 1.7e308) {
-                st->nonfinite = 1;
-                st->done = 1;
-            }
-            break;  // the early-out test runs after rounding thr to T
-        }
-        case FIN_SIGMA0:
-            st->sigma = tot;
-            break;
-        case FIN_ALPHA:
-            st->pq = tot;
-            st->alpha = st->sigma / tot;
-            if (!(st->alpha == st->alpha) || fabs(st->alpha) > 1.7e308) {
-                st->nonfinite = 1;
-                st->done = 1;
-            }
-            break;
-        case FIN_RNORM:
-            st->rnorm = tot;
-            st->iter += 1;
-            if (!(tot == tot)) {
-                st->nonfinite = 1;
-                st->done = 1;
-            } else if (tot < st->threshold) {  // pcg.rs:58-63
-                st->converged = 1;
-                st->done = 1;
-            } else if (st->iter >= st->max_iter) {
-                st->done = 1;
-            }
-            break;
-        case FIN_BETA: {
-            double sn = tot;
-            st->beta = sn / st->sigma;
-            st->sigma = sn;
-            if (!(st->beta == st->beta)) {
-                st->nonfinite = 1;
-                st->done = 1;
-            }
-            break;
-        }
-    }
-}
-
-// Threshold rounding: the stop test compares T values, so the threshold is
-// rounded to T like the reference's `threshold: T`.
-template <typename T>
-__global__ void k_round_threshold(SolverState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->threshold = (double)(T)st->threshold;
-        if (!st->done && st->b_linf < st->threshold) {  // pcg.rs:35-38
-            st->converged = 1;
-            st->done = 1;
-        }
-    }
-}
-
-__global__ void k_state_init(SolverState* st, int max_iter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        SolverState s = {};
-        s.max_iter = max_iter;
-        *st = s;
-    }
-}
-
-// partial sum and max|.| of r (the right-hand side)
-template <typename T>
-__global__ __launch_bounds__(256) void k_stats(int64_t n, const T* r, double* psum, double* pmax) {
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double s = 0.0, m = 0.0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) {
-            double v = (double)r[i];
-            s += v;
-            m = absmax_acc(m, v);
-        }
-    }
-    double ts = block_sum(s);
-    double tm = block_max(m);
-    if (threadIdx.x == 0) {
-        psum[blockIdx.x] = ts;
-        pmax[blockIdx.x] = tm;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_sub_mean(int64_t n, T* r, const SolverState* st, double* pmax) {
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double mean = st->mean, m = 0.0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) {
-            T v = (T)((double)r[i] - mean);
-            r[i] = v;
-            m = absmax_acc(m, (double)v);
-        }
-    }
-    double tm = block_max(m);
-    if (threadIdx.x == 0) pmax[blockIdx.x] = tm;
-}
-
-// x = 0
-template <typename T>
-__global__ __launch_bounds__(256) void k_zero(int64_t n, T* x) {
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) x[i] = (T)0;
-    }
-}
-
-// p = z ; partial z.r
-template <typename T>
-__global__ __launch_bounds__(256) void k_copy_dot(int64_t n, T* p, const T* z, const T* r, double* part,
-                                                  const SolverState* st) {
-    if (st->done) return;
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) {
-            T zv = z[i];
-            p[i] = zv;
-            s += (double)zv * (double)r[i];
-        }
-    }
-    double t = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// partial z.r
-template <typename T>
-__global__ __launch_bounds__(256) void k_dot(int64_t n, const T* a, const T* b, double* part,
-                                             const SolverState* st) {
-    if (st->done) return;
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) s += (double)a[i] * (double)b[i];
-    }
-    double t = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// x += alpha p ; r -= alpha q ; partial ||r||_inf (and r.r for the identity
-// preconditioner, where z aliases r).
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(int64_t n, T* x, T* r, const T* p, const T* q,
-                                                double* pmax, double* prr, const SolverState* st) {
-    if (st->done) return;
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double alpha = st->alpha;
-    double m = 0.0, s = 0.0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) {
-            x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-            T rv = (T)((double)r[i] - alpha * (double)q[i]);
-            r[i] = rv;
-            m = absmax_acc(m, (double)rv);
-            s += (double)rv * (double)rv;
-        }
-    }
-    double tm = block_max(m);
-    if (prr) {
-        double ts = block_sum(s);
-        if (threadIdx.x == 0) prr[blockIdx.x] = ts;
-    }
-    if (threadIdx.x == 0) pmax[blockIdx.x] = tm;
-}
-
-// p = z + beta p
-template <typename T>
-__global__ __launch_bounds__(256) void k_search(int64_t n, T* p, const T* z, const SolverState* st) {
-    if (st->done) return;
-    int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
-    double beta = st->beta;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
-    }
+__global__ __launch_bounds__(1024) void k_finalize(int op, const double* part, const double* part1, int np,
+                                                   SolverState* st, SolveCfg cfg, int use_max) {
+    finalize_range(op, part, part1, np, st, cfg, use_max);
 }
 
 // q = A p with partial p.q ; natural layout, optional 2D obstacle mask.
@@ -288,48 +59,15 @@ __global__ __launch_bounds__(256) void k_apply_dot_nat(Dims D, Grid3 G, T* q, co
     if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
 }
 
-// q = A p with partial p.q ; red-black layout (thread = x-pair). With FUSE
-// the search update of the previous iteration is folded in: every p value is
-// formed as (T)(z + beta*p_old) (the same rounding as a separate
-// p = z + beta p pass), the pair's own values are stored to pnew (a second
-// buffer: neighbours still read p_old) and q = A pnew.
-template <typename T, int DIM, bool FUSE>
-__global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T* pnew, const T* zv,
-                                                      T dt, double* part, const SolverState* st) {
+// q = A p with partial p.q ; red-black layout (thread = x-pair).
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T dt, double* part,
+                                                      const SolverState* st) {
     if (st->done) return;
     int m, y, z, L;
     tile_xyz(G, m, y, z, L);
     double s = 0.0;
-    const double beta = FUSE ? st->beta : 0.0;
-    auto P = [&](int j) -> T {
-        if constexpr (FUSE) return (T)((double)zv[j] + beta * (double)p[j]);
-        else return p[j];
-    };
-    if (m < (D.w >> 1) && y < D.h) {
-        RBRow R = rb_row<DIM>(D, z, y, m);
-        const T zero = (T)0;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            int i = k == 0 ? R.i0 : R.i1;
-            int has = k == 0 ? R.has0 : R.has1;
-            int j[6];
-            rb_nbr_idx<DIM>(R, k, j);
-            T raw[6];
-#pragma unroll
-            for (int qq = 0; qq < 6; ++qq) raw[qq] = (qq < 4 || DIM == 3) ? P(j[qq]) : zero;
-            T a = (has & 1) ? raw[0] : zero;
-            T b = (has & 2) ? raw[1] : zero;
-            T c = (has & 4) ? raw[2] : zero;
-            T e = (has & 8) ? raw[3] : zero;
-            T f = (DIM == 3 && (has & 16)) ? raw[4] : zero;
-            T g = (DIM == 3 && (has & 32)) ? raw[5] : zero;
-            T pc = P(i);
-            if constexpr (FUSE) pnew[i] = pc;
-            T v = poisson_from<T, DIM>(pc, a, b, c, e, f, g, has, D.bc, dt);
-            q[i] = v;
-            s += (double)pc * (double)v;
-        }
-    }
+    if (m < (D.w >> 1) && y < D.h) s = apply_pair<T, DIM>(D, q, p, dt, m, y, z);
     double t = block_sum(s);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
 }
@@ -403,8 +141,9 @@ struct Solver {
     Grid3 G;
     MG<T> mg;
     SolveCfg cfg;
-    // vectors (layout per rb); q aliases aux except on the fused red-black path
-    T *x, *r, *aux, *p, *q, *p2;
+    // vectors (layout per rb); q (= A p) shares aux with z, as the reference's
+    // `auxiliary` holds both (pcg.rs:51,65)
+    T *x, *r, *aux, *p, *q;
     int it;  // iterations enqueued in this solve
 };
 
@@ -447,9 +186,11 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     return PANO_OK;
 }
 
+// part1 (optional) holds a second set of np partials added element by element.
 template <typename T>
-static void fin(Solver<T>* S, int op, double* part, int np, int use_max) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, S->c->stream, op, part, np, S->st, S->cfg, use_max);
+static void fin(Solver<T>* S, int op, double* part, int np, int use_max, const double* part1 = nullptr) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, S->c->stream, op, (const double*)part, part1, np, S->st,
+                       S->cfg, use_max);
     if (op == FIN_RNORM) S->c->timer.checks += 1;
 }
 
@@ -463,6 +204,7 @@ static pano_status precond_dot(Solver<T>* S, T* z, const T* r, double* part, int
         else L3((k_jacobi_dot<T, 2>), S->G, D, S->G, z, r, S->dt, S->mask, part, S->st);
         *np = S->ntile;
     } else if (S->precond == PANO_PRECOND_MULTIGRID) {
+        // black-sweep partials in part[0, nb), red-sweep partials in part[nb, 2nb)
         PANO_CHECK(mg_vcycle<T>(c, &S->mg, z, r, S->st, part, np));
     } else {
         klaunch(c, "dot", 2.0 * S->n * sizeof(T), k_dot<T>, dim3(S->nflat), dim3(kBlock), S->n, (const T*)z, r,
@@ -515,29 +257,14 @@ static pano_status solver_iteration(Solver<T>* S) {
     const double unit = (double)S->n * sizeof(T);  // one field pass
     const dim3 B(kTX, kTY), Gd(S->G.nb);
     const SolverState* cst = S->st;
-    // Folding p = z + beta p into q = A p (FUSE) doubles the loads per stencil
-    // point; measured slower on gfx950 than the two passes, so it is off.
-    const bool fuse = false;
     // q = A p ; alpha   (pcg.rs:51-53)
     if (S->rb) {
-        if (fuse) {
-            if (D.d > 1)
-                klaunch(c, "apply_dot", 4 * unit, k_apply_dot_rb<T, 3, true>, Gd, B, D, S->G, S->q, (const T*)S->p,
-                        S->p2, (const T*)S->aux, S->dt, S->part0, cst);
-            else
-                klaunch(c, "apply_dot", 4 * unit, k_apply_dot_rb<T, 2, true>, Gd, B, D, S->G, S->q, (const T*)S->p,
-                        S->p2, (const T*)S->aux, S->dt, S->part0, cst);
-            T* t = S->p;
-            S->p = S->p2;
-            S->p2 = t;
-        } else {
-            if (D.d > 1)
-                klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, false>, Gd, B, D, S->G, S->q, (const T*)S->p,
-                        (T*)nullptr, (const T*)nullptr, S->dt, S->part0, cst);
-            else
-                klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2, false>, Gd, B, D, S->G, S->q, (const T*)S->p,
-                        (T*)nullptr, (const T*)nullptr, S->dt, S->part0, cst);
-        }
+        if (D.d > 1)
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->part0, cst);
+        else
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->part0, cst);
     } else {
         if (D.d > 1)
             klaunch(c, "apply_dot", 2 * unit, k_apply_dot_nat<T, 3>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
@@ -561,11 +288,11 @@ static pano_status solver_iteration(Solver<T>* S) {
         z = S->aux;
         int np = 0;
         PANO_CHECK(precond_dot<T>(S, z, S->r, S->part0, &np));
-        fin(S, FIN_BETA, S->part0, np, 0);
+        if (S->rb) fin(S, FIN_BETA, S->part0, np / 2, 0, S->part0 + np / 2);
+        else fin(S, FIN_BETA, S->part0, np, 0);
     }
     // p = z + beta p   (pcg.rs:72-77)
-    if (!fuse)
-        klaunch(c, "search", 3 * unit, k_search<T>, dim3(S->nflat), dim3(kBlock), S->n, S->p, (const T*)z, cst);
+    klaunch(c, "search", 3 * unit, k_search<T>, dim3(S->nflat), dim3(kBlock), S->n, S->p, (const T*)z, cst);
     S->it += 1;
     PANO_HIP(hipGetLastError());
     return PANO_OK;
@@ -655,14 +382,6 @@ pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, con
     S.aux = auxiliary;
     S.p = search;
     S.q = nullptr;
-    S.p2 = nullptr;
-    if (S.rb) {
-        void *qv, *p2;
-        PANO_CHECK(c->workspace("pcg_q", (size_t)S.n * sizeof(T), &qv));
-        PANO_CHECK(c->workspace("pcg_p2", (size_t)S.n * sizeof(T), &p2));
-        S.q = (T*)qv;
-        S.p2 = (T*)p2;
-    }
     Grid3 Gn = grid3(D.w, D.h, D.d);
     if (S.rb) {
         void* xr;
@@ -716,14 +435,6 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     S.p = (T*)s;
     S.x = (T*)xv;
     S.q = nullptr;
-    S.p2 = nullptr;
-    if (S.rb) {
-        void *qv, *p2;
-        PANO_CHECK(c->workspace("step_q", nb, &qv));
-        PANO_CHECK(c->workspace("step_p2", nb, &p2));
-        S.q = (T*)qv;
-        S.p2 = (T*)p2;
-    }
     double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
     const double* sp = P->src_radius > 0 ? sphere : nullptr;
     // source -> advection of density and velocity from the old velocity
@@ -826,12 +537,10 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     Solver<float> S;
     PANO_CHECK(solver_setup<float>(c, D, &ip, &S));
     const int64_t n = S.n;
-    void *r32, *aux, *pp, *q, *p2, *e, *b64, *x64, *st2v, *part2;
+    void *r32, *aux, *pp, *e, *b64, *x64, *st2v, *part2;
     PANO_CHECK(c->workspace("mx_r32", n * sizeof(float), &r32));
     PANO_CHECK(c->workspace("mx_aux", n * sizeof(float), &aux));
     PANO_CHECK(c->workspace("mx_p", n * sizeof(float), &pp));
-    PANO_CHECK(c->workspace("mx_q", n * sizeof(float), &q));
-    PANO_CHECK(c->workspace("mx_p2", n * sizeof(float), &p2));
     PANO_CHECK(c->workspace("mx_e", n * sizeof(float), &e));
     PANO_CHECK(c->workspace("mx_b64", n * sizeof(double), &b64));
     PANO_CHECK(c->workspace("mx_x64", n * sizeof(double), &x64));
@@ -841,8 +550,7 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     S.r = (float*)r32;
     S.aux = (float*)aux;
     S.p = (float*)pp;
-    S.q = (float*)q;
-    S.p2 = (float*)p2;
+    S.q = nullptr;
     S.x = (float*)e;
     SolverState* st2 = (SolverState*)st2v;
     double* part = (double*)part2;
@@ -860,12 +568,14 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     hipLaunchKernelGGL((k_zero<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, xx);
     hipLaunchKernelGGL((k_stats<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, (const double*)bb, S.part0, part);
     if (cfg2.remove_mean) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_STATS, (const double*)S.part0, S.nflat,
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_STATS, (const double*)S.part0,
+                           (const double*)nullptr, S.nflat,
                            st2, cfg2, 0);
         hipLaunchKernelGGL((k_sub_mean<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, bb, (const SolverState*)st2,
                            part);
     }
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_THRESH, (const double*)part, S.nflat, st2,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_THRESH, (const double*)part,
+                       (const double*)nullptr, S.nflat, st2,
                        cfg2, 1);
     SolverState* hs = c->host_state;
     const std::string key = hint_key(D, &ip, sizeof(float));
@@ -878,7 +588,8 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
         else
             hipLaunchKernelGGL(k_resid64_rb<2>, dim3(S.G.nb), dim3(kTX, kTY), 0, s, D, S.G, S.r, (const double*)xx,
                                (const double*)bb, M->inner.dt, part);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_RNORM, (const double*)part, S.ntile, st2,
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_RNORM, (const double*)part,
+                           (const double*)nullptr, S.ntile, st2,
                            cfg2, 1);
         PANO_HIP(hipMemcpyAsync(&hs[1], st2, sizeof(SolverState), hipMemcpyDeviceToHost, s));
         PANO_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,291 @@
+// Device side of the preconditioned CG shared by the single-grid solver
+// (pcg.hip) and the batched engine (batch.hip): the solver-state updates of
+// the one-workgroup finalize step and the flat vector kernels of
+// panopaea/src/pcg.rs:40-79 (fields in T, FP64 accumulation, updates formed in
+// FP64 and rounded once to T).
+#pragma once
+
+#include "pano_internal.hpp"
+
+namespace pano {
+
+constexpr int kItems = 8;  // elements per thread in the flat vector kernels
+constexpr int kFlatPer = kBlock * kItems;
+
+enum FinOp { FIN_STATS = 0, FIN_THRESH, FIN_SIGMA0, FIN_ALPHA, FIN_RNORM, FIN_BETA };
+
+struct SolveCfg {
+    int32_t tol_kind;
+    int32_t remove_mean;
+    double tol;
+    int64_t n;
+};
+
+// Reduces the FP64 partials part0[0..np) (+ part1[0..np) when part1 is not
+// null) in a fixed order and applies `op` to the solver state. Run by one
+// workgroup (all threads).
+__device__ __forceinline__ void finalize_range(int op, const double* part0, const double* part1, int np,
+                                               SolverState* st, const SolveCfg& cfg, int use_max) {
+    if (op >= FIN_SIGMA0 && st->done) return;  // uniform: every thread returns
+    // eight independent accumulators per thread (loads in flight), combined in a fixed order
+    double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int stride = blockDim.x * 8;
+    for (int base = threadIdx.x; base < np; base += stride) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int i = base + u * blockDim.x;
+            double v = i < np ? part0[i] : 0.0;
+            if (part1 && i < np) v += part1[i];
+            a8[u] = use_max ? absmax_acc(a8[u], v) : a8[u] + v;
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = use_max ? absmax_acc(acc, a8[u]) : acc + a8[u];
+    double tot = use_max ? block_max(acc) : block_sum(acc);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_STATS:  // tot = sum(b): the mean to remove
+            st->mean = tot / (double)cfg.n;
+            break;
+        case FIN_THRESH: {  // tot = ||b||_inf (after mean removal)
+            st->b_linf = tot;
+            st->rnorm = tot;
+            double thr = cfg.tol_kind == PANO_TOL_ABS_LINF ? cfg.tol : cfg.tol * tot;
+            st->threshold = thr;
+            st->iter = 0;
+            st->done = 0;
+            st->converged = 0;
+            st->nonfinite = 0;
+            if (!(tot == tot) || tot > 1.7e308) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;  // the early-out test runs after rounding thr to T
+        }
+        case FIN_SIGMA0:
+            st->sigma = tot;
+            break;
+        case FIN_ALPHA:
+            st->pq = tot;
+            st->alpha = st->sigma / tot;
+            if (!(st->alpha == st->alpha) || fabs(st->alpha) > 1.7e308) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;
+        case FIN_RNORM:
+            st->rnorm = tot;
+            st->iter += 1;
+            if (!(tot == tot)) {
+                st->nonfinite = 1;
+                st->done = 1;
+            } else if (tot < st->threshold) {  // pcg.rs:58-63
+                st->converged = 1;
+                st->done = 1;
+            } else if (st->iter >= st->max_iter) {
+                st->done = 1;
+            }
+            break;
+        case FIN_BETA: {
+            double sn = tot;
+            st->beta = sn / st->sigma;
+            st->sigma = sn;
+            if (!(st->beta == st->beta)) {
+                st->nonfinite = 1;
+                st->done = 1;
+            }
+            break;
+        }
+    }
+}
+
+// Threshold rounding: the stop test compares T values, so the threshold is
+// rounded to T like the reference's `threshold: T`.
+template <typename T>
+__global__ void k_round_threshold(SolverState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->threshold = (double)(T)st->threshold;
+        if (!st->done && st->b_linf < st->threshold) {  // pcg.rs:35-38
+            st->converged = 1;
+            st->done = 1;
+        }
+    }
+}
+
+static __global__ void k_state_init(SolverState* st, int max_iter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        SolverState s = {};
+        s.max_iter = max_iter;
+        *st = s;
+    }
+}
+
+// partial sum and max|.| of r (the right-hand side)
+// ---- flat vector kernels: block lb of a field of n elements covers
+// [lb*kFlatPer, (lb+1)*kFlatPer) with kItems strided elements per thread ------------------
+
+// partial sum and max|.| of r (the right-hand side)
+template <typename T>
+__device__ __forceinline__ void flat_stats(int lb, int64_t n, const T* r, double* sum, double* mx) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    double s = 0.0, m = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            double v = (double)r[i];
+            s += v;
+            m = absmax_acc(m, v);
+        }
+    }
+    *sum = block_sum(s);
+    *mx = block_max(m);
+}
+
+template <typename T>
+__device__ __forceinline__ double flat_sub_mean(int lb, int64_t n, T* r, double mean) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    double m = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            T v = (T)((double)r[i] - mean);
+            r[i] = v;
+            m = absmax_acc(m, (double)v);
+        }
+    }
+    return block_max(m);
+}
+
+template <typename T>
+__device__ __forceinline__ void flat_zero(int lb, int64_t n, T* x) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) x[i] = (T)0;
+    }
+}
+
+// p = z ; returns the block's z.r
+template <typename T>
+__device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T* z, const T* r) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            T zv = z[i];
+            p[i] = zv;
+            s += (double)zv * (double)r[i];
+        }
+    }
+    return block_sum(s);
+}
+
+template <typename T>
+__device__ __forceinline__ double flat_dot(int lb, int64_t n, const T* a, const T* b) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) s += (double)a[i] * (double)b[i];
+    }
+    return block_sum(s);
+}
+
+// x += alpha p ; r -= alpha q ; block max|r| (and r.r when want_rr)
+template <typename T>
+__device__ __forceinline__ void flat_update(int lb, int64_t n, T* x, T* r, const T* p, const T* q, double alpha,
+                                            bool want_rr, double* mx, double* rr) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    double m = 0.0, s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) {
+            x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+            T rv = (T)((double)r[i] - alpha * (double)q[i]);
+            r[i] = rv;
+            m = absmax_acc(m, (double)rv);
+            s += (double)rv * (double)rv;
+        }
+    }
+    *mx = block_max(m);
+    *rr = want_rr ? block_sum(s) : 0.0;
+}
+
+// p = z + beta p
+template <typename T>
+__device__ __forceinline__ void flat_search(int lb, int64_t n, T* p, const T* z, double beta) {
+    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+    }
+}
+
+// ---- single-grid wrappers ---------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_stats(int64_t n, const T* r, double* psum, double* pmax) {
+    double s, m;
+    flat_stats(blockIdx.x, n, r, &s, &m);
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = s;
+        pmax[blockIdx.x] = m;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_mean(int64_t n, T* r, const SolverState* st, double* pmax) {
+    double m = flat_sub_mean(blockIdx.x, n, r, st->mean);
+    if (threadIdx.x == 0) pmax[blockIdx.x] = m;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(int64_t n, T* x) {
+    flat_zero(blockIdx.x, n, x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_dot(int64_t n, T* p, const T* z, const T* r, double* part,
+                                                  const SolverState* st) {
+    if (st->done) return;
+    double t = flat_copy_dot(blockIdx.x, n, p, z, r);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(int64_t n, const T* a, const T* b, double* part,
+                                             const SolverState* st) {
+    if (st->done) return;
+    double t = flat_dot(blockIdx.x, n, a, b);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||_inf (and r.r for the identity
+// preconditioner, where z aliases r).
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(int64_t n, T* x, T* r, const T* p, const T* q, double* pmax,
+                                                double* prr, const SolverState* st) {
+    if (st->done) return;
+    double m, s;
+    flat_update(blockIdx.x, n, x, r, p, q, st->alpha, prr != nullptr, &m, &s);
+    if (threadIdx.x == 0) {
+        pmax[blockIdx.x] = m;
+        if (prr) prr[blockIdx.x] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_search(int64_t n, T* p, const T* z, const SolverState* st) {
+    if (st->done) return;
+    flat_search(blockIdx.x, n, p, z, st->beta);
+}
+
+}  // namespace pano
