@@ -33,16 +33,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kernel", default="smooth_l0", help="kernel whose roofline is reported")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--dims", default=None)
     ap.add_argument("--workload", default="smoke3d_256",
-                    choices=["smoke3d_256", "poisson512_mixed"])
+                    choices=["smoke3d_256", "poisson512_mixed", "batch64_2d", "stream_copy"])
     ap.add_argument("--cpu-check", action="store_true",
                     help="poisson512_mixed: also solve in pure FP64 with the CPU oracle (minutes at 512^3)")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 5
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the fluid step runs on a single MI355X); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
@@ -50,6 +53,10 @@ def main():
 
     if args.workload == "poisson512_mixed":
         return bench_mixed(args)
+    if args.workload == "batch64_2d":
+        return bench_batch(args)
+    if args.workload == "stream_copy":
+        return bench_stream_copy(args)
 
     import panopaea_amd as pa
     from panopaea_amd import _lib
@@ -234,6 +241,119 @@ def bench_mixed(args):
         "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
+
+
+def batch_step_bytes(dims, iters, levels):
+    """Algorithmic bytes of one 2D MG-PCG fluid step (f32 units of N cells:
+    advect 6, divergence 3, gradient 5, MG V-cycle 16.25 per level with the
+    geometric factor 1/4, CG 2 + 6 + 3 per iteration)."""
+    n = dims[0] * dims[1]
+    vc = 16.25 * sum(0.25 ** l for l in range(levels - 1))
+    return 4.0 * n * (6 + 3 + 5 + vc + (2 + 6 + 3 + vc) * iters)
+
+
+def bench_batch(args):
+    """64 uneven square 2D grids advanced together: one set of launches per
+    phase covers every grid, phases replayed as hipGraphs, tile shapes from
+    the autotuner. Timed: K steps of all grids from the seeded state."""
+    import panopaea_amd as pa
+    from panopaea_amd import workloads as W
+    wl = W.BATCH64_2D
+    grids = W.batch_grids(wl)
+    steps = args.steps if args.steps_given else wl["steps"]
+    t0 = time.time()
+    B = pa.Batch(grids, mg_levels=wl["mg_levels"], mg_nu=wl["mg_nu"], tol=wl["tol"], use_graph=True,
+                 autotune=True)
+    tiles = B.tiles()
+    init = [W.batch_initial(wl, g) for g in grids]
+    print(f"bench: batch of {len(grids)} grids set up in {time.time() - t0:.2f}s tiles {tiles}", file=sys.stderr,
+          flush=True)
+
+    def load():
+        for i, (rho, faces) in enumerate(init):
+            B.set_state(i, rho, faces)
+
+    load()
+    for k in range(args.warmup):
+        its = B.step()
+        print(f"bench: warmup {k} max iterations={max(its)}", file=sys.stderr, flush=True)
+    load()
+    B.ctx.synchronize()
+    lat, its_all = [], []
+    t_start = time.perf_counter()
+    for k in range(steps):
+        a = time.perf_counter()
+        its_all.append(B.step())
+        lat.append(time.perf_counter() - a)
+    B.ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    cells = sum(g["dims"][0] * g["dims"][1] for g in grids)
+    its = np.array(its_all)
+    bytes_total = sum(batch_step_bytes(g["dims"], int(its[s, i]), wl["mg_levels"])
+                      for s in range(steps) for i, g in enumerate(grids))
+    achieved = bytes_total / elapsed / 1e9
+    print(f"bench: {steps} steps {elapsed * 1e3:.1f} ms, CG iterations per step max {its.max(axis=1).tolist()}",
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as o
+        g = grids[0]
+        rho, faces = init[0][0].copy(), init[0][1].copy()
+        P = o.params(g["dims"], bc=g["bc"], dt=g["dt"], precond="mg", levels=wl["mg_levels"], tol=wl["tol"],
+                     max_iterations=500)
+        reps, t = 0, time.perf_counter()
+        while time.perf_counter() - t < 10.0:
+            o.step(P, rho, faces)
+            reps += 1
+        el = time.perf_counter() - t
+        n0 = g["dims"][0] * g["dims"][1]
+        cpu = {"value": n0 * reps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} steps of grid 0 ({g['dims'][0]}^2) of the batch from its seeded state, CPU "
+                         f"oracle (C, f32 fields, MG-PCG), 1 thread, {el:.1f}s"}
+    B.close()
+    out = {
+        "metric": "cell-updates/s, 64 uneven 2D grids advect+project @1e-6 resid, batched hipGraph steps",
+        "value": cells * steps / elapsed, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / steps * 1e3,
+        "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded band-limited velocity, Gaussian density)",
+        "config": {"workload": wl["name"], "grids": len(grids), "cells": cells,
+                   "sizes": sorted(set(g["dims"][0] for g in grids)),
+                   "precond": "multigrid 4 levels, 2+2 red-black GS", "tol": "1e-6 rel L-inf",
+                   "tiles": {"natural": list(tiles["nat"]), "red_black": list(tiles["rb"])},
+                   "cg_iterations_max_per_step": its.max(axis=1).tolist()},
+        "roofline": {"bound": "hbm", "kernel": "whole step (model bytes / wall time)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_stream_copy(args):
+    """Device copy of a 1 GiB f32 buffer (torch's elementwise copy kernel):
+    the measured copy bandwidth, and the known-size kernel that calibrates
+    the FETCH_SIZE / WRITE_SIZE counters (1 GiB read, 1 GiB written per
+    launch)."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.warmup)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2.0 * 4 * n / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "bytes_read": 4 * n,
+                      "bytes_written": 4 * n, "ms_per_copy": ms, "steps": args.steps}), flush=True)
 
 
 def cpu_baseline(wl, dims, cfg, init_vel):

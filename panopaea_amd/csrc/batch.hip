@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 
 #include "batch.hpp"
 
@@ -18,14 +19,19 @@ namespace pano {
 
 // ---- launch decode -------------------------------------------------------------------------
 
-__device__ __forceinline__ int find_entry(const BEntry* e, int n, int b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (e[mid].start <= b) lo = mid;
-        else hi = mid - 1;
+// Index of the entry whose block range holds b: the count of first blocks
+// <= b, taken with one ballot per 64 entries (two levels for up to 4096).
+__device__ __forceinline__ int find_entry(const int* start, int n, int b) {
+    const int lane = __lane_id();
+    int base = 0;
+    if (n > 64) {
+        int k = lane * 64;
+        unsigned long long m = __ballot(k < n && start[k] <= b);
+        base = 64 * (__popcll(m) - 1);
     }
-    return lo;
+    int k = base + lane;
+    unsigned long long m = __ballot(k < n && start[k] <= b);
+    return base + __popcll(m) - 1;
 }
 __device__ __forceinline__ void local_xyz(const Grid3& G, int lb, int& x, int& y, int& z) {
     int bx = lb % G.gx;
@@ -40,17 +46,16 @@ __device__ __forceinline__ T* fld(char* arena, int64_t off) {
     return reinterpret_cast<T*>(arena + off);
 }
 
-#define BATCH_ENTRY(tab, ne, nbt)              \
-    const int b_ = xcd_remap(blockIdx.x, nbt); \
-    const int g_ = find_entry(tab, ne, b_);    \
-    const BEntry& E = tab[g_];                 \
+#define BATCH_ENTRY(T)                                      \
+    const int b_ = xcd_remap(blockIdx.x, T.nbt);            \
+    const BEntry& E = T.e[find_entry(T.start, T.n, b_)];    \
     const int lb = b_ - E.start;
 
 // ---- step kernels ----------------------------------------------------------------------------
 
 // density advection: o[0]/o[1] = rho ping/pong, o[2]/o[3] = u ping/pong
-__global__ __launch_bounds__(256) void kb_advect_cell(const BEntry* tab, int ne, int nbt, char* A, int ping) {
-    BATCH_ENTRY(tab, ne, nbt);
+__global__ __launch_bounds__(256) void kb_advect_cell(BTab T, char* A, int ping) {
+    BATCH_ENTRY(T);
     int x, y, z;
     local_xyz(E.G, lb, x, y, z);
     const Dims& D = E.D;
@@ -65,9 +70,9 @@ __global__ __launch_bounds__(256) void kb_advect_cell(const BEntry* tab, int ne,
 }
 
 // one velocity component: axis 0 (x faces), 1 (y faces), 2 (z faces); o[0]/o[1] = u ping/pong
-__global__ __launch_bounds__(256) void kb_advect_face(const BEntry* tab, int ne, int nbt, char* A, int ping,
+__global__ __launch_bounds__(256) void kb_advect_face(BTab T, char* A, int ping,
                                                       int axis) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     int x, y, z;
     local_xyz(E.G, lb, x, y, z);
     const Dims& D = E.D;
@@ -88,8 +93,8 @@ __global__ __launch_bounds__(256) void kb_advect_face(const BEntry* tab, int ne,
 }
 
 // r = -div u into the red-black layout: o[0] = r, o[1] = u
-__global__ __launch_bounds__(256) void kb_neg_div(const BEntry* tab, int ne, int nbt, char* A) {
-    BATCH_ENTRY(tab, ne, nbt);
+__global__ __launch_bounds__(256) void kb_neg_div(BTab T, char* A) {
+    BATCH_ENTRY(T);
     int x, y, z;
     local_xyz(E.G, lb, x, y, z);
     const Dims& D = E.D;
@@ -100,8 +105,8 @@ __global__ __launch_bounds__(256) void kb_neg_div(const BEntry* tab, int ne, int
 }
 
 // u += dt d0_dual(p) with p red-black: o[0] = u, o[1] = p
-__global__ __launch_bounds__(256) void kb_project(const BEntry* tab, int ne, int nbt, char* A, int axis) {
-    BATCH_ENTRY(tab, ne, nbt);
+__global__ __launch_bounds__(256) void kb_project(BTab T, char* A, int axis) {
+    BATCH_ENTRY(T);
     int x, y, z;
     local_xyz(E.G, lb, x, y, z);
     const Dims& D = E.D;
@@ -123,17 +128,17 @@ __global__ __launch_bounds__(256) void kb_project(const BEntry* tab, int ne, int
 
 // ---- solver kernels --------------------------------------------------------------------------
 
-__global__ void kb_state_init(const BEntry* tab, SolverState* st, int max_iter) {
+__global__ void kb_state_init(BTab T, SolverState* st, int max_iter) {
     if (threadIdx.x != 0) return;
     SolverState s = {};
     s.max_iter = max_iter;
-    st[tab[blockIdx.x].sidx] = s;
+    st[T.e[blockIdx.x].sidx] = s;
 }
 
 template <typename Tr>
-__global__ void kb_round_threshold(const BEntry* tab, SolverState* st) {
+__global__ void kb_round_threshold(BTab T, SolverState* st) {
     if (threadIdx.x != 0) return;
-    SolverState* s = &st[tab[blockIdx.x].sidx];
+    SolverState* s = &st[T.e[blockIdx.x].sidx];
     s->threshold = (double)(Tr)s->threshold;
     if (!s->done && s->b_linf < s->threshold) {  // pcg.rs:35-38
         s->converged = 1;
@@ -143,19 +148,18 @@ __global__ void kb_round_threshold(const BEntry* tab, SolverState* st) {
 
 // One workgroup per grid: reduces the grid's partials of the launch `tab`
 // (blocks [start, start + G.nb)) and updates its state.
-__global__ __launch_bounds__(1024) void kb_finalize(int op, const BEntry* tab, const double* part0,
-                                                    const double* part1, SolverState* st, const SolveCfg* cfg,
-                                                    int use_max) {
-    const BEntry& E = tab[blockIdx.x];
+__global__ __launch_bounds__(1024) void kb_finalize(int op, BTab T, const double* part0, const double* part1,
+                                                    SolverState* st, const SolveCfg* cfg, int use_max) {
+    const BEntry& E = T.e[blockIdx.x];
     finalize_range(op, part0 + E.start, part1 ? part1 + E.start : nullptr, E.G.nb, &st[E.sidx], cfg[E.sidx],
                    use_max);
     if (op == FIN_STATS && threadIdx.x == 0 && !cfg[E.sidx].remove_mean) st[E.sidx].mean = 0.0;
 }
 
 // flat vector kernels over the "flat" table: o[0] = x, o[1] = r, o[2] = z (aux, q), o[3] = p
-__global__ __launch_bounds__(256) void kb_zero_stats(const BEntry* tab, int ne, int nbt, char* A, double* psum,
+__global__ __launch_bounds__(256) void kb_zero_stats(BTab T, char* A, double* psum,
                                                      double* pmax) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
     flat_zero(lb, n, fld<float>(A, E.o[0]));
     double s, m;
@@ -165,25 +169,25 @@ __global__ __launch_bounds__(256) void kb_zero_stats(const BEntry* tab, int ne, 
         pmax[b_] = m;
     }
 }
-__global__ __launch_bounds__(256) void kb_sub_mean(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_sub_mean(BTab T, char* A,
                                                    const SolverState* st, double* pmax) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
     double m = flat_sub_mean(lb, n, fld<float>(A, E.o[1]), st[E.sidx].mean);
     if (threadIdx.x == 0) pmax[b_] = m;
 }
-__global__ __launch_bounds__(256) void kb_copy_dot(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_copy_dot(BTab T, char* A,
                                                    const SolverState* st, double* part) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
     double t = flat_copy_dot(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]),
                              (const float*)fld<float>(A, E.o[1]));
     if (threadIdx.x == 0) part[b_] = t;
 }
-__global__ __launch_bounds__(256) void kb_update(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st,
+__global__ __launch_bounds__(256) void kb_update(BTab T, char* A, const SolverState* st,
                                                  double* pmax) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
     double m, s;
@@ -191,17 +195,17 @@ __global__ __launch_bounds__(256) void kb_update(const BEntry* tab, int ne, int 
                 (const float*)fld<float>(A, E.o[2]), st[E.sidx].alpha, false, &m, &s);
     if (threadIdx.x == 0) pmax[b_] = m;
 }
-__global__ __launch_bounds__(256) void kb_search(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+__global__ __launch_bounds__(256) void kb_search(BTab T, char* A, const SolverState* st) {
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
     flat_search(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]), st[E.sidx].beta);
 }
 
 // red-black pair kernels over rb_[l]: o[0] = z_l, o[1] = r_l, o[2] = res, o[3] = p (level 0)
-__global__ __launch_bounds__(256) void kb_apply(const BEntry* tab, int ne, int nbt, char* A, const SolverState* st,
+__global__ __launch_bounds__(256) void kb_apply(BTab T, char* A, const SolverState* st,
                                                 double* part) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
@@ -217,33 +221,31 @@ __global__ __launch_bounds__(256) void kb_apply(const BEntry* tab, int ne, int n
     if (threadIdx.x == 0 && threadIdx.y == 0) part[b_] = t;
 }
 
-__global__ __launch_bounds__(256) void kb_smooth_first(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_smooth_first(BTab T, char* A,
                                                        const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st && st[E.sidx].done) return;
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
     const Dims& D = E.D;
     if (m >= (D.w >> 1) || y >= D.h) return;
-    RInv<float> ri;
-    for (int k = 0; k < 8; ++k) ri.v[k] = E.rinv[k];
+    const RInv<float>& ri = *reinterpret_cast<const RInv<float>*>(E.rinv);
     float* zv = fld<float>(A, E.o[0]);
     const float* r = fld<float>(A, E.o[1]);
     if (D.d > 1) smooth_first_pair<float, 3>(D, zv, r, (float)E.s, ri, m, y, z);
     else smooth_first_pair<float, 2>(D, zv, r, (float)E.s, ri, m, y, z);
 }
 
-__global__ __launch_bounds__(256) void kb_smooth_half(const BEntry* tab, int ne, int nbt, char* A, int color,
+__global__ __launch_bounds__(256) void kb_smooth_half(BTab T, char* A, int color,
                                                       double* part, const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st && st[E.sidx].done) return;
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
     const Dims& D = E.D;
     double acc = 0.0;
     if (m < (D.w >> 1) && y < D.h) {
-        RInv<float> ri;
-        for (int k = 0; k < 8; ++k) ri.v[k] = E.rinv[k];
+        const RInv<float>& ri = *reinterpret_cast<const RInv<float>*>(E.rinv);
         float* zv = fld<float>(A, E.o[0]);
         const float* r = fld<float>(A, E.o[1]);
         acc = D.d > 1 ? smooth_half_cell<float, 3>(D, zv, r, (float)E.s, ri, color, m, y, z)
@@ -255,9 +257,9 @@ __global__ __launch_bounds__(256) void kb_smooth_half(const BEntry* tab, int ne,
     }
 }
 
-__global__ __launch_bounds__(256) void kb_residual(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_residual(BTab T, char* A,
                                                    const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
@@ -272,9 +274,9 @@ __global__ __launch_bounds__(256) void kb_residual(const BEntry* tab, int ne, in
 
 // restriction into level l+1 over co_[l]: o[0] = r_{l+1}, o[1] = res (level l); LC = coarse layout
 template <int LC>
-__global__ __launch_bounds__(256) void kb_restrict(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_restrict(BTab T, char* A,
                                                    const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int I, J, K;
     local_xyz(E.G, lb, I, J, K);
@@ -287,9 +289,9 @@ __global__ __launch_bounds__(256) void kb_restrict(const BEntry* tab, int ne, in
 
 // prolongation from level l+1 over prl_[l]: o[0] = z_l, o[1] = z_{l+1}
 template <int LC>
-__global__ __launch_bounds__(256) void kb_prolong(const BEntry* tab, int ne, int nbt, char* A,
+__global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
                                                   const SolverState* st) {
-    BATCH_ENTRY(tab, ne, nbt);
+    BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
@@ -300,32 +302,17 @@ __global__ __launch_bounds__(256) void kb_prolong(const BEntry* tab, int ne, int
     else prolong_pair<float, 2, LC>(E.D, E.Dc, zf, zc, m, y, z);
 }
 
-__device__ __forceinline__ int find_fdm(const BFdm* e, int n, int b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (e[mid].start <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+#define FDM_ENTRY(T) const BFdm& F = T.e[find_entry(T.start, T.n, blockIdx.x)];
 
-// Coarsest level, pass `phase` 0 (forward x,y; a 2D grid solves completely)
-// or 1 (inverse y,x of a 3D grid). One workgroup per z-plane of each grid.
-__global__ __launch_bounds__(256) void kb_fdm_plane(const BFdm* tab, int ne, char* A, int phase,
-                                                    const SolverState* st) {
+// 3D coarsest level, pass `phase` 0 (forward x,y) or 1 (inverse y,x): one
+// workgroup per z-plane of each grid.
+__global__ __launch_bounds__(256) void kb_fdm_plane(FTab T, char* A, int phase, const SolverState* st) {
     extern __shared__ double fdm_lds[];
-    const BFdm& F = tab[find_fdm(tab, ne, blockIdx.x)];
+    FDM_ENTRY(T);
     if (st[F.sidx].done) return;
     int k = blockIdx.x - F.start;
     int64_t P = (int64_t)F.nx * F.ny;
     const double eps = 1e-10;
-    if (!F.is3d) {
-        if (phase == 0)
-            fdm_plane_body<float, float>(fdm_lds, F.nx, F.ny, 2, fld<float>(A, F.o_r), fld<float>(A, F.o_z), F.Vx,
-                                         F.Vy, F.lx, F.ly, F.s, eps);
-        return;
-    }
     if (phase == 0)
         fdm_plane_body<float, double>(fdm_lds, F.nx, F.ny, 0, fld<float>(A, F.o_r) + k * P,
                                       fld<double>(A, F.o_t0) + k * P, F.Vx, F.Vy, F.lx, F.ly, F.s, eps);
@@ -334,13 +321,35 @@ __global__ __launch_bounds__(256) void kb_fdm_plane(const BFdm* tab, int ne, cha
                                       fld<float>(A, F.o_z) + k * P, F.Vx, F.Vy, F.lx, F.ly, F.s, eps);
 }
 
-// Coarsest level of the 3D grids: one workgroup per y-row.
-__global__ __launch_bounds__(256) void kb_fdm_zcol(const BFdm* tab, int ne, char* A, const SolverState* st) {
+// 3D coarsest level: one workgroup per y-row.
+__global__ __launch_bounds__(256) void kb_fdm_zcol(FTab T, char* A, const SolverState* st) {
     extern __shared__ double fdm_lds[];
-    const BFdm& F = tab[find_fdm(tab, ne, blockIdx.x)];
+    FDM_ENTRY(T);
     if (st[F.sidx].done) return;
     fdm_zcol_body(fdm_lds, F.nx, F.ny, F.nz, blockIdx.x - F.start, fld<double>(A, F.o_t0), F.Vz, F.lx, F.ly, F.lz,
                   F.s, 1e-10);
+}
+
+// 2D coarsest level: row pass 0 (forward x) or 2 (inverse x) over row blocks.
+__global__ __launch_bounds__(256) void kb_fdm2_rows(FTab T, char* A, int pass, const SolverState* st) {
+    extern __shared__ double fdm_lds[];
+    FDM_ENTRY(T);
+    if (st[F.sidx].done) return;
+    int blk = blockIdx.x - F.start;
+    if (pass == 0)
+        fdm2_rows_body<float, double>(fdm_lds, F.nx, F.ny, blk, 0, (const float*)fld<float>(A, F.o_r),
+                                      fld<double>(A, F.o_t0), F.Vx);
+    else
+        fdm2_rows_body<double, float>(fdm_lds, F.nx, F.ny, blk, 1, (const double*)fld<double>(A, F.o_t0),
+                                      fld<float>(A, F.o_z), F.Vx);
+}
+
+// 2D coarsest level: column pass over column blocks.
+__global__ __launch_bounds__(256) void kb_fdm2_cols(FTab T, char* A, const SolverState* st) {
+    extern __shared__ double fdm_lds[];
+    FDM_ENTRY(T);
+    if (st[F.sidx].done) return;
+    fdm2_cols_body(fdm_lds, F.nx, F.ny, blockIdx.x - F.start, fld<double>(A, F.o_t0), F.Vy, F.lx, F.ly, F.s, 1e-10);
 }
 
 // ---- host side -------------------------------------------------------------------------------
@@ -367,9 +376,10 @@ static Dims level_dims(const Dims& D, int l) {
 pano_status hgrid_check(const Dims& D, int levels) {
     PANO_CHECK(mg_check(D, levels));
     Dims C = level_dims(D, levels - 1);
-    if (C.w * C.h > kFdmPlane || (D.d > 1 && C.w * C.d > kFdmPlane))
-        return fail(PANO_ERR_BAD_SHAPE, "batched solve: coarsest level %dx%dx%d larger than %d cells per plane", C.d,
-                    C.h, C.w, kFdmPlane);
+    bool fits = D.d > 1 ? (C.w * C.h <= kFdmPlane && C.w * C.d <= kFdmPlane) : (C.w <= kFdmPlane && C.h <= kFdmPlane);
+    if (!fits)
+        return fail(PANO_ERR_BAD_SHAPE, "batched solve: coarsest level %dx%dx%d too large for the LDS solve (%d cells)",
+                    C.d, C.h, C.w, kFdmPlane);
     return PANO_OK;
 }
 
@@ -439,7 +449,7 @@ BatchCore::~BatchCore() {
     rel(nat_.d), rel(fx_.d), rel(fy_.d), rel(fz_.d), rel(flat_.d);
     for (int p = 0; p < 2; ++p) rel(div_[p].d), rel(prx_[p].d), rel(pry_[p].d), rel(prz_[p].d);
     for (int l = 0; l < 16; ++l) rel(rb_[l].d), rel(co_[l].d), rel(prl_[l].d);
-    rel(plane_.d), rel(zcol_.d);
+    rel(plane_.d), rel(zcol_.d), rel(rows2_.d), rel(cols2_.d);
     rel(states), rel(cfgs), rel(part0_), rel(part1_), rel(part2_);
 }
 
@@ -451,22 +461,38 @@ pano_status BatchCore::ensure_slots(int n) {
     return PANO_OK;
 }
 
-pano_status BatchCore::upload(Table& t) {
-    t.nbt = 0;
-    for (auto& e : t.h) t.nbt = std::max(t.nbt, e.start + e.G.nb);
-    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, std::max<size_t>(1, t.h.size()) * sizeof(BEntry)));
-    if (!t.h.empty())
-        PANO_HIP(hipMemcpyAsync(t.d, t.h.data(), t.h.size() * sizeof(BEntry), hipMemcpyHostToDevice, c->stream));
+// Entries then their first blocks, in one allocation and one copy. The
+// staging vector stays alive until the next upload of the table, after the
+// stream has been synchronised.
+template <typename TableT>
+static pano_status upload_table(Ctx* c, TableT& t, int nbt) {
+    using Entry = typename std::remove_reference<decltype(t.h[0])>::type;
+    if (t.h.size() > (size_t)kMaxEntries)
+        return fail(PANO_ERR_INVALID, "batched launch with %zu grids (at most %d)", t.h.size(), kMaxEntries);
+    t.nbt = nbt;
+    size_t n = t.h.size();
+    size_t bytes = std::max<size_t>(1, n * (sizeof(Entry) + sizeof(int)));
+    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, bytes));
+    if (n == 0) return PANO_OK;
+    PANO_HIP(hipStreamSynchronize(c->stream));
+    t.stage.resize(n * (sizeof(Entry) + sizeof(int)));
+    memcpy(t.stage.data(), t.h.data(), n * sizeof(Entry));
+    int* st = reinterpret_cast<int*>(t.stage.data() + n * sizeof(Entry));
+    for (size_t i = 0; i < n; ++i) st[i] = t.h[i].start;
+    PANO_HIP(hipMemcpyAsync(t.d, t.stage.data(), t.stage.size(), hipMemcpyHostToDevice, c->stream));
     return PANO_OK;
 }
 
+pano_status BatchCore::upload(Table& t) {
+    int nbt = 0;
+    for (auto& e : t.h) nbt = std::max(nbt, e.start + e.G.nb);
+    return upload_table(c, t, nbt);
+}
+
 pano_status BatchCore::upload(FdmTable& t) {
-    t.nbt = 0;
-    for (auto& e : t.h) t.nbt = std::max(t.nbt, e.start + e.nblk);
-    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, std::max<size_t>(1, t.h.size()) * sizeof(BFdm)));
-    if (!t.h.empty())
-        PANO_HIP(hipMemcpyAsync(t.d, t.h.data(), t.h.size() * sizeof(BFdm), hipMemcpyHostToDevice, c->stream));
-    return PANO_OK;
+    int nbt = 0;
+    for (auto& e : t.h) nbt = std::max(nbt, e.start + e.nblk);
+    return upload_table(c, t, nbt);
 }
 
 pano_status BatchCore::ensure_partials(size_t n) {
@@ -498,9 +524,12 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     }
     plane_.h.clear();
     zcol_.h.clear();
+    rows2_.h.clear();
+    cols2_.h.clear();
     levels_ = ids.empty() ? 0 : grids[ids[0]].levels;
     any3d_ = false;
     int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
+    int srow = 0, scol = 0;
     size_t maxp = 0;
     for (int id : ids) {
         const HGrid& g = grids[id];
@@ -601,6 +630,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 co_[l].h.push_back(co);
             } else {
                 BFdm F;
+                memset(&F, 0, sizeof(F));
                 F.sidx = g.sidx;
                 F.is3d = D.d > 1;
                 F.nx = L.w;
@@ -616,16 +646,26 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 F.o_r = g.o_rl[l];
                 F.o_z = g.o_zl[l];
                 F.o_t0 = g.o_t0;
-                F.start = splane;
-                F.nblk = L.d;
-                splane += F.nblk;
-                plane_.h.push_back(F);
                 if (D.d > 1) {
+                    F.start = splane;
+                    F.nblk = L.d;
+                    splane += F.nblk;
+                    plane_.h.push_back(F);
                     BFdm Z = F;
                     Z.start = szcol;
                     Z.nblk = L.h;
                     szcol += Z.nblk;
                     zcol_.h.push_back(Z);
+                } else {
+                    F.start = srow;
+                    F.nblk = (L.h + fdm2_rows_per(L.w) - 1) / fdm2_rows_per(L.w);
+                    srow += F.nblk;
+                    rows2_.h.push_back(F);
+                    BFdm Cc = F;
+                    Cc.start = scol;
+                    Cc.nblk = (L.w + fdm2_cols_per(L.h) - 1) / fdm2_cols_per(L.h);
+                    scol += Cc.nblk;
+                    cols2_.h.push_back(Cc);
                 }
             }
             s *= 0.25;
@@ -651,21 +691,22 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     }
     PANO_CHECK(upload(plane_));
     PANO_CHECK(upload(zcol_));
+    PANO_CHECK(upload(rows2_));
+    PANO_CHECK(upload(cols2_));
     PANO_CHECK(ensure_partials(std::max<size_t>(maxp, 1)));
     return PANO_OK;
 }
 
-#define BL(kern, T_, ...)                                                                            \
-    do {                                                                                              \
-        if ((T_).nbt > 0)                                                                             \
-            hipLaunchKernelGGL(kern, dim3((T_).nbt), dim3(blk.x, blk.y), 0, c->stream, (T_).d,         \
-                               (int)(T_).h.size(), (T_).nbt, __VA_ARGS__);                            \
+#define BL(kern, T_, ...)                                                                                 \
+    do {                                                                                                   \
+        if ((T_).nbt > 0)                                                                                  \
+            hipLaunchKernelGGL(kern, dim3((T_).nbt), dim3(blk.x, blk.y), 0, c->stream, (T_).view(), __VA_ARGS__); \
     } while (0)
 
 void BatchCore::fin(int op, const double* p0, const double* p1, const Table& t, int use_max) {
     if (t.h.empty()) return;
-    hipLaunchKernelGGL(kb_finalize, dim3((unsigned)t.h.size()), dim3(1024), 0, c->stream, op, (const BEntry*)t.d,
-                       p0, p1, states, (const SolveCfg*)cfgs, use_max);
+    hipLaunchKernelGGL(kb_finalize, dim3((unsigned)t.h.size()), dim3(1024), 0, c->stream, op, t.view(), p0, p1,
+                       states, (const SolveCfg*)cfgs, use_max);
 }
 
 pano_status BatchCore::enqueue_advect(int ping) {
@@ -712,15 +753,22 @@ pano_status BatchCore::vcycle(bool with_dot) {
         else BL(kb_restrict<REDBLACK>, co_[l], arena, (const SolverState*)states);
         blk = sblk;
     }
+    if (!rows2_.h.empty()) {
+        const SolverState* cst = states;
+        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), fdm_lds_, c->stream, rows2_.view(), arena, 0,
+                           cst);
+        hipLaunchKernelGGL(kb_fdm2_cols, dim3(cols2_.nbt), dim3(256), fdm_lds_, c->stream, cols2_.view(), arena,
+                           cst);
+        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), fdm_lds_, c->stream, rows2_.view(), arena, 2,
+                           cst);
+    }
     if (!plane_.h.empty()) {
-        hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream, (const BFdm*)plane_.d,
-                           (int)plane_.h.size(), arena, 0, (const SolverState*)states);
-        if (!zcol_.h.empty()) {
-            hipLaunchKernelGGL(kb_fdm_zcol, dim3(zcol_.nbt), dim3(256), fdm_lds_, c->stream, (const BFdm*)zcol_.d,
-                               (int)zcol_.h.size(), arena, (const SolverState*)states);
-            hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream,
-                               (const BFdm*)plane_.d, (int)plane_.h.size(), arena, 1, (const SolverState*)states);
-        }
+        const SolverState* cst = states;
+        hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream, plane_.view(), arena, 0,
+                           cst);
+        hipLaunchKernelGGL(kb_fdm_zcol, dim3(zcol_.nbt), dim3(256), fdm_lds_, c->stream, zcol_.view(), arena, cst);
+        hipLaunchKernelGGL(kb_fdm_plane, dim3(plane_.nbt), dim3(256), fdm_lds_, c->stream, plane_.view(), arena, 1,
+                           cst);
     }
     for (int l = Lc - 1; l >= 0; --l) {
         if (l + 1 == Lc) BL(kb_prolong<NATURAL>, prl_[l], arena, (const SolverState*)states);
@@ -739,15 +787,15 @@ pano_status BatchCore::vcycle(bool with_dot) {
 // threshold from ||b||_inf, z = M r, p = z, sigma = z.r.
 pano_status BatchCore::enqueue_init() {
     if (flat_.h.empty()) return PANO_OK;
-    hipLaunchKernelGGL(kb_state_init, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream, (const BEntry*)flat_.d,
-                       states, max_iterations);
+    hipLaunchKernelGGL(kb_state_init, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream, flat_.view(), states,
+                       max_iterations);
     dim3 blk(kBlock, 1);
     BL(kb_zero_stats, flat_, arena, part0_, part1_);
     fin(FIN_STATS, part0_, nullptr, flat_, 0);
     BL(kb_sub_mean, flat_, arena, (const SolverState*)states, part1_);
     fin(FIN_THRESH, part1_, nullptr, flat_, 1);
     hipLaunchKernelGGL(kb_round_threshold<float>, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream,
-                       (const BEntry*)flat_.d, states);
+                       flat_.view(), states);
     PANO_CHECK(vcycle(false));
     blk = dim3(kBlock, 1);
     BL(kb_copy_dot, flat_, arena, (const SolverState*)states, part0_);

@@ -43,6 +43,22 @@ struct BFdm {
     int64_t o_r, o_z, o_t0;  // arena byte offsets: coarse rhs (f32), solution (f32), FP64 temp
 };
 
+// Device view of a launch table: entries sorted by first block, and their
+// first blocks in a compact array for the lookup.
+struct BTab {
+    const BEntry* e;
+    const int* start;
+    int n;    // entries (at most kMaxEntries)
+    int nbt;  // blocks of the launch
+};
+struct FTab {
+    const BFdm* e;
+    const int* start;
+    int n;
+    int nbt;
+};
+constexpr int kMaxEntries = 4096;
+
 // Host description of one grid of the batch and where its fields live.
 struct HGrid {
     Dims D;
@@ -104,15 +120,23 @@ public:
 private:
     struct Table {
         std::vector<BEntry> h;
-        BEntry* d = nullptr;
-        int nbt = 0;   // total blocks
+        BEntry* d = nullptr;   // entries, then their first blocks (int)
+        int nbt = 0;           // total blocks
         size_t cap = 0;
+        std::vector<char> stage;
+        BTab view() const {
+            return BTab{d, reinterpret_cast<const int*>(d + h.size()), (int)h.size(), nbt};
+        }
     };
     struct FdmTable {
         std::vector<BFdm> h;
         BFdm* d = nullptr;
         int nbt = 0;
         size_t cap = 0;
+        std::vector<char> stage;
+        FTab view() const {
+            return FTab{d, reinterpret_cast<const int*>(d + h.size()), (int)h.size(), nbt};
+        }
     };
     Table nat_;            // level-0 cells, natural tiles (advection)
     Table fx_, fy_, fz_;   // face tiles per component (advection)
@@ -122,7 +146,8 @@ private:
     Table rb_[16];         // red-black pair tiles per level
     Table co_[16];         // coarse-cell tiles of the restriction into level l+1
     Table prl_[16];        // pair tiles of level l for the prolongation from l+1
-    FdmTable plane_, zcol_;
+    FdmTable plane_, zcol_;  // 3D coarsest solve
+    FdmTable rows2_, cols2_;  // 2D coarsest solve
     int levels_ = 0;
     bool any3d_ = false;
     size_t fdm_lds_ = 0;

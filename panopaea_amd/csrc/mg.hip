@@ -266,6 +266,21 @@ __global__ __launch_bounds__(256) void k_fdm_plane(int nx, int ny, int mode, con
                               eps);
 }
 
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(256) void k_fdm2_rows(int nx, int ny, int inverse, const Tin* in, Tout* out,
+                                                   const double* Vx, const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    fdm2_rows_body<Tin, Tout>(fdm_lds, nx, ny, blockIdx.x, inverse, in, out, Vx);
+}
+
+__global__ __launch_bounds__(256) void k_fdm2_cols(int nx, int ny, double* t, const double* Vy, const double* lx,
+                                                   const double* ly, double s, double eps, const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    fdm2_cols_body(fdm_lds, nx, ny, blockIdx.x, t, Vy, lx, ly, s, eps);
+}
+
 __global__ __launch_bounds__(256) void k_fdm_zcol(int nx, int ny, int nz, double* data, const double* Vz,
                                                   const double* lx, const double* ly, const double* lz, double s,
                                                   double eps, const SolverState* st) {
@@ -286,12 +301,21 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     hipStream_t S = c->stream;
     const double *Vx = m->fdm_v[0], *Vy = m->fdm_v[1], *Vz = m->fdm_v[2];
     const double *lx = m->fdm_lam[0], *ly = m->fdm_lam[1], *lz = m->fdm_lam[2];
-    if (nx * ny <= kFdmPlane && (DIM == 2 || nx * nz <= kFdmPlane)) {
+    if (DIM == 2 && nx <= kFdmPlane && ny <= kFdmPlane) {
+        // rows (forward x), columns (y, divide, inverse y), rows (inverse x)
         size_t lds = 2 * sizeof(double) * (size_t)kFdmPlane;
-        if (DIM == 2) {
-            hipLaunchKernelGGL((k_fdm_plane<T, T>), dim3(1), dim3(256), lds, S, nx, ny, 2, r, z, Vx, Vy, lx, ly, s,
-                               eps, st);
-        } else {
+        unsigned nr = (unsigned)((ny + fdm2_rows_per(nx) - 1) / fdm2_rows_per(nx));
+        unsigned ncol = (unsigned)((nx + fdm2_cols_per(ny) - 1) / fdm2_cols_per(ny));
+        hipLaunchKernelGGL((k_fdm2_rows<T, double>), dim3(nr), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, st);
+        hipLaunchKernelGGL(k_fdm2_cols, dim3(ncol), dim3(256), lds, S, nx, ny, t0, Vy, lx, ly, s, eps, st);
+        hipLaunchKernelGGL((k_fdm2_rows<double, T>), dim3(nr), dim3(256), lds, S, nx, ny, 1, (const double*)t0, z,
+                           Vx, st);
+        PANO_HIP(hipGetLastError());
+        return PANO_OK;
+    }
+    if (DIM == 3 && nx * ny <= kFdmPlane && nx * nz <= kFdmPlane) {
+        size_t lds = 2 * sizeof(double) * (size_t)kFdmPlane;
+        {
             hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, Vy,
                                (const double*)nullptr, (const double*)nullptr, s, eps, st);
             hipLaunchKernelGGL(k_fdm_zcol, dim3(ny), dim3(256), lds, S, nx, ny, nz, t0, Vz, lx, ly, lz, s, eps, st);

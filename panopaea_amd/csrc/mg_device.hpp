@@ -307,6 +307,68 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
     for (int e = threadIdx.x; e < P; e += blockDim.x) dst[e] = (Tout)a[e];
 }
 
+// 2D coarsest level as three passes over lines, each output element summed in
+// the order of lds_axis (so the result equals fdm_plane_body mode 2):
+//  pass 0, rows: t[j,i] = sum_q Vx[q*nx+i] * r[j*nx+q]        (forward x)
+//  pass 1, columns: u[j,i] = sum_q Vy[q*ny+j] * t[q*nx+i]    (forward y),
+//          u /= s*(lx[i]+ly[j]) (0 on the null space), t[j,i] = sum_k Vy[j*ny+k] * u[k*nx+i]
+//  pass 2, rows: z[j,i] = sum_k Vx[i*nx+k] * t[j*nx+k]        (inverse x)
+// Rows per workgroup: fdm2_rows_per(nx); columns per workgroup: fdm2_cols_per(ny).
+__host__ __device__ inline int fdm2_rows_per(int nx) { return nx >= 256 ? 1 : 256 / nx; }
+__host__ __device__ inline int fdm2_cols_per(int ny) { return ny >= 256 ? 1 : 256 / ny; }
+
+// rows [blk*R, blk*R + R) of pass 0 (Tin -> double t) or pass 2 (double t -> Tout); lds: R*nx doubles
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int blk, int inverse, const Tin* src,
+                                               Tout* dst, const double* Vx) {
+    const int R = fdm2_rows_per(nx);
+    const int j0 = blk * R;
+    const int nr = min(R, ny - j0);
+    const int P = nr * nx;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) lds[e] = (double)src[(int64_t)j0 * nx + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int jl = e / nx, i = e - jl * nx;
+        const double* a = lds + jl * nx;
+        double acc = 0.0;
+        if (inverse)
+            for (int k = 0; k < nx; ++k) acc += Vx[i * nx + k] * a[k];
+        else
+            for (int q = 0; q < nx; ++q) acc += Vx[q * nx + i] * a[q];
+        dst[(int64_t)j0 * nx + e] = (Tout)acc;
+    }
+}
+
+// columns [blk*C, blk*C + C) of pass 1, in place on t; lds: 2*ny*C doubles
+__device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int blk, double* t, const double* Vy,
+                                               const double* lx, const double* ly, double s, double eps) {
+    const int C = fdm2_cols_per(ny);
+    const int i0 = blk * C;
+    const int nc = min(C, nx - i0);
+    const int P = ny * nc;
+    double* a = lds;
+    double* b = lds + ny * C;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int j = e / nc, ci = e - j * nc;
+        a[j * C + ci] = t[(int64_t)j * nx + i0 + ci];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int j = e / nc, ci = e - j * nc;
+        double acc = 0.0;
+        for (int q = 0; q < ny; ++q) acc += Vy[q * ny + j] * a[q * C + ci];
+        double lam = lx[i0 + ci] + ly[j];
+        b[j * C + ci] = lam > eps ? acc / (s * lam) : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int j = e / nc, ci = e - j * nc;
+        double acc = 0.0;
+        for (int k = 0; k < ny; ++k) acc += Vy[j * ny + k] * b[k * C + ci];
+        t[(int64_t)j * nx + i0 + ci] = acc;
+    }
+}
+
 // One y-row j of the 3D coarsest level: forward z, divide by s*(lx+ly+lz),
 // inverse z, in place on the (nz x nx) slice of data (lds: 2*nx*nz doubles).
 __device__ __forceinline__ void fdm_zcol_body(double* lds, int nx, int ny, int nz, int j, double* data,

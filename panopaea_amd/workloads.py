@@ -9,7 +9,13 @@ Resolutions of details the workload statements leave open (stated once):
   (z, y, x) = (0.1 N, N/2, N/2), reset to density 1 every step; buoyancy
   w += dt * beta * (rho[z-1] + rho[z]) / 2 on z faces, from the density at
   the start of the step;
-- the Poisson-only 512^3 problem has closed walls (the mean of b removed).
+- the Poisson-only 512^3 problem has closed walls (the mean of b removed);
+- the batch of 64 grids: grid i's side N_i is SIZES[floor(u_i * 7)] with u_i
+  the i-th SplitMix64 draw from seed 7 (top 53 bits / 2^53); grid i's
+  velocity has seed i (grid 0 is the 128^2 workload's seed), 16 modes,
+  k <= 8; density a Gaussian of sigma 0.1 N at the centre; closed walls;
+  the pressure solve is MG-PCG (4 levels, 2+2 sweeps, every side a multiple
+  of 8) rather than Jacobi, to 1e-6 relative L-inf.
 """
 import numpy as np
 
@@ -36,6 +42,52 @@ POISSON512_MIXED = dict(
     name="poisson512_mixed", dims=(512, 512, 512), seed=0, bc=(0, 0, 0, 0, 0, 0), precond="mg", mg_levels=4,
     mg_nu=2, inner_tol=1e-4, outer_tol=1e-10, dt=1.0,
 )
+
+
+# 64 independent square 2D grids per launch, 20 steps each at dt = 0.5/N.
+BATCH64_2D = dict(
+    name="batch64_2d", count=64, size_seed=7, sizes=(64, 96, 128, 192, 256, 384, 512), steps=20, n_modes=16,
+    k_max=8.0, sigma_frac=0.1, bc=(0, 0, 0, 0, 0, 0), mg_levels=4, mg_nu=2, tol=1e-6,
+)
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(seed):
+    """SplitMix64 stream (Steele, Lea, Flood 2014)."""
+    state = seed & _M64
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & _M64
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        yield z ^ (z >> 31)
+
+
+def uniform01(seed):
+    """Uniform [0, 1) doubles from the top 53 bits of SplitMix64."""
+    for z in splitmix64(seed):
+        yield (z >> 11) * (1.0 / (1 << 53))
+
+
+def batch_grids(wl=BATCH64_2D):
+    """The batch's grid descriptions: [dict(dims=(N, N), dt=0.5/N, bc, seed)]."""
+    u = uniform01(wl["size_seed"])
+    sizes = wl["sizes"]
+    out = []
+    for i in range(wl["count"]):
+        n = sizes[min(int(next(u) * len(sizes)), len(sizes) - 1)]
+        out.append(dict(dims=(n, n), dt=0.5 / n, bc=wl["bc"], seed=i))
+    return out
+
+
+def batch_initial(wl, g):
+    """Host (density, faces) of one batch grid."""
+    grid = Grid2d(g["dims"], np.float32, bc=g["bc"])
+    faces = gen_bandlimited_faces(grid, g["seed"], wl["n_modes"], wl["k_max"], 1.0)
+    h, w = g["dims"]
+    rho = gen_gaussian_cells(grid, (0.0, h / 2.0, w / 2.0), wl["sigma_frac"] * w)
+    return rho, faces
 
 
 def make_grid(wl, dims=None, dtype=np.float32, ctx=None):
