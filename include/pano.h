@@ -247,6 +247,62 @@ pano_status pano_batch_step(pano_batch, int32_t* iterations);
  * candidates: 64x4, 32x8, 128x2, 16x16). */
 pano_status pano_batch_tiles(pano_batch, int32_t* tile_nat, int32_t* tile_rb, double* tune_us);
 
+/* ---- serving: continuous batching of projection solves ---------------------- */
+/* No reference counterpart. A request is one projection of a velocity field
+ * (b = -div u, MG-PCG to a relative L-inf tolerance, u -= dt grad p), the
+ * solve of dec_fluid.rs:68-141 without the source and advection. Requests
+ * wait in FIFO order and are admitted when their fields fit in the device
+ * memory budget and a solver slot is free; every tick advances all admitted
+ * requests together (new ones: divergence and CG set-up; all: CG
+ * iterations; converged ones: projection), so requests arriving while
+ * others run join the running batch. Fields stay on the device until
+ * released. A request's result does not depend on the other requests
+ * (same tile shapes). */
+typedef struct pano_server_s* pano_server;
+typedef struct {
+    int64_t hbm_budget;      /* bytes of device memory for request fields */
+    int32_t max_active;      /* admitted requests at once (1..4096) */
+    int32_t mg_levels, mg_nu;
+    double tol;              /* relative L-inf residual */
+    int32_t max_iterations;
+    int32_t iters_per_tick;  /* CG iterations enqueued per tick */
+    int32_t tile_nat[2];     /* (x, y) threads, x*y = 256 */
+    int32_t tile_rb[2];
+} pano_server_params;
+typedef struct {
+    pano_batch_grid grid;    /* dims, walls/open sides, dt (f32 fields) */
+    const float* faces;      /* host velocity (copied at submit), or NULL: seeded band-limited field */
+    uint64_t seed;           /* band-limited field when faces is NULL */
+    int32_t n_modes;
+    double k_max, amplitude;
+} pano_server_request;
+#define PANO_REQ_QUEUED 0
+#define PANO_REQ_ADMITTED 1
+#define PANO_REQ_DONE 2
+typedef struct {
+    int32_t state;           /* PANO_REQ_* */
+    int32_t iterations, converged;
+    double r_linf, b_linf;
+    double t_submit, t_admit, t_done;  /* seconds on the server clock */
+    int64_t cells, bytes;    /* cells and device bytes of the request */
+} pano_server_query;
+pano_status pano_server_create(pano_ctx, const pano_server_params*, pano_server* out);
+pano_status pano_server_destroy(pano_server);
+/* Queue a request; *id identifies it. */
+pano_status pano_server_submit(pano_server, const pano_server_request*, int64_t* id);
+/* Admit, advance and retire; returns the queue/active/done counts, the
+ * device bytes in use and the server clock (s). */
+pano_status pano_server_tick(pano_server, int32_t* queued, int32_t* active, int64_t* done_total, int64_t* bytes_used,
+                             double* now);
+/* Ids of requests that completed since the last call (up to cap). */
+pano_status pano_server_completed(pano_server, int64_t* ids, int32_t cap, int32_t* n);
+pano_status pano_server_status(pano_server, int64_t id, pano_server_query* out);
+/* Host copies of a completed request's velocity (faces) and pressure
+ * (cells, natural order); either may be NULL. */
+pano_status pano_server_fetch(pano_server, int64_t id, float* faces, float* pressure);
+/* Frees a completed request's device memory and forgets it. */
+pano_status pano_server_release(pano_server, int64_t id);
+
 /* ---- seeded synthetic inputs (host, deterministic) ------------------------- */
 /* Band-limited velocity: each face component is the sum of n_modes
  * sinusoids amplitude*sin(2*pi*(k.X)/N + phi), integer k with

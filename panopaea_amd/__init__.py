@@ -29,7 +29,7 @@ from ._lib import (PanoError, PcgInfo, PcgParams, StepParams, GridDesc, PANO_OK,
 __all__ = [
     "Context", "default_context", "DeviceField", "Grid2d", "Grid3d", "Identity", "Jacobi", "Multigrid",
     "PoissonOperator", "precond_conjugate_gradient", "advect", "advect_mac", "neg_divergence", "project",
-    "fluid_step", "PanoError", "StepConfig", "Batch",
+    "fluid_step", "PanoError", "StepConfig", "Batch", "Server",
 ]
 
 
@@ -412,6 +412,111 @@ class Batch:
     def close(self):
         if getattr(self, "handle", None):
             _lib.lib().pano_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _batch_grid(g):
+    a = _lib.BatchGrid()
+    dims = tuple(g["dims"])
+    a.ndim = len(dims)
+    a.nz, a.ny, a.nx = (1,) + dims if len(dims) == 2 else dims
+    bc = tuple(g.get("bc") or (0,) * 6)
+    for j in range(6):
+        a.bc[j] = int(bc[j]) if j < len(bc) else 0
+    a.dt = float(g["dt"])
+    return a
+
+
+class Server:
+    """Continuous batching of projection solves (pano_server_*): requests are
+    admitted by device-memory budget, advanced together tick by tick, and
+    keep their fields on the device until released."""
+
+    QUEUED, ADMITTED, DONE = 0, 1, 2
+
+    def __init__(self, hbm_budget=16 << 30, max_active=256, mg_levels=4, mg_nu=2, tol=1e-6, max_iterations=500,
+                 iters_per_tick=2, tile_nat=(64, 4), tile_rb=(64, 4), ctx=None):
+        self.ctx = ctx or default_context()
+        P = _lib.ServerParams()
+        P.hbm_budget, P.max_active, P.mg_levels, P.mg_nu = int(hbm_budget), max_active, mg_levels, mg_nu
+        P.tol, P.max_iterations, P.iters_per_tick = tol, max_iterations, iters_per_tick
+        P.tile_nat[0], P.tile_nat[1] = tile_nat
+        P.tile_rb[0], P.tile_rb[1] = tile_rb
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().pano_server_create(self.ctx.handle, ctypes.byref(P), ctypes.byref(h)))
+        self.handle = h
+        self._geo = {}
+
+    def submit(self, grid, faces=None, seed=0, n_modes=16, k_max=8.0, amplitude=1.0):
+        """grid = dict(dims, bc, dt); faces: host f32 velocity, or None for the
+        seeded band-limited field. Returns the request id."""
+        q = _lib.ServerRequest()
+        q.grid = _batch_grid(grid)
+        arr = None
+        if faces is not None:
+            arr = np.ascontiguousarray(faces, np.float32)
+            q.faces = arr.ctypes.data
+        q.seed, q.n_modes, q.k_max, q.amplitude = seed, n_modes, k_max, amplitude
+        rid = ctypes.c_int64()
+        _lib.check(_lib.lib().pano_server_submit(self.handle, ctypes.byref(q), ctypes.byref(rid)))
+        self._geo[rid.value] = _Grid2d_or_3d(grid["dims"], grid.get("bc"))
+        return rid.value
+
+    def tick(self):
+        qd, ac, dn, used, now = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        _lib.check(_lib.lib().pano_server_tick(self.handle, ctypes.byref(qd), ctypes.byref(ac), ctypes.byref(dn),
+                                               ctypes.byref(used), ctypes.byref(now)))
+        return dict(queued=qd.value, active=ac.value, done_total=dn.value, bytes_used=used.value, now=now.value)
+
+    def completed(self, cap=4096):
+        ids = (ctypes.c_int64 * cap)()
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().pano_server_completed(self.handle, ids, cap, ctypes.byref(n)))
+        return list(ids[:n.value])
+
+    def status(self, rid):
+        q = _lib.ServerQuery()
+        _lib.check(_lib.lib().pano_server_status(self.handle, rid, ctypes.byref(q)))
+        return {k: getattr(q, k) for k, _ in _lib.ServerQuery._fields_}
+
+    def fetch(self, rid):
+        g = self._geo[rid]
+        f = np.empty(g.num_elem_1(), np.float32)
+        p = np.empty(g.num_elem_2(), np.float32)
+        _lib.check(_lib.lib().pano_server_fetch(self.handle, rid, f.ctypes.data, p.ctypes.data))
+        return f, p
+
+    def release(self, rid):
+        _lib.check(_lib.lib().pano_server_release(self.handle, rid))
+        self._geo.pop(rid, None)
+
+    def run_until_idle(self, on_done=None, max_ticks=1000000):
+        """Tick until nothing is queued or running. on_done(id), if given, is
+        called for each completed request (it may fetch and release it:
+        completed requests hold their memory until released, so a budget
+        that is full of them admits nothing)."""
+        for _ in range(max_ticks):
+            t = self.tick()
+            if on_done is not None:
+                for rid in self.completed():
+                    on_done(rid)
+            if t["queued"] == 0 and t["active"] == 0:
+                t = self.tick()  # completes the last retirements
+                if on_done is not None:
+                    for rid in self.completed():
+                        on_done(rid)
+                return t
+        raise PanoError(-1, "server did not drain")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().pano_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -74,6 +74,21 @@ class BatchParams(ctypes.Structure):
                 ("use_graph", c_i32), ("autotune", c_i32), ("tile_nat", c_i32 * 2), ("tile_rb", c_i32 * 2)]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("hbm_budget", c_i64), ("max_active", c_i32), ("mg_levels", c_i32), ("mg_nu", c_i32), ("tol", c_f64),
+                ("max_iterations", c_i32), ("iters_per_tick", c_i32), ("tile_nat", c_i32 * 2), ("tile_rb", c_i32 * 2)]
+
+
+class ServerRequest(ctypes.Structure):
+    _fields_ = [("grid", BatchGrid), ("faces", c_void_p), ("seed", ctypes.c_uint64), ("n_modes", c_i32),
+                ("k_max", c_f64), ("amplitude", c_f64)]
+
+
+class ServerQuery(ctypes.Structure):
+    _fields_ = [("state", c_i32), ("iterations", c_i32), ("converged", c_i32), ("r_linf", c_f64), ("b_linf", c_f64),
+                ("t_submit", c_f64), ("t_admit", c_f64), ("t_done", c_f64), ("cells", c_i64), ("bytes", c_i64)]
+
+
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", c_f64), ("pcg", PcgParams), ("buoyancy", c_f64), ("src_center", c_f64 * 3),
                 ("src_radius", c_f64)]
@@ -119,6 +134,14 @@ _SIGS = {
     "pano_batch_get_state": ([c_void_p, c_i32, c_void_p, c_void_p], c_i32),
     "pano_batch_step": ([c_void_p, _P(c_i32)], c_i32),
     "pano_batch_tiles": ([c_void_p, _P(c_i32), _P(c_i32), _P(c_f64)], c_i32),
+    "pano_server_create": ([c_void_p, _P(ServerParams), _P(c_void_p)], c_i32),
+    "pano_server_destroy": ([c_void_p], c_i32),
+    "pano_server_submit": ([c_void_p, _P(ServerRequest), _P(c_i64)], c_i32),
+    "pano_server_tick": ([c_void_p, _P(c_i32), _P(c_i32), _P(c_i64), _P(c_i64), _P(c_f64)], c_i32),
+    "pano_server_completed": ([c_void_p, _P(c_i64), c_i32, _P(c_i32)], c_i32),
+    "pano_server_status": ([c_void_p, c_i64, _P(ServerQuery)], c_i32),
+    "pano_server_fetch": ([c_void_p, c_i64, c_void_p, c_void_p], c_i32),
+    "pano_server_release": ([c_void_p, c_i64], c_i32),
     "pano_gen_bandlimited_faces": ([_P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_bandlimited_faces_device": ([c_void_p, _P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p],
                                           c_i32),

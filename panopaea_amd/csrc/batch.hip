@@ -383,7 +383,7 @@ pano_status hgrid_check(const Dims& D, int levels) {
     return PANO_OK;
 }
 
-pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base) {
+pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base, bool proj) {
     const Dims& D = g.D;
     g.cells = (int64_t)D.d * D.h * D.w;
     g.faces = face_layout(D).total;
@@ -393,8 +393,13 @@ pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base) {
         off = align256(off + bytes);
         return o;
     };
-    for (int i = 0; i < 2; ++i) g.o_rho[i] = take(g.cells * 4);
-    for (int i = 0; i < 2; ++i) g.o_u[i] = take(g.faces * 4);
+    if (proj) {
+        g.o_rho[0] = g.o_rho[1] = -1;
+        g.o_u[0] = g.o_u[1] = take(g.faces * 4);
+    } else {
+        for (int i = 0; i < 2; ++i) g.o_rho[i] = take(g.cells * 4);
+        for (int i = 0; i < 2; ++i) g.o_u[i] = take(g.faces * 4);
+    }
     g.o_x = take(g.cells * 4);
     g.o_r = take(g.cells * 4);
     g.o_z = take(g.cells * 4);
@@ -446,11 +451,10 @@ BatchCore::~BatchCore() {
     auto rel = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    rel(nat_.d), rel(fx_.d), rel(fy_.d), rel(fz_.d), rel(flat_.d);
-    for (int p = 0; p < 2; ++p) rel(div_[p].d), rel(prx_[p].d), rel(pry_[p].d), rel(prz_[p].d);
-    for (int l = 0; l < 16; ++l) rel(rb_[l].d), rel(co_[l].d), rel(prl_[l].d);
-    rel(plane_.d), rel(zcol_.d), rel(rows2_.d), rel(cols2_.d);
-    rel(states), rel(cfgs), rel(part0_), rel(part1_), rel(part2_);
+    rel(tab_dev_);
+    if (tab_host_) (void)hipHostFree(tab_host_);
+    if (owns_slots) rel(states), rel(cfgs);
+    rel(part0_), rel(part1_), rel(part2_);
 }
 
 pano_status BatchCore::ensure_slots(int n) {
@@ -461,38 +465,42 @@ pano_status BatchCore::ensure_slots(int n) {
     return PANO_OK;
 }
 
-// Entries then their first blocks, in one allocation and one copy. The
-// staging vector stays alive until the next upload of the table, after the
-// stream has been synchronised.
+// Tables are packed into the pinned staging buffer (place) and copied to the
+// device buffer in one transfer at the end of build. The staging buffer is
+// rewritten only by the next build, which callers issue after the stream
+// has drained past the previous copy (a tick starts with a synchronisation;
+// a static batch builds once).
 template <typename TableT>
-static pano_status upload_table(Ctx* c, TableT& t, int nbt) {
+static size_t table_bytes(const TableT& t) {
+    using Entry = typename std::remove_reference<decltype(t.h[0])>::type;
+    return (t.h.size() * (sizeof(Entry) + sizeof(int)) + 255) & ~size_t(255);
+}
+
+template <typename TableT>
+static pano_status place_table(TableT& t, char* host, char* dev, size_t& used, int nbt) {
     using Entry = typename std::remove_reference<decltype(t.h[0])>::type;
     if (t.h.size() > (size_t)kMaxEntries)
         return fail(PANO_ERR_INVALID, "batched launch with %zu grids (at most %d)", t.h.size(), kMaxEntries);
     t.nbt = nbt;
     size_t n = t.h.size();
-    size_t bytes = std::max<size_t>(1, n * (sizeof(Entry) + sizeof(int)));
-    PANO_CHECK(grow(c, (void**)&t.d, &t.cap, bytes));
-    if (n == 0) return PANO_OK;
-    PANO_HIP(hipStreamSynchronize(c->stream));
-    t.stage.resize(n * (sizeof(Entry) + sizeof(int)));
-    memcpy(t.stage.data(), t.h.data(), n * sizeof(Entry));
-    int* st = reinterpret_cast<int*>(t.stage.data() + n * sizeof(Entry));
+    t.d = reinterpret_cast<const Entry*>(dev + used);
+    memcpy(host + used, t.h.data(), n * sizeof(Entry));
+    int* st = reinterpret_cast<int*>(host + used + n * sizeof(Entry));
     for (size_t i = 0; i < n; ++i) st[i] = t.h[i].start;
-    PANO_HIP(hipMemcpyAsync(t.d, t.stage.data(), t.stage.size(), hipMemcpyHostToDevice, c->stream));
+    used += table_bytes(t);
     return PANO_OK;
 }
 
-pano_status BatchCore::upload(Table& t) {
+pano_status BatchCore::place(Table& t) {
     int nbt = 0;
     for (auto& e : t.h) nbt = std::max(nbt, e.start + e.G.nb);
-    return upload_table(c, t, nbt);
+    return place_table(t, tab_host_, tab_dev_, tab_used_, nbt);
 }
 
-pano_status BatchCore::upload(FdmTable& t) {
+pano_status BatchCore::place(FdmTable& t) {
     int nbt = 0;
     for (auto& e : t.h) nbt = std::max(nbt, e.start + e.nblk);
-    return upload_table(c, t, nbt);
+    return place_table(t, tab_host_, tab_dev_, tab_used_, nbt);
 }
 
 pano_status BatchCore::ensure_partials(size_t n) {
@@ -673,26 +681,48 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         maxp = std::max(maxp, (size_t)std::max({snat, sflat, srb[0]}));
     }
     fdm_lds_ = 2 * sizeof(double) * (size_t)kFdmPlane;
-    PANO_CHECK(upload(nat_));
-    PANO_CHECK(upload(fx_));
-    PANO_CHECK(upload(fy_));
-    PANO_CHECK(upload(fz_));
-    PANO_CHECK(upload(flat_));
+    // pack every table into the staging buffer, then one copy
+    size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
+                   table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
+    for (int p = 0; p < 2; ++p)
+        total += table_bytes(div_[p]) + table_bytes(prx_[p]) + table_bytes(pry_[p]) + table_bytes(prz_[p]);
+    for (int l = 0; l + 1 < levels_; ++l) total += table_bytes(rb_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]);
+    total = std::max<size_t>(total, 256);
+    PANO_CHECK(grow(c, (void**)&tab_dev_, &tab_cap_, total));
+    if (tab_host_cap_ < total) {
+        PANO_HIP(hipStreamSynchronize(c->stream));
+        if (tab_host_) PANO_HIP(hipHostFree(tab_host_));
+        tab_host_ = nullptr;
+        tab_host_cap_ = 0;
+        if (hipHostMalloc((void**)&tab_host_, total) != hipSuccess) {
+            (void)hipGetLastError();
+            tab_host_ = nullptr;
+            return fail(PANO_ERR_OOM, "pinned launch tables (%zu bytes)", total);
+        }
+        tab_host_cap_ = total;
+    }
+    tab_used_ = 0;
+    PANO_CHECK(place(nat_));
+    PANO_CHECK(place(fx_));
+    PANO_CHECK(place(fy_));
+    PANO_CHECK(place(fz_));
+    PANO_CHECK(place(flat_));
     for (int p = 0; p < 2; ++p) {
-        PANO_CHECK(upload(div_[p]));
-        PANO_CHECK(upload(prx_[p]));
-        PANO_CHECK(upload(pry_[p]));
-        PANO_CHECK(upload(prz_[p]));
+        PANO_CHECK(place(div_[p]));
+        PANO_CHECK(place(prx_[p]));
+        PANO_CHECK(place(pry_[p]));
+        PANO_CHECK(place(prz_[p]));
     }
     for (int l = 0; l + 1 < levels_; ++l) {
-        PANO_CHECK(upload(rb_[l]));
-        PANO_CHECK(upload(co_[l]));
-        PANO_CHECK(upload(prl_[l]));
+        PANO_CHECK(place(rb_[l]));
+        PANO_CHECK(place(co_[l]));
+        PANO_CHECK(place(prl_[l]));
     }
-    PANO_CHECK(upload(plane_));
-    PANO_CHECK(upload(zcol_));
-    PANO_CHECK(upload(rows2_));
-    PANO_CHECK(upload(cols2_));
+    PANO_CHECK(place(plane_));
+    PANO_CHECK(place(zcol_));
+    PANO_CHECK(place(rows2_));
+    PANO_CHECK(place(cols2_));
+    PANO_HIP(hipMemcpyAsync(tab_dev_, tab_host_, tab_used_, hipMemcpyHostToDevice, c->stream));
     PANO_CHECK(ensure_partials(std::max<size_t>(maxp, 1)));
     return PANO_OK;
 }

@@ -82,8 +82,9 @@ struct TileShapes {
     int rb_x = 64, rb_y = 4;    // red-black x-pair tiles
 };
 
-// Byte size of a grid's fields and their offsets from `base`.
-pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base);
+// Byte size of a grid's fields and their offsets from `base`. A projection
+// grid (proj) has one velocity buffer and no density.
+pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base, bool proj = false);
 
 // Checks a grid for the batched solver (f32, multigrid with `levels`).
 pano_status hgrid_check(const Dims& D, int levels);
@@ -99,6 +100,7 @@ public:
     SolverState* states = nullptr;  // device, one per slot
     SolveCfg* cfgs = nullptr;       // device, one per slot
     int nslots = 0;
+    bool owns_slots = true;         // false: states/cfgs belong to another core
 
     BatchCore() = default;
     BatchCore(const BatchCore&) = delete;
@@ -118,22 +120,20 @@ public:
     pano_status tune_smooth();
 
 private:
+    // A launch table lives in the core's one device table buffer: its
+    // entries, then their first blocks (int).
     struct Table {
         std::vector<BEntry> h;
-        BEntry* d = nullptr;   // entries, then their first blocks (int)
-        int nbt = 0;           // total blocks
-        size_t cap = 0;
-        std::vector<char> stage;
+        const BEntry* d = nullptr;
+        int nbt = 0;  // total blocks
         BTab view() const {
             return BTab{d, reinterpret_cast<const int*>(d + h.size()), (int)h.size(), nbt};
         }
     };
     struct FdmTable {
         std::vector<BFdm> h;
-        BFdm* d = nullptr;
+        const BFdm* d = nullptr;
         int nbt = 0;
-        size_t cap = 0;
-        std::vector<char> stage;
         FTab view() const {
             return FTab{d, reinterpret_cast<const int*>(d + h.size()), (int)h.size(), nbt};
         }
@@ -156,10 +156,13 @@ private:
     double* part2_ = nullptr;
     size_t part_cap_ = 0;
     size_t slot_cap_[2] = {0, 0};
+    char* tab_dev_ = nullptr;      // all launch tables (device)
+    char* tab_host_ = nullptr;     // their pinned staging copy
+    size_t tab_cap_ = 0, tab_host_cap_ = 0, tab_used_ = 0;
     size_t part_bytes_[3] = {0, 0, 0};
 
-    pano_status upload(Table& t);
-    pano_status upload(FdmTable& t);
+    pano_status place(Table& t);
+    pano_status place(FdmTable& t);
     pano_status ensure_partials(size_t n);
     void fin(int op, const double* p0, const double* p1, const Table& t, int use_max);
     pano_status vcycle(bool with_dot);

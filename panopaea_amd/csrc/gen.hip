@@ -234,6 +234,38 @@ static pano_status gen_args(const pano_grid* g, int32_t n_modes, double k_max) {
     return PANO_OK;
 }
 
+pano_status gen_tables(const pano_grid* g, uint64_t seed, int n_modes, double k_max, std::vector<double>& tab) {
+    PANO_CHECK(gen_args(g, n_modes, k_max));
+    Dims D = dims_of(g);
+    int ncomp = g->ndim == 3 ? 3 : 2;
+    CompGeom G[3];
+    comp_geoms(D, ncomp, n_modes, G);
+    std::vector<Mode> modes = draw_modes(g->ndim, seed, n_modes, k_max);
+    tab = build_tables(D, g->ndim, modes, n_modes, G, ncomp);
+    return PANO_OK;
+}
+
+pano_status gen_faces_enqueue(Ctx* c, const pano_grid* g, int n_modes, double amplitude, const double* dtab,
+                              void* faces) {
+    Dims D = dims_of(g);
+    int ncomp = g->ndim == 3 ? 3 : 2;
+    CompGeom G[3];
+    comp_geoms(D, ncomp, n_modes, G);
+    for (int k = 0; k < ncomp; ++k) {
+        int axis = ncomp == 3 ? (k == 0 ? 2 : (k == 1 ? 1 : 0)) : (k == 0 ? 1 : 0);
+        int64_t n = (int64_t)G[k].nxc * G[k].nyc * G[k].nzc;
+        if (g->dtype == PANO_F64)
+            hipLaunchKernelGGL(k_gen_faces<double>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, D,
+                               axis, G[k], dtab, n_modes, amplitude, (double*)faces);
+        else
+            hipLaunchKernelGGL(k_gen_faces<float>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, D,
+                               axis, G[k], dtab, n_modes, amplitude, (float*)faces);
+    }
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+
 }  // namespace pano
 
 using namespace pano;
@@ -262,26 +294,12 @@ pano_status pano_gen_bandlimited_faces_device(pano_ctx ctx, const pano_grid* g, 
     PANO_CHECK(gen_args(g, n_modes, k_max));
     if (!faces) return fail(PANO_ERR_INVALID, "faces is null");
     PANO_HIP(hipSetDevice(c->device));
-    Dims D = dims_of(g);
-    int ncomp = g->ndim == 3 ? 3 : 2;
-    CompGeom G[3];
-    comp_geoms(D, ncomp, n_modes, G);
-    std::vector<Mode> modes = draw_modes(g->ndim, seed, n_modes, k_max);
-    std::vector<double> tab = build_tables(D, g->ndim, modes, n_modes, G, ncomp);
+    std::vector<double> tab;
+    PANO_CHECK(gen_tables(g, seed, n_modes, k_max, tab));
     void* dtab;
     PANO_CHECK(c->workspace("gen_tables", tab.size() * sizeof(double), &dtab));
     PANO_HIP(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    for (int k = 0; k < ncomp; ++k) {
-        int axis = ncomp == 3 ? (k == 0 ? 2 : (k == 1 ? 1 : 0)) : (k == 0 ? 1 : 0);
-        int64_t n = (int64_t)G[k].nxc * G[k].nyc * G[k].nzc;
-        if (g->dtype == PANO_F64)
-            hipLaunchKernelGGL(k_gen_faces<double>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, D,
-                               axis, G[k], (const double*)dtab, n_modes, amplitude, (double*)faces);
-        else
-            hipLaunchKernelGGL(k_gen_faces<float>, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, c->stream, D,
-                               axis, G[k], (const double*)dtab, n_modes, amplitude, (float*)faces);
-    }
-    PANO_HIP(hipGetLastError());
+    PANO_CHECK(gen_faces_enqueue(c, g, n_modes, amplitude, (const double*)dtab, faces));
     PANO_HIP(hipStreamSynchronize(c->stream));
     return PANO_OK;
 }

@@ -192,6 +192,13 @@ inline bool closed_box(const Dims& D) { return D.bc == 0; }
 // ---- kernels launched from several translation units ---------------------------
 pano_status fill_zero(Ctx* c, void* dst, size_t bytes);
 
+// Seeded band-limited velocity in two parts: the host tables (mode draws
+// and separable per-axis factors) and the asynchronous device evaluation
+// from tables already on the device.
+pano_status gen_tables(const pano_grid* g, uint64_t seed, int n_modes, double k_max, std::vector<double>& tab);
+pano_status gen_faces_enqueue(Ctx* c, const pano_grid* g, int n_modes, double amplitude, const double* dtab,
+                              void* faces);
+
 // Fluid-step operator launches (fluid.hip), templated on T in {float,double}.
 template <typename T>
 pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces);
