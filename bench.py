@@ -39,7 +39,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--dims", default=None)
     ap.add_argument("--workload", default="smoke3d_256",
-                    choices=["smoke3d_256", "poisson512_mixed", "batch64_2d", "stream_copy"])
+                    choices=["smoke3d_256", "poisson512_mixed", "batch64_2d", "serve", "stream_copy"])
+    ap.add_argument("--duration", type=float, default=None, help="serve: seconds of arrivals (default 60)")
     ap.add_argument("--cpu-check", action="store_true",
                     help="poisson512_mixed: also solve in pure FP64 with the CPU oracle (minutes at 512^3)")
     args = ap.parse_args()
@@ -57,6 +58,8 @@ def main():
         return bench_batch(args)
     if args.workload == "stream_copy":
         return bench_stream_copy(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import panopaea_amd as pa
     from panopaea_amd import _lib
@@ -117,7 +120,8 @@ def main():
         avg_s = tot_ms.value / launches.value / 1e3
         achieved = bpl.value / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(args.kernel, dims),
                 "launches": launches.value, "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": bpl.value}
 
@@ -332,6 +336,77 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """Open-loop serving: projection requests arrive by a seeded Poisson
+    process; the loop submits each at its arrival time, ticks the server,
+    and on completion records latency (completion - arrival) and releases
+    the request's memory. Velocity fields are generated on the device at
+    admission from each request's seed."""
+    import panopaea_amd as pa
+    from panopaea_amd import workloads as W
+    wl = W.SERVE
+    reqs = W.serve_requests(wl, args.duration)
+    S = pa.Server(hbm_budget=64 << 30, max_active=1024, mg_levels=wl["mg_levels"], mg_nu=wl["mg_nu"], tol=wl["tol"],
+                  iters_per_tick=2)
+    # warm-up: one request of each extreme size, not measured
+    for g in (dict(dims=(1024, 1024), dt=0.5 / 1024), dict(dims=(256, 256, 256), dt=0.5 / 256, bc=(0, 0, 0, 0, 0, 1))):
+        S.submit(g, seed=12345)
+    S.run_until_idle(lambda rid: S.release(rid))
+    print(f"bench: serve {len(reqs)} requests over {reqs[-1][0]:.1f}s", file=sys.stderr, flush=True)
+    t0 = S.tick()["now"]
+    arrival, lat, kinds = {}, [], []
+    cells_done, i, n = 0, 0, len(reqs)
+    ticks, active_sum, peak_bytes, peak_active, last_print = 0, 0, 0, 0, 0.0
+    while len(lat) < n:
+        t = S.tick()
+        ticks += 1
+        now = t["now"] - t0
+        active_sum += t["active"]
+        peak_bytes = max(peak_bytes, t["bytes_used"])
+        peak_active = max(peak_active, t["active"])
+        for rid in S.completed():
+            st = S.status(rid)
+            ta, nd = arrival.pop(rid)
+            lat.append(st["t_done"] - t0 - ta)
+            kinds.append(nd)
+            cells_done += st["cells"]
+            if not st["converged"]:
+                print(f"bench: request {rid} did not converge", file=sys.stderr, flush=True)
+            S.release(rid)
+        while i < n and reqs[i][0] <= now:
+            ta, g, seed, nm, km = reqs[i]
+            rid = S.submit(g, seed=seed, n_modes=nm, k_max=km)
+            arrival[rid] = (ta, len(g["dims"]))
+            i += 1
+        if now - last_print >= 10.0:
+            last_print = now
+            print(f"bench: t={now:.0f}s submitted {i} done {len(lat)} active {t['active']} queued {t['queued']} "
+                  f"bytes {t['bytes_used'] / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+        if t["active"] == 0 and t["queued"] == 0 and i < n and reqs[i][0] > now:
+            time.sleep(min(reqs[i][0] - now, 0.01))
+    elapsed = S.tick()["now"] - t0
+    S.close()
+    lat_ms = np.array(lat) * 1e3
+    kinds = np.array(kinds)
+    total_cells = sum(int(np.prod(g["dims"])) for _, g, *_ in reqs)
+    out = {
+        "metric": "served projection solves: p50/p99 latency (ms) and cells/s, Poisson arrivals 200/s, mixed 2D/3D",
+        "value": cells_done / elapsed, "unit": "cells/s", "n_gpus": 1, "steps": n, "warmup": 2,
+        "ms_per_step": float(np.mean(lat_ms)), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "p50_ms_2d": float(np.percentile(lat_ms[kinds == 2], 50)), "p99_ms_2d": float(np.percentile(lat_ms[kinds == 2], 99)),
+        "p50_ms_3d": float(np.percentile(lat_ms[kinds == 3], 50)), "p99_ms_3d": float(np.percentile(lat_ms[kinds == 3], 99)),
+        "dtype": "f32", "data": "synthetic (seeded arrivals, sizes and band-limited velocity)",
+        "config": {"workload": wl["name"], "requests": n, "arrival_rate": wl["rate"],
+                   "duration_s": reqs[-1][0], "offered_cells_per_s": total_cells / reqs[-1][0],
+                   "elapsed_s": elapsed, "ticks": ticks, "mean_active": active_sum / max(ticks, 1),
+                   "peak_active": peak_active, "peak_hbm_gib": peak_bytes / 2**30, "hbm_budget_gib": 64,
+                   "precond": "multigrid 4 levels, 2+2 red-black GS", "tol": "1e-6 rel L-inf"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_stream_copy(args):
     """Device copy of a 1 GiB f32 buffer (torch's elementwise copy kernel):
     the measured copy bandwidth, and the known-size kernel that calibrates
@@ -354,6 +429,21 @@ def bench_stream_copy(args):
     gbs = 2.0 * 4 * n / (ms * 1e-3) / 1e9
     print(json.dumps({"metric": "device copy GB/s (read + write)", "value": gbs, "unit": "GB/s", "bytes_read": 4 * n,
                       "bytes_written": 4 * n, "ms_per_copy": ms, "steps": args.steps}), flush=True)
+
+
+def profiled_traffic(kernel, dims):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary of the same workload (None if there is none)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_smoke3d_{'x'.join(map(str, dims))}_pmc.md")
+    if tuple(dims) == (256, 256, 256):
+        path = os.path.join(ROOT, "profiles", f"{kernel}_smoke3d_256_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(wl, dims, cfg, init_vel):

@@ -90,6 +90,41 @@ def batch_initial(wl, g):
     return rho, faces
 
 
+# Served projection requests: Poisson arrivals at 200/s for 60 s (seed 11);
+# 70% 2D squares N in [128, 1024], 30% 3D cubes N in [64, 256], N a
+# multiple of 8 drawn uniformly; 2D closed walls with 16 modes k <= 8, 3D
+# open top with 32 modes k <= 16 (the 128^2 and 256^3 workloads' fields);
+# dt = 0.5/N; MG-PCG (4 levels, 2+2) to 1e-6 relative L-inf.
+SERVE = dict(
+    name="serve_projection_mix", rate=200.0, duration=60.0, seed=11, p2d=0.7, n2d=(128, 1024), n3d=(64, 256),
+    step=8, mg_levels=4, mg_nu=2, tol=1e-6,
+)
+
+
+def serve_requests(wl=SERVE, duration=None):
+    """[(t_arrival, grid dict, seed, n_modes, k_max)] in arrival order. Each
+    request takes three draws of the stream: inter-arrival time
+    -ln(1 - u)/rate, kind (2D when u < p2d), size index."""
+    u = uniform01(wl["seed"])
+    T = wl["duration"] if duration is None else duration
+    t, out, k = 0.0, [], 0
+    while True:
+        t += -np.log1p(-next(u)) / wl["rate"]
+        is2d = next(u) < wl["p2d"]
+        lo, hi = wl["n2d"] if is2d else wl["n3d"]
+        count = (hi - lo) // wl["step"] + 1
+        n = lo + wl["step"] * min(int(next(u) * count), count - 1)
+        if t >= T:
+            return out
+        if is2d:
+            g = dict(dims=(n, n), dt=0.5 / n, bc=(0, 0, 0, 0, 0, 0))
+            out.append((t, g, k, 16, 8.0))
+        else:
+            g = dict(dims=(n, n, n), dt=0.5 / n, bc=(0, 0, 0, 0, 0, 1))
+            out.append((t, g, k, 32, 16.0))
+        k += 1
+
+
 def make_grid(wl, dims=None, dtype=np.float32, ctx=None):
     dims = tuple(dims or wl["dims"])
     if len(dims) == 2:
