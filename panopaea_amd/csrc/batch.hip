@@ -203,37 +203,55 @@ __global__ __launch_bounds__(256) void kb_search(BTab T, char* A, const SolverSt
 }
 
 // red-black pair kernels over rb_[l]: o[0] = z_l, o[1] = r_l, o[2] = res, o[3] = p (level 0)
-__global__ __launch_bounds__(256) void kb_apply(BTab T, char* A, const SolverState* st,
-                                                double* part) {
+__global__ __launch_bounds__(256) void kb_apply(BTab T, char* A, const SolverState* st, double* part) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
-    int m, y, z;
-    local_xyz(E.G, lb, m, y, z);
+    int m, y, zb;
+    local_xyz(E.G, lb, m, y, zb);
     const Dims& D = E.D;
     double s = 0.0;
     if (m < (D.w >> 1) && y < D.h) {
         float* q = fld<float>(A, E.o[0]);
         const float* p = fld<float>(A, E.o[3]);
         float dt = (float)E.s;
-        s = D.d > 1 ? apply_pair<float, 3>(D, q, p, dt, m, y, z) : apply_pair<float, 2>(D, q, p, dt, m, y, z);
+        s = with_run(D, [&](auto dc, auto rc) {
+            return apply_run<float, decltype(dc)::value, decltype(rc)::value>(D, q, p, dt, m, y, zb * rc);
+        });
     }
     double t = block_sum(s);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[b_] = t;
 }
 
-__global__ __launch_bounds__(256) void kb_smooth_first(BTab T, char* A,
-                                                       const SolverState* st) {
+// red cells of the first sweep from z = 0 (the black half-sweep follows)
+__global__ __launch_bounds__(256) void kb_red_init(BTab T, char* A, const SolverState* st) {
     BATCH_ENTRY(T);
     if (st && st[E.sidx].done) return;
-    int m, y, z;
-    local_xyz(E.G, lb, m, y, z);
+    int m, y, zb;
+    local_xyz(E.G, lb, m, y, zb);
     const Dims& D = E.D;
     if (m >= (D.w >> 1) || y >= D.h) return;
     const RInv<float>& ri = *reinterpret_cast<const RInv<float>*>(E.rinv);
     float* zv = fld<float>(A, E.o[0]);
     const float* r = fld<float>(A, E.o[1]);
-    if (D.d > 1) smooth_first_pair<float, 3>(D, zv, r, (float)E.s, ri, m, y, z);
-    else smooth_first_pair<float, 2>(D, zv, r, (float)E.s, ri, m, y, z);
+    with_run(D, [&](auto dc, auto rc) {
+        red_init_run<float, decltype(dc)::value, decltype(rc)::value>(D, zv, r, (float)E.s, ri, m, y, zb * rc);
+    });
+}
+
+// the whole first sweep from z = 0 (both colours, one launch)
+__global__ __launch_bounds__(256) void kb_smooth_first(BTab T, char* A, const SolverState* st) {
+    BATCH_ENTRY(T);
+    if (st && st[E.sidx].done) return;
+    int m, y, zb;
+    local_xyz(E.G, lb, m, y, zb);
+    const Dims& D = E.D;
+    if (m >= (D.w >> 1) || y >= D.h) return;
+    const RInv<float>& ri = *reinterpret_cast<const RInv<float>*>(E.rinv);
+    float* zv = fld<float>(A, E.o[0]);
+    const float* r = fld<float>(A, E.o[1]);
+    with_run(D, [&](auto dc, auto rc) {
+        smooth_first_run<float, decltype(dc)::value, decltype(rc)::value>(D, zv, r, (float)E.s, ri, m, y, zb * rc);
+    });
 }
 
 __global__ __launch_bounds__(256) void kb_smooth_half(BTab T, char* A, int color,
@@ -248,8 +266,10 @@ __global__ __launch_bounds__(256) void kb_smooth_half(BTab T, char* A, int color
         const RInv<float>& ri = *reinterpret_cast<const RInv<float>*>(E.rinv);
         float* zv = fld<float>(A, E.o[0]);
         const float* r = fld<float>(A, E.o[1]);
-        acc = D.d > 1 ? smooth_half_cell<float, 3>(D, zv, r, (float)E.s, ri, color, m, y, z)
-                      : smooth_half_cell<float, 2>(D, zv, r, (float)E.s, ri, color, m, y, z);
+        acc = with_run(D, [&](auto dc, auto rc) {
+            return smooth_half_run<float, decltype(dc)::value, decltype(rc)::value>(D, zv, r, (float)E.s, ri, color,
+                                                                                   m, y, z * rc);
+        });
     }
     if (part) {
         double t = block_sum(acc);
@@ -268,8 +288,9 @@ __global__ __launch_bounds__(256) void kb_residual(BTab T, char* A,
     float* res = fld<float>(A, E.o[2]);
     const float* zv = fld<float>(A, E.o[0]);
     const float* r = fld<float>(A, E.o[1]);
-    if (D.d > 1) residual_pair<float, 3>(D, res, zv, r, (float)E.s, m, y, z);
-    else residual_pair<float, 2>(D, res, zv, r, (float)E.s, m, y, z);
+    with_run(D, [&](auto dc, auto rc) {
+        residual_run<float, decltype(dc)::value, decltype(rc)::value>(D, res, zv, r, (float)E.s, m, y, z * rc);
+    });
 }
 
 // restriction into level l+1 over co_[l]: o[0] = r_{l+1}, o[1] = res (level l); LC = coarse layout
@@ -298,8 +319,9 @@ __global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
     if (m >= (E.D.w >> 1) || y >= E.D.h) return;
     float* zf = fld<float>(A, E.o[0]);
     const float* zc = fld<float>(A, E.o[1]);
-    if (E.D.d > 1) prolong_pair<float, 3, LC>(E.D, E.Dc, zf, zc, m, y, z);
-    else prolong_pair<float, 2, LC>(E.D, E.Dc, zf, zc, m, y, z);
+    with_run(E.D, [&](auto dc, auto rc) {
+        prolong_run<float, decltype(dc)::value, LC, decltype(rc)::value>(E.D, E.Dc, zf, zc, m, y, z * rc);
+    });
 }
 
 #define FDM_ENTRY(T) const BFdm& F = T.e[find_entry(T.start, T.n, blockIdx.x)];
@@ -536,6 +558,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     cols2_.h.clear();
     levels_ = ids.empty() ? 0 : grids[ids[0]].levels;
     any3d_ = false;
+    for (int l = 0; l < 16; ++l) split_first_[l] = false;
     int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
     int srow = 0, scol = 0;
     size_t maxp = 0;
@@ -617,7 +640,9 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 r.s = (double)(float)s;
                 RInv<float> ri = make_rinv<float>((float)s);
                 for (int k = 0; k < 8; ++k) r.rinv[k] = ri.v[k];
-                r.G = grid3t(L.w >> 1, L.h, L.d, shapes.rb_x, shapes.rb_y);
+                int run = run_len(L);  // planes per thread
+                split_first_[l] = split_first_[l] || run == 4;
+                r.G = grid3t(L.w >> 1, L.h, (L.d + run - 1) / run, shapes.rb_x, shapes.rb_y);
                 r.start = srb[l];
                 srb[l] += r.G.nb;
                 r.o[0] = g.o_zl[l];
@@ -680,7 +705,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         }
         maxp = std::max(maxp, (size_t)std::max({snat, sflat, srb[0]}));
     }
-    fdm_lds_ = 2 * sizeof(double) * (size_t)kFdmPlane;
+    fdm_lds_ = kFdmLdsBytes;
     // pack every table into the staging buffer, then one copy
     size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
                    table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
@@ -771,7 +796,12 @@ pano_status BatchCore::vcycle(bool with_dot) {
     const int Lc = levels_ - 1;
     dim3 blk(shapes.rb_x, shapes.rb_y);
     for (int l = 0; l < Lc; ++l) {
-        BL(kb_smooth_first, rb_[l], arena, (const SolverState*)states);
+        if (split_first_[l]) {  // bandwidth-bound levels: red cells, then the black half-sweep
+            BL(kb_red_init, rb_[l], arena, (const SolverState*)states);
+            BL(kb_smooth_half, rb_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
+        } else {  // latency-bound levels: one fused pass (same values)
+            BL(kb_smooth_first, rb_[l], arena, (const SolverState*)states);
+        }
         for (int k = 1; k < nu; ++k) {
             BL(kb_smooth_half, rb_[l], arena, 0, (double*)nullptr, (const SolverState*)states);
             BL(kb_smooth_half, rb_[l], arena, 1, (double*)nullptr, (const SolverState*)states);

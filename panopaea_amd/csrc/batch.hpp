@@ -150,6 +150,7 @@ private:
     FdmTable rows2_, cols2_;  // 2D coarsest solve
     int levels_ = 0;
     bool any3d_ = false;
+    bool split_first_[16] = {};  // first sweep as red cells + black half-sweep at this level
     size_t fdm_lds_ = 0;
     double* part0_ = nullptr;
     double* part1_ = nullptr;

@@ -10,49 +10,83 @@
 
 namespace pano {
 
+// A thread computes the values of its run of planes z0 + k (k < NZ, plane
+// index clamped to the last of P planes for the loads), then stores those in range.
+#define PLANE_RUN_BEGIN(NZ)          \
+    int x, y, zb, L;                 \
+    tile_xyz(G, x, y, zb, L);        \
+    const int z0 = zb * (NZ);
+#define PLANE_Z(k, P) imin(z0 + (k), (P) - 1)
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_cell(Dims D, Grid3 G, T* dst, const T* q, T dt,
                                                      Faces<const T> v, Sphere S) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    dst[cell_index(D, z, y, x)] = advect_cell_value<T, DIM>(D, q, S, dt, v, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) out[k] = advect_cell_value<T, DIM>(D, q, S, dt, v, PLANE_Z(k, D.d), y, x);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) dst[cell_index(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_fx(Dims D, Grid3 G, T* dst, const T* qx, T dt,
                                                    Faces<const T> v) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x > D.w || y >= D.h) return;
-    dst[fx_idx(D, z, y, x)] = advect_face_x<T, DIM>(D, qx, dt, v, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) out[k] = advect_face_x<T, DIM>(D, qx, dt, v, PLANE_Z(k, D.d), y, x);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) dst[fx_idx(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_fy(Dims D, Grid3 G, T* dst, const T* qy, T dt,
                                                    Faces<const T> v) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y > D.h) return;
-    dst[fy_idx(D, z, y, x)] = advect_face_y<T, DIM>(D, qy, dt, v, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) out[k] = advect_face_y<T, DIM>(D, qy, dt, v, PLANE_Z(k, D.d), y, x);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) dst[fy_idx(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_advect_fz(Dims D, Grid3 G, T* dst, const T* qz, T dt,
                                                    Faces<const T> v, T buoy_c, const T* rho,
                                                    Sphere S) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<3>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    dst[fz_idx(D, z, y, x)] = advect_face_z<T>(D, qz, dt, v, z, y, x, buoy_c, rho, S);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        out[k] = advect_face_z<T>(D, qz, dt, v, PLANE_Z(k, D.d + 1), y, x, buoy_c, rho, S);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k <= D.d) dst[fz_idx(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_neg_div(Dims D, Grid3 G, T* b, Faces<const T> v) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    b[cell_index(D, z, y, x)] = -div_value<T, DIM>(D, v, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, PLANE_Z(k, D.d), y, x);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) b[cell_index(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T, int DIM>
@@ -89,27 +123,48 @@ __global__ __launch_bounds__(256) void k_poisson_masked2(Dims D, Grid3 G, T* q, 
 
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x > D.w || y >= D.h) return;
-    int64_t i = fx_idx(D, z, y, x);
-    ux[i] = project_x<T, L_>(D, ux[i], p, dt, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = PLANE_Z(k, D.d);
+        out[k] = project_x<T, L_>(D, ux[fx_idx(D, z, y, x)], p, dt, z, y, x);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) ux[fx_idx(D, z0 + k, y, x)] = out[k];
 }
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const T* p, T dt) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y > D.h) return;
-    int64_t i = fy_idx(D, z, y, x);
-    uy[i] = project_y<T, L_>(D, uy[i], p, dt, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = PLANE_Z(k, D.d);
+        out[k] = project_y<T, L_>(D, uy[fy_idx(D, z, y, x)], p, dt, z, y, x);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) uy[fy_idx(D, z0 + k, y, x)] = out[k];
 }
 template <typename T, int L_>
 __global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const T* p, T dt) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<3>;
+    PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    int64_t i = fz_idx(D, z, y, x);
-    uz[i] = project_z<T, L_>(D, uz[i], p, dt, z, y, x);
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = PLANE_Z(k, D.d + 1);
+        out[k] = project_z<T, L_>(D, uz[fz_idx(D, z, y, x)], p, dt, z, y, x);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k <= D.d) uz[fz_idx(D, z0 + k, y, x)] = out[k];
 }
 
 static Sphere make_sphere(const double* s) {
@@ -140,7 +195,8 @@ template <typename T>
 pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
                           const double* src_sphere) {
     Faces<const T> v = split_faces(D, vel);
-    Grid3 G = grid3(D.w, D.h, D.d);
+    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
+    Grid3 G = grid3_run(D.w, D.h, D.d, nz);
     Sphere S = make_sphere(src_sphere);
     const double cells = (double)D.d * D.h * D.w * sizeof(T);
     // read rho and every velocity component once, write rho'
@@ -160,9 +216,10 @@ pano_status launch_advect_mac(Ctx* c, const Dims& D, T* dst, const T* src, doubl
     Faces<const T> q = split_faces(D, src);
     Faces<T> o = split_faces(D, dst);
     Sphere S = make_sphere(src_sphere);
-    Grid3 Gx = grid3(D.w + 1, D.h, D.d), Gy = grid3(D.w, D.h + 1, D.d);
+    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
+    Grid3 Gx = grid3_run(D.w + 1, D.h, D.d, nz), Gy = grid3_run(D.w, D.h + 1, D.d, nz);
     if (D.d > 1) {
-        Grid3 Gz = grid3(D.w, D.h, D.d + 1);
+        Grid3 Gz = grid3_run(D.w, D.h, D.d + 1, nz);
         LAUNCH3((k_advect_fx<T, 3>), Gx, D, Gx, o.x, q.x, (T)dt, v);
         LAUNCH3((k_advect_fy<T, 3>), Gy, D, Gy, o.y, q.y, (T)dt, v);
         LAUNCH3((k_advect_fz<T>), Gz, D, Gz, o.z, q.z, (T)dt, v, (T)buoy_c, rho, S);
@@ -176,7 +233,7 @@ pano_status launch_advect_mac(Ctx* c, const Dims& D, T* dst, const T* src, doubl
 template <typename T>
 pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces) {
     Faces<const T> v = split_faces(D, faces);
-    Grid3 G = grid3(D.w, D.h, D.d);
+    Grid3 G = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
     if (D.d > 1) LAUNCH3((k_neg_div<T, 3>), G, D, G, b, v);
     else LAUNCH3((k_neg_div<T, 2>), G, D, G, b, v);
     return PANO_OK;
@@ -201,9 +258,10 @@ pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t
 template <typename T, int L_>
 static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
     Faces<T> u = split_faces(D, faces);
-    Grid3 Gx = grid3(D.w + 1, D.h, D.d), Gy = grid3(D.w, D.h + 1, D.d);
+    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
+    Grid3 Gx = grid3_run(D.w + 1, D.h, D.d, nz), Gy = grid3_run(D.w, D.h + 1, D.d, nz);
     if (D.d > 1) {
-        Grid3 Gz = grid3(D.w, D.h, D.d + 1);
+        Grid3 Gz = grid3_run(D.w, D.h, D.d + 1, nz);
         LAUNCH3((k_project_x<T, 3, L_>), Gx, D, Gx, u.x, p, (T)dt);
         LAUNCH3((k_project_y<T, 3, L_>), Gy, D, Gy, u.y, p, (T)dt);
         LAUNCH3((k_project_z<T, L_>), Gz, D, Gz, u.z, p, (T)dt);

@@ -49,6 +49,7 @@ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 __device__ __forceinline__ float tfloor(float v) { return floorf(v); }
 __device__ __forceinline__ double tfloor(double v) { return floor(v); }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 // ---- face layout views ---------------------------------------------------------------
 template <typename T>
@@ -456,6 +457,12 @@ inline Grid3 grid3(int64_t W, int64_t H, int64_t Dz) {
     g.nb = g.gx * g.gy * g.gz;
     return g;
 }
+// Plane runs of the natural-layout fluid kernels: in 3D a thread computes
+// kPlaneRun values along z (every load of the run first), then stores them.
+template <int DIM>
+constexpr int kPlaneRun = DIM == 3 ? 4 : 1;
+// tiles of a W x H x P launch with runs of nz planes
+inline Grid3 grid3_run(int64_t W, int64_t H, int64_t P, int nz) { return grid3(W, H, (P + nz - 1) / nz); }
 __device__ __forceinline__ void tile_xyz(const Grid3& g, int& x, int& y, int& z, int& L) {
     L = xcd_remap(blockIdx.x, g.nb);
     int bx = L % g.gx;

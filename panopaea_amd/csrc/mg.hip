@@ -14,6 +14,7 @@
 // symmetric positive definite preconditioner for CG.
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "mg.hpp"
 #include "mg_device.hpp"
@@ -22,27 +23,40 @@ namespace pano {
 
 // ---- single-grid kernels: tile decode + the shared per-element bodies -------------------
 
-// First full sweep from z = 0 (no read of z).
-template <typename T, int DIM>
+#define RUN_TILE(G, NZ)                \
+    int m, y, zb, L;                   \
+    tile_xyz(G, m, y, zb, L);          \
+    const int z0 = zb * (NZ);
+
+// Red cells of the first sweep from z = 0 (the black half-sweep follows).
+template <typename T, int DIM, int NZ>
+__global__ __launch_bounds__(256) void k_red_init(Dims D, Grid3 G, T* zv, const T* r, T s, RInv<T> ri,
+                                                  const SolverState* st) {
+    if (st->done) return;
+    RUN_TILE(G, NZ);
+    if (m >= (D.w >> 1) || y >= D.h) return;
+    red_init_run<T, DIM, NZ>(D, zv, r, s, ri, m, y, z0);
+}
+
+// The whole first sweep from z = 0 (2D levels: one launch instead of two).
+template <typename T, int DIM, int NZ>
 __global__ __launch_bounds__(256) void k_smooth_first(Dims D, Grid3 G, T* zv, const T* r, T s, RInv<T> ri,
                                                       const SolverState* st) {
     if (st->done) return;
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    RUN_TILE(G, NZ);
     if (m >= (D.w >> 1) || y >= D.h) return;
-    smooth_first_pair<T, DIM>(D, zv, r, s, ri, m, y, z);
+    smooth_first_run<T, DIM, NZ>(D, zv, r, s, ri, m, y, z0);
 }
 
 // One colour of a red-black Gauss-Seidel sweep. With part, also the
 // workgroup's partial sum of z.r over the updated cells.
-template <typename T, int DIM>
+template <typename T, int DIM, int NZ>
 __global__ __launch_bounds__(256) void k_smooth_half(Dims D, Grid3 G, T* zv, const T* r, T s, RInv<T> ri,
                                                      int color, double* part, const SolverState* st) {
     if (st->done) return;
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    RUN_TILE(G, NZ);
     double acc = 0.0;
-    if (m < (D.w >> 1) && y < D.h) acc = smooth_half_cell<T, DIM>(D, zv, r, s, ri, color, m, y, z);
+    if (m < (D.w >> 1) && y < D.h) acc = smooth_half_run<T, DIM, NZ>(D, zv, r, s, ri, color, m, y, z0);
     if (part) {  // uniform across the workgroup
         double t = block_sum(acc);
         if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
@@ -50,14 +64,13 @@ __global__ __launch_bounds__(256) void k_smooth_half(Dims D, Grid3 G, T* zv, con
 }
 
 // res = r - s*(nc*z - sum)
-template <typename T, int DIM>
+template <typename T, int DIM, int NZ>
 __global__ __launch_bounds__(256) void k_mg_residual(Dims D, Grid3 G, T* res, const T* zv, const T* r, T s,
                                                      const SolverState* st) {
     if (st->done) return;
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    RUN_TILE(G, NZ);
     if (m >= (D.w >> 1) || y >= D.h) return;
-    residual_pair<T, DIM>(D, res, zv, r, s, m, y, z);
+    residual_run<T, DIM, NZ>(D, res, zv, r, s, m, y, z0);
 }
 
 // r_c = R res
@@ -72,14 +85,89 @@ __global__ __launch_bounds__(256) void k_restrict(Dims Df, Dims Dc, Grid3 G, T* 
 }
 
 // z_f += P z_c
-template <typename T, int DIM, int LC>
+template <typename T, int DIM, int LC, int NZ>
 __global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* zf, const T* zc,
                                                  const SolverState* st) {
     if (st->done) return;
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    RUN_TILE(G, NZ);
     if (m >= (Df.w >> 1) || y >= Df.h) return;
-    prolong_pair<T, DIM, LC>(Df, Dc, zf, zc, m, y, z);
+    prolong_run<T, DIM, LC, NZ>(Df, Dc, zf, zc, m, y, z0);
+}
+
+// ---- 3D f32 levels: prolongation from LDS ------------------------------------------------
+// z_f += P z_c (3D) on the pair tiles of grid3_rb, with the workgroup's coarse
+// neighbourhood ((kTX+2) x (kTY/2+2) x ((NZ+1)/2+2) cells) copied to LDS
+// first; the arithmetic of prolong_run.
+template <int LC, int NZ>
+__global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G, float* zf, const float* zc,
+                                                      const SolverState* st) {
+    if (st->done) return;
+    constexpr int CX = kTX + 2, CY = kTY / 2 + 2, CZ = (NZ + 1) / 2 + 2;
+    __shared__ float ct[CZ * CY * CX];
+    const int L = xcd_remap(blockIdx.x, G.nb);
+    const int bx = L % G.gx, tq = L / G.gx, by = tq % G.gy, bz = tq / G.gy;
+    const int m0 = bx * kTX, y0 = by * kTY, z0 = bz * NZ;
+    const int Ib = m0 - 1, Jb = (y0 >> 1) - 1, Kb = (z0 >> 1) - 1;
+    const int tid = threadIdx.x + kTX * threadIdx.y;
+    constexpr int NP = (CZ * CY * CX + kTX * kTY - 1) / (kTX * kTY);
+    float vc[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {  // all loads first (clamped, valid indices)
+        int e = imin(tid + i * kTX * kTY, CZ * CY * CX - 1);
+        int ix = e % CX, t2 = e / CX, iy = t2 % CY, iz = t2 / CY;
+        int I = imin(imax(Ib + ix, 0), Dc.w - 1);
+        int J = imin(imax(Jb + iy, 0), Dc.h - 1);
+        int K = imin(imax(Kb + iz, 0), Dc.d - 1);
+        vc[i] = zc[cidx<LC>(Dc, K, J, I)];
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+        if (tid + i * kTX * kTY < CZ * CY * CX) ct[tid + i * kTX * kTY] = vc[i];
+    __syncthreads();
+    const int m = m0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (m >= (Df.w >> 1) || y >= Df.h) return;
+    const float w34 = 0.75f, w14 = 0.25f, one = 1.0f, zero = 0.0f;
+    int J0, J1;
+    bool ey;
+    p1(y, Dc.h, J0, J1, ey);
+    const float wy0 = ey ? one : w34, wy1 = ey ? zero : w14;
+    const bool el = m == 0, er = m + 1 >= Dc.w;
+    const int Il = el ? m : m - 1, Ir = er ? m : m + 1;
+    const float wl0 = el ? one : w34, wl1 = el ? zero : w14;
+    const float wr0 = er ? one : w34, wr1 = er ? zero : w14;
+    float f0[NZ], f1[NZ];
+    int i0[NZ], i1[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        RBRow R = rb_row<3>(Df, zclamp(Df, z0 + k), y, m);
+        i0[k] = R.i0;
+        i1[k] = R.i1;
+        f0[k] = zf[R.i0];
+        f1[k] = zf[R.i1];
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = z0 + k;
+        if (z >= Df.d) continue;
+        int K0, K1;
+        bool ez;
+        p1(z, Dc.d, K0, K1, ez);
+        const float wz0 = ez ? one : w34, wz1 = ez ? zero : w14;
+        float px0[2][2], px1[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const float* row = ct + (((a == 0 ? K0 : K1) - Kb) * CY + ((b == 0 ? J0 : J1) - Jb)) * CX - Ib;
+                float cm = row[m], cl = row[Il], cr = row[Ir];
+                px0[a][b] = wl0 * cm + wl1 * cl;
+                px1[a][b] = wr0 * cm + wr1 * cr;
+            }
+        float v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
+        float v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
+        zf[i0[k]] = f0[k] + v0;
+        zf[i1[k]] = f1[k] + v1;
+    }
 }
 
 // ---- coarsest level: fast diagonalisation in FP64 (natural layout) ---------------------
@@ -303,7 +391,7 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     const double *lx = m->fdm_lam[0], *ly = m->fdm_lam[1], *lz = m->fdm_lam[2];
     if (DIM == 2 && nx <= kFdmPlane && ny <= kFdmPlane) {
         // rows (forward x), columns (y, divide, inverse y), rows (inverse x)
-        size_t lds = 2 * sizeof(double) * (size_t)kFdmPlane;
+        size_t lds = kFdmLdsBytes;
         unsigned nr = (unsigned)((ny + fdm2_rows_per(nx) - 1) / fdm2_rows_per(nx));
         unsigned ncol = (unsigned)((nx + fdm2_cols_per(ny) - 1) / fdm2_cols_per(ny));
         hipLaunchKernelGGL((k_fdm2_rows<T, double>), dim3(nr), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, st);
@@ -314,7 +402,7 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
         return PANO_OK;
     }
     if (DIM == 3 && nx * ny <= kFdmPlane && nx * nz <= kFdmPlane) {
-        size_t lds = 2 * sizeof(double) * (size_t)kFdmPlane;
+        size_t lds = kFdmLdsBytes;
         {
             hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, Vy,
                                (const double*)nullptr, (const double*)nullptr, s, eps, st);
@@ -350,6 +438,89 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     return PANO_OK;
 }
 
+// One level of the V-cycle's down leg: pre-smoothing from z = 0, residual,
+// restriction into level l+1. NZ = run_len of the level.
+template <typename T, int DIM, int NZ>
+static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, const SolverState* st) {
+    const int Lc = m->levels - 1;
+    const Dims& D = m->dims[l];
+    const Dims& Dc = m->dims[l + 1];
+    const dim3 B(kTX, kTY);
+    const double cfrac = DIM == 3 ? 0.125 : 0.25;
+    T s = (T)m->s[l];
+    RInv<T> ri = make_rinv<T>(s);
+    Grid3 G = grid3_rb(D);
+    const double unit = (double)D.d * D.h * D.w * sizeof(T);
+    const char* nh = l == 0 ? "smooth_l0" : "";
+    // first sweep from z = 0: on bandwidth-bound levels red cells then the
+    // black half-sweep (two light passes), elsewhere one fused launch
+    if (NZ == 4) {
+        klaunch(c, l == 0 ? "red_init_l0" : "", unit, k_red_init<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s,
+                ri, st);
+        klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
+                (double*)nullptr, st);
+    } else {
+        klaunch(c, l == 0 ? "smooth_first_l0" : "", 2 * unit, k_smooth_first<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l],
+                rl[l], s, ri, st);
+    }
+    for (int k = 1; k < m->nu; ++k) {
+        klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0,
+                (double*)nullptr, st);
+        klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
+                (double*)nullptr, st);
+    }
+    klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM, NZ>, dim3(G.nb), B, D, G, m->res, zl[l],
+            rl[l], s, st);
+    Grid3 Gc = grid3(Dc.w, Dc.h, Dc.d);
+    const char* nr = l == 0 ? "restrict_l0" : "";
+    if (l + 1 == Lc)
+        klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, NATURAL>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                (const T*)m->res, st);
+    else
+        klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, REDBLACK>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                (const T*)m->res, st);
+}
+
+// One level of the up leg: prolongation from level l+1, post-smoothing; at
+// level 0 the last (black, red) pair also yields the partials of z.r.
+template <typename T, int DIM, int NZ>
+static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, const SolverState* st,
+                     double* dot_part, int* dot_np) {
+    const int Lc = m->levels - 1;
+    const Dims& D = m->dims[l];
+    const Dims& Dc = m->dims[l + 1];
+    const dim3 B(kTX, kTY);
+    const double cfrac = DIM == 3 ? 0.125 : 0.25;
+    T s = (T)m->s[l];
+    RInv<T> ri = make_rinv<T>(s);
+    Grid3 G = grid3_rb(D);
+    const double unit = (double)D.d * D.h * D.w * sizeof(T);
+    const char* nh = l == 0 ? "smooth_l0" : "";
+    const char* np = l == 0 ? "prolong_l0" : "";
+    if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+        if (l + 1 == Lc)
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
+                    (const T*)m->z[l + 1], st);
+        else
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
+                    (const T*)m->z[l + 1], st);
+    } else if (l + 1 == Lc) {
+        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
+                (const T*)m->z[l + 1], st);
+    } else {
+        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
+                (const T*)m->z[l + 1], st);
+    }
+    for (int k = 0; k < m->nu; ++k) {
+        bool last = l == 0 && k == m->nu - 1 && dot_part;
+        double* pb = last ? dot_part : nullptr;
+        double* pr = last ? dot_part + G.nb : nullptr;
+        klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1, pb, st);
+        klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0, pr, st);
+        if (last) *dot_np = 2 * G.nb;
+    }
+}
+
 template <typename T, int DIM>
 static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const SolverState* st, double* dot_part,
                               int* dot_np) {
@@ -362,59 +533,19 @@ static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const Solver
         zl[l] = m->z[l];
         rl[l] = m->r[l];
     }
-    const dim3 B(kTX, kTY);
-    const double cfrac = DIM == 3 ? 0.125 : 0.25;
     for (int l = 0; l < Lc; ++l) {
-        const Dims& D = m->dims[l];
-        const Dims& Dc = m->dims[l + 1];
-        T s = (T)m->s[l];
-        RInv<T> ri = make_rinv<T>(s);
-        Grid3 G = grid3(D.w >> 1, D.h, D.d);
-        const double unit = (double)D.d * D.h * D.w * sizeof(T);
-        const char* nh = l == 0 ? "smooth_l0" : "";
-        klaunch(c, l == 0 ? "smooth_first_l0" : "", 2 * unit, k_smooth_first<T, DIM>, dim3(G.nb), B, D, G, zl[l],
-                rl[l], s, ri, st);
-        for (int k = 1; k < m->nu; ++k) {
-            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0,
-                    (double*)nullptr, st);
-            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
-                    (double*)nullptr, st);
+        switch (run_len(m->dims[l])) {
+            case 4: level_down<T, DIM, 4>(c, m, l, zl, rl, st); break;
+            case 2: level_down<T, DIM, 2>(c, m, l, zl, rl, st); break;
+            default: level_down<T, DIM, 1>(c, m, l, zl, rl, st); break;
         }
-        klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM>, dim3(G.nb), B, D, G, m->res, zl[l],
-                rl[l], s, st);
-        Grid3 Gc = grid3(Dc.w, Dc.h, Dc.d);
-        const char* nr = l == 0 ? "restrict_l0" : "";
-        if (l + 1 == Lc)
-            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, NATURAL>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
-                    (const T*)m->res, st);
-        else
-            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, REDBLACK>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
-                    (const T*)m->res, st);
     }
     PANO_CHECK(coarse_solve<T, DIM>(c, m, m->z[Lc], m->r[Lc], st));
     for (int l = Lc - 1; l >= 0; --l) {
-        const Dims& D = m->dims[l];
-        const Dims& Dc = m->dims[l + 1];
-        T s = (T)m->s[l];
-        RInv<T> ri = make_rinv<T>(s);
-        Grid3 G = grid3(D.w >> 1, D.h, D.d);
-        const double unit = (double)D.d * D.h * D.w * sizeof(T);
-        const char* nh = l == 0 ? "smooth_l0" : "";
-        const char* np = l == 0 ? "prolong_l0" : "";
-        if (l + 1 == Lc)
-            klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL>, dim3(G.nb), B, D, Dc, G, zl[l],
-                    (const T*)m->z[l + 1], st);
-        else
-            klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, REDBLACK>, dim3(G.nb), B, D, Dc, G, zl[l],
-                    (const T*)m->z[l + 1], st);
-        for (int k = 0; k < m->nu; ++k) {
-            // the final (black, red) pair at level 0 also yields the partials of z.r
-            bool last = l == 0 && k == m->nu - 1 && dot_part;
-            double* pb = last ? dot_part : nullptr;
-            double* pr = last ? dot_part + G.nb : nullptr;
-            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1, pb, st);
-            klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0, pr, st);
-            if (last) *dot_np = 2 * G.nb;
+        switch (run_len(m->dims[l])) {
+            case 4: level_up<T, DIM, 4>(c, m, l, zl, rl, st, dot_part, dot_np); break;
+            case 2: level_up<T, DIM, 2>(c, m, l, zl, rl, st, dot_part, dot_np); break;
+            default: level_up<T, DIM, 1>(c, m, l, zl, rl, st, dot_part, dot_np); break;
         }
     }
     PANO_HIP(hipGetLastError());

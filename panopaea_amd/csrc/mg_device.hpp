@@ -6,6 +6,8 @@
 // (z, y, x>>1); a thread owns the x-pair (2m, 2m+1) of row (z, y).
 #pragma once
 
+#include <type_traits>
+
 #include "fluid_device.hpp"
 
 namespace pano {
@@ -35,89 +37,6 @@ __device__ __forceinline__ T nsum(const T* n) {
 template <typename T>
 __device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RInv<T>& ri) {
     return nc ? (r + s * sum) * ri.v[nc] : (T)0;
-}
-
-// q = A p for the x-pair (m, y, z); returns the pair's p.q in FP64.
-template <typename T, int DIM>
-__device__ __forceinline__ double apply_pair(const Dims& D, T* q, const T* p, T dt, int m, int y, int z) {
-    RBRow R = rb_row<DIM>(D, z, y, m);
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        int i = k == 0 ? R.i0 : R.i1;
-        int has = k == 0 ? R.has0 : R.has1;
-        T n[6];
-        rb_nbrs<T, DIM>(p, R, k, n);
-        T pc = p[i];
-        T v = poisson_from<T, DIM>(pc, n[0], n[1], n[2], n[3], n[4], n[5], has, D.bc, dt);
-        q[i] = v;
-        s += (double)pc * (double)v;
-    }
-    return s;
-}
-
-// First full sweep from z = 0 for the x-pair: red z = (r + s*0)*rinv[nc];
-// the black update recomputes its six red neighbours from r (no read of z).
-template <typename T, int DIM>
-__device__ __forceinline__ void smooth_first_pair(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri, int m,
-                                                  int y, int z) {
-    const T zero = (T)0;
-    RBRow R = rb_row<DIM>(D, z, y, m);
-    int c0 = (y + z) & 1;
-    int kr = c0;  // the red cell of the pair: x0 when c0 == 0, else x1
-    int kb = 1 - c0;
-    int ir = kr == 0 ? R.i0 : R.i1, ib = kb == 0 ? R.i0 : R.i1;
-    int hr = kr == 0 ? R.has0 : R.has1, hb = kb == 0 ? R.has0 : R.has1;
-    zv[ir] = gs_value(r[ir], s, zero, ncount_bits<DIM>(hr, D.bc), ri);
-    int xb = 2 * m + kb;
-    T rn[6];
-    rb_nbrs<T, DIM>(r, R, kb, rn);
-    T n[6];
-    n[0] = (hb & 1) ? gs_value(rn[0], s, zero, ncount<DIM>(D, z, y, xb - 1), ri) : zero;
-    n[1] = (hb & 2) ? gs_value(rn[1], s, zero, ncount<DIM>(D, z, y, xb + 1), ri) : zero;
-    n[2] = (hb & 4) ? gs_value(rn[2], s, zero, ncount<DIM>(D, z, y - 1, xb), ri) : zero;
-    n[3] = (hb & 8) ? gs_value(rn[3], s, zero, ncount<DIM>(D, z, y + 1, xb), ri) : zero;
-    if constexpr (DIM == 3) {
-        n[4] = (hb & 16) ? gs_value(rn[4], s, zero, ncount<DIM>(D, z - 1, y, xb), ri) : zero;
-        n[5] = (hb & 32) ? gs_value(rn[5], s, zero, ncount<DIM>(D, z + 1, y, xb), ri) : zero;
-    } else {
-        n[4] = zero;
-        n[5] = zero;
-    }
-    zv[ib] = gs_value(r[ib], s, nsum<T, DIM>(n), ncount_bits<DIM>(hb, D.bc), ri);
-}
-
-// One colour of a red-black Gauss-Seidel sweep at (m, y, z); returns z.r of
-// the updated cell in FP64.
-template <typename T, int DIM>
-__device__ __forceinline__ double smooth_half_cell(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri,
-                                                   int color, int m, int y, int z) {
-    RBRow R = rb_row<DIM>(D, z, y, m);
-    int k = (color + y + z) & 1;  // parity of the updated cell's x
-    int i = k == 0 ? R.i0 : R.i1;
-    int has = k == 0 ? R.has0 : R.has1;
-    T n[6];
-    rb_nbrs<T, DIM>(zv, R, k, n);
-    T rv = r[i];
-    T v = gs_value(rv, s, nsum<T, DIM>(n), ncount_bits<DIM>(has, D.bc), ri);
-    zv[i] = v;
-    return (double)v * (double)rv;
-}
-
-// res = r - s*(nc*z - sum) for the x-pair
-template <typename T, int DIM>
-__device__ __forceinline__ void residual_pair(const Dims& D, T* res, const T* zv, const T* r, T s, int m, int y,
-                                              int z) {
-    RBRow R = rb_row<DIM>(D, z, y, m);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        int i = k == 0 ? R.i0 : R.i1;
-        int has = k == 0 ? R.has0 : R.has1;
-        T n[6];
-        rb_nbrs<T, DIM>(zv, R, k, n);
-        T az = s * ((T)ncount_bits<DIM>(has, D.bc) * zv[i] - nsum<T, DIM>(n));
-        res[i] = r[i] - az;
-    }
 }
 
 // 1D restriction weights: (2I-1)/8, (2I)*3/8, (2I+1)*3/8, (2I+2)/8, a missing
@@ -195,56 +114,281 @@ __device__ __forceinline__ void p1(int i, int n, int& I0, int& I1, bool& edge) {
     I1 = edge ? I : J;
 }
 
-// z_f += P z_c (nested: x, then y, then z interpolation) for the x-pair
-// (2m, 2m+1): coarse column m with m-1 (for 2m) and m+1 (for 2m+1).
-template <typename T, int DIM, int LC>
-__device__ __forceinline__ void prolong_pair(const Dims& Df, const Dims& Dc, T* zf, const T* zc, int m, int y,
-                                             int z) {
-    const T w34 = (T)0.75, w14 = (T)0.25, one = (T)1, zero = (T)0;
-    int J0, J1, K0 = 0, K1 = 0;
-    bool ey, ez = true;
-    p1(y, Dc.h, J0, J1, ey);
-    T wy0 = ey ? one : w34, wy1 = ey ? zero : w14;
-    T wz0 = one, wz1 = zero;
-    if constexpr (DIM == 3) {
-        p1(z, Dc.d, K0, K1, ez);
-        wz0 = ez ? one : w34;
-        wz1 = ez ? zero : w14;
+// ---- z-runs: NZ consecutive cells along z per thread ------------------------------------
+// A thread handles the x-pair m of rows (z0 + k, y), k < NZ, issuing every load
+// of the run before any store. No launch loads a cell that another thread of
+// it stores (a half-sweep writes one colour and reads the other; the other
+// bodies write a different field, or cells that only their own thread reads),
+// so the order of loads and stores does not change any value. Run cells beyond
+// the last plane load a valid (clamped) row and store nothing. The per-thread
+// partial sums add the run's terms in k order.
+// Planes per thread of the pair kernels at a level: long runs where the
+// level is large enough to be bandwidth-bound, single planes on the small,
+// latency-bound levels (and in 2D).
+__host__ __device__ inline int run_len(const Dims& D) {
+    if (D.d <= 1) return 1;
+    int64_t n = (int64_t)D.d * D.h * D.w;
+    return n >= (int64_t(1) << 23) ? 4 : (n >= (int64_t(1) << 20) ? 2 : 1);
+}
+// red-black pair tiles of a level: x-pairs by rows by runs of planes
+inline Grid3 grid3_rb(const Dims& D) {
+    int nz = run_len(D);
+    return grid3(D.w >> 1, D.h, (D.d + nz - 1) / nz);
+}
+// Calls f(dim, run) with compile-time constants for the level's dimension
+// and run length.
+template <typename F>
+__host__ __device__ inline auto with_run(const Dims& D, F&& f) {
+    using C2 = std::integral_constant<int, 2>;
+    using C3 = std::integral_constant<int, 3>;
+    using R1 = std::integral_constant<int, 1>;
+    using R2 = std::integral_constant<int, 2>;
+    using R4 = std::integral_constant<int, 4>;
+    if (D.d <= 1) return f(C2{}, R1{});
+    int nz = run_len(D);
+    if (nz == 4) return f(C3{}, R4{});
+    if (nz == 2) return f(C3{}, R2{});
+    return f(C3{}, R1{});
+}
+__device__ __forceinline__ int zclamp(const Dims& D, int z) { return z < D.d ? z : D.d - 1; }
+
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri,
+                                                  int color, int m, int y, int z0) {
+    T nb[NZ][6], rv[NZ];
+    int idx[NZ], nc[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(D, z0 + k);
+        RBRow R = rb_row<DIM>(D, z, y, m);
+        int c = (color + y + z) & 1;  // parity of the updated cell's x
+        idx[k] = c == 0 ? R.i0 : R.i1;
+        int has = c == 0 ? R.has0 : R.has1;
+        rb_nbrs<T, DIM>(zv, R, c, nb[k]);
+        rv[k] = r[idx[k]];
+        nc[k] = ncount_bits<DIM>(has, D.bc);
     }
-    // x weights: x0 = 2m pairs with m-1, x1 = 2m+1 with m+1 (reflected at the ends)
-    bool el = m == 0, er = m + 1 >= Dc.w;
-    int Il = el ? m : m - 1, Ir = er ? m : m + 1;
-    T wl0 = el ? one : w34, wl1 = el ? zero : w14;
-    T wr0 = er ? one : w34, wr1 = er ? zero : w14;
-    T px0[2][2], px1[2][2];
+    double acc = 0.0;
 #pragma unroll
-    for (int a = 0; a < (DIM == 3 ? 2 : 1); ++a) {
-        int kk = a == 0 ? K0 : K1;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            int jj = b == 0 ? J0 : J1;
-            T cm = zc[cidx<LC>(Dc, kk, jj, m)];
-            T cl = zc[cidx<LC>(Dc, kk, jj, Il)];
-            T cr = zc[cidx<LC>(Dc, kk, jj, Ir)];
-            px0[a][b] = wl0 * cm + wl1 * cl;
-            px1[a][b] = wr0 * cm + wr1 * cr;
+    for (int k = 0; k < NZ; ++k) {
+        if (z0 + k < D.d) {
+            T v = gs_value(rv[k], s, nsum<T, DIM>(nb[k]), nc[k], ri);
+            zv[idx[k]] = v;
+            acc += (double)v * (double)rv[k];
         }
     }
-    T v0, v1;
-    if constexpr (DIM == 3) {
-        v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
-        v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
-    } else {
-        v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
-        v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
+    return acc;
+}
+
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T dt, int m, int y, int z0) {
+    T nb[NZ][2][6], pc[NZ][2];
+    int idx[NZ][2], hs[NZ][2];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(D, z0 + k);
+        RBRow R = rb_row<DIM>(D, z, y, m);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            idx[k][c] = c == 0 ? R.i0 : R.i1;
+            hs[k][c] = c == 0 ? R.has0 : R.has1;
+            rb_nbrs<T, DIM>(p, R, c, nb[k][c]);
+            pc[k][c] = p[idx[k][c]];
+        }
     }
-    RBRow R = rb_row<DIM>(Df, z, y, m);
-    zf[R.i0] = zf[R.i0] + v0;
-    zf[R.i1] = zf[R.i1] + v1;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        if (z0 + k < D.d) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const T* n = nb[k][c];
+                T v = poisson_from<T, DIM>(pc[k][c], n[0], n[1], n[2], n[3], n[4], n[5], hs[k][c], D.bc, dt);
+                q[idx[k][c]] = v;
+                s += (double)pc[k][c] * (double)v;
+            }
+        }
+    }
+    return s;
+}
+
+// Red cells of the first sweep from z = 0: z = (r + s*0) * rinv[nc]; the
+// black half-sweep that follows completes the sweep.
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ void red_init_run(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri, int m, int y,
+                                             int z0) {
+    T rr[NZ];
+    int ir[NZ], hr[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(D, z0 + k);
+        RBRow R = rb_row<DIM>(D, z, y, m);
+        int c0 = (y + z) & 1;  // the red (colour 0) cell of the pair is x0 when c0 == 0
+        ir[k] = c0 == 0 ? R.i0 : R.i1;
+        hr[k] = c0 == 0 ? R.has0 : R.has1;
+        rr[k] = r[ir[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) zv[ir[k]] = gs_value(rr[k], s, (T)0, ncount_bits<DIM>(hr[k], D.bc), ri);
+}
+
+// The whole first sweep from z = 0 for the x-pair run (both colours in one
+// pass: the black update recomputes its red neighbours from r). Same values
+// as red_init_run followed by the black half-sweep; cheaper where launches
+// are latency-bound (2D levels).
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ void smooth_first_run(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri, int m,
+                                                 int y, int z0) {
+    const T zero = (T)0;
+    T rr[NZ], rbv[NZ], rn[NZ][6];
+    int ir[NZ], ib[NZ], hr[NZ], hb[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(D, z0 + k);
+        RBRow R = rb_row<DIM>(D, z, y, m);
+        int c0 = (y + z) & 1;
+        int kr = c0, kb = 1 - c0;  // the red cell of the pair is x0 when c0 == 0
+        ir[k] = kr == 0 ? R.i0 : R.i1;
+        ib[k] = kb == 0 ? R.i0 : R.i1;
+        hr[k] = kr == 0 ? R.has0 : R.has1;
+        hb[k] = kb == 0 ? R.has0 : R.has1;
+        rr[k] = r[ir[k]];
+        rbv[k] = r[ib[k]];
+        rb_nbrs<T, DIM>(r, R, kb, rn[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = z0 + k;
+        if (z >= D.d) continue;
+        int xb = 2 * m + (1 - ((y + z) & 1));
+        int h = hb[k];
+        zv[ir[k]] = gs_value(rr[k], s, zero, ncount_bits<DIM>(hr[k], D.bc), ri);
+        T n[6];
+        n[0] = (h & 1) ? gs_value(rn[k][0], s, zero, ncount<DIM>(D, z, y, xb - 1), ri) : zero;
+        n[1] = (h & 2) ? gs_value(rn[k][1], s, zero, ncount<DIM>(D, z, y, xb + 1), ri) : zero;
+        n[2] = (h & 4) ? gs_value(rn[k][2], s, zero, ncount<DIM>(D, z, y - 1, xb), ri) : zero;
+        n[3] = (h & 8) ? gs_value(rn[k][3], s, zero, ncount<DIM>(D, z, y + 1, xb), ri) : zero;
+        if constexpr (DIM == 3) {
+            n[4] = (h & 16) ? gs_value(rn[k][4], s, zero, ncount<DIM>(D, z - 1, y, xb), ri) : zero;
+            n[5] = (h & 32) ? gs_value(rn[k][5], s, zero, ncount<DIM>(D, z + 1, y, xb), ri) : zero;
+        } else {
+            n[4] = zero;
+            n[5] = zero;
+        }
+        zv[ib[k]] = gs_value(rbv[k], s, nsum<T, DIM>(n), ncount_bits<DIM>(h, D.bc), ri);
+    }
+}
+
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ void residual_run(const Dims& D, T* res, const T* zv, const T* r, T s, int m, int y,
+                                             int z0) {
+    T nb[NZ][2][6], zc[NZ][2], rc[NZ][2];
+    int idx[NZ][2], hs[NZ][2];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(D, z0 + k);
+        RBRow R = rb_row<DIM>(D, z, y, m);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            idx[k][c] = c == 0 ? R.i0 : R.i1;
+            hs[k][c] = c == 0 ? R.has0 : R.has1;
+            rb_nbrs<T, DIM>(zv, R, c, nb[k][c]);
+            zc[k][c] = zv[idx[k][c]];
+            rc[k][c] = r[idx[k][c]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        if (z0 + k < D.d) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                T az = s * ((T)ncount_bits<DIM>(hs[k][c], D.bc) * zc[k][c] - nsum<T, DIM>(nb[k][c]));
+                res[idx[k][c]] = rc[k][c] - az;
+            }
+        }
+    }
+}
+
+template <typename T, int DIM, int LC, int NZ>
+__device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* zf, const T* zc, int m, int y,
+                                            int z0) {
+    const T w34 = (T)0.75, w14 = (T)0.25, one = (T)1, zero = (T)0;
+    int J0, J1;
+    bool ey;
+    p1(y, Dc.h, J0, J1, ey);
+    const T wy0 = ey ? one : w34, wy1 = ey ? zero : w14;
+    // x weights: x0 = 2m pairs with m-1, x1 = 2m+1 with m+1 (reflected at the ends)
+    const bool el = m == 0, er = m + 1 >= Dc.w;
+    const int Il = el ? m : m - 1, Ir = er ? m : m + 1;
+    const T wl0 = el ? one : w34, wl1 = el ? zero : w14;
+    const T wr0 = er ? one : w34, wr1 = er ? zero : w14;
+    T cm[NZ][2][2], cl[NZ][2][2], cr[NZ][2][2], f0[NZ], f1[NZ];
+    int i0[NZ], i1[NZ];
+    T wz0[NZ], wz1[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = zclamp(Df, z0 + k);
+        int K0 = 0, K1 = 0;
+        bool ez = true;
+        if constexpr (DIM == 3) p1(z, Dc.d, K0, K1, ez);
+        wz0[k] = DIM == 3 ? (ez ? one : w34) : one;
+        wz1[k] = DIM == 3 ? (ez ? zero : w14) : zero;
+#pragma unroll
+        for (int a = 0; a < (DIM == 3 ? 2 : 1); ++a) {
+            int kk = a == 0 ? K0 : K1;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                int jj = b == 0 ? J0 : J1;
+                cm[k][a][b] = zc[cidx<LC>(Dc, kk, jj, m)];
+                cl[k][a][b] = zc[cidx<LC>(Dc, kk, jj, Il)];
+                cr[k][a][b] = zc[cidx<LC>(Dc, kk, jj, Ir)];
+            }
+        }
+        RBRow R = rb_row<DIM>(Df, z, y, m);
+        i0[k] = R.i0;
+        i1[k] = R.i1;
+        f0[k] = zf[R.i0];
+        f1[k] = zf[R.i1];
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        if (z0 + k >= Df.d) continue;
+        T px0[2][2], px1[2][2];
+#pragma unroll
+        for (int a = 0; a < (DIM == 3 ? 2 : 1); ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                px0[a][b] = wl0 * cm[k][a][b] + wl1 * cl[k][a][b];
+                px1[a][b] = wr0 * cm[k][a][b] + wr1 * cr[k][a][b];
+            }
+        T v0, v1;
+        if constexpr (DIM == 3) {
+            v0 = wz0[k] * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1[k] * (wy0 * px0[1][0] + wy1 * px0[1][1]);
+            v1 = wz0[k] * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1[k] * (wy0 * px1[1][0] + wy1 * px1[1][1]);
+        } else {
+            v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
+            v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
+        }
+        zf[i0[k]] = f0[k] + v0;
+        zf[i1[k]] = f1[k] + v1;
+    }
 }
 
 // ---- coarsest level in LDS: whole planes of at most kFdmPlane cells --------------------
 constexpr int kFdmPlane = 4096;  // 32 KiB of doubles per buffer
+constexpr int kFdmVStage = 8192; // doubles of LDS for the staged 1D eigenbases
+// dynamic LDS of the coarsest-level kernels: two plane buffers, then the eigenbases
+constexpr size_t kFdmLdsBytes = (2 * (size_t)kFdmPlane + kFdmVStage) * sizeof(double);
+
+// Copies the n x n eigenbasis V to `dst` (LDS) when it fits beside `used`
+// doubles already staged there; returns the copy or V itself. The caller
+// synchronises before the first use.
+__device__ __forceinline__ const double* stage_basis(double* dst, int used, const double* V, int n) {
+    if (used + n * n > kFdmVStage) return V;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) dst[used + e] = V[e];
+    return dst + used;
+}
 
 // Transform the plane in `a` (ny rows of nx) along x (axis 0) or y (axis 1),
 // forward (coefficients = V^T v) or inverse (values = V c), into `b`.
@@ -281,6 +425,9 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
     double* b = lds + nx * ny;
     int P = nx * ny;
     for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = (double)src[e];
+    double* vs = lds + 2 * kFdmPlane;
+    Vx = stage_basis(vs, 0, Vx, nx);
+    Vy = stage_basis(vs, Vx == vs ? nx * nx : 0, Vy, ny);
     __syncthreads();
     if (mode == 1) {
         lds_axis(a, b, nx, ny, 1, 1, Vy);
@@ -326,6 +473,7 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     const int nr = min(R, ny - j0);
     const int P = nr * nx;
     for (int e = threadIdx.x; e < P; e += blockDim.x) lds[e] = (double)src[(int64_t)j0 * nx + e];
+    Vx = stage_basis(lds + 2 * kFdmPlane, 0, Vx, nx);
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int jl = e / nx, i = e - jl * nx;
@@ -352,6 +500,7 @@ __device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int 
         int j = e / nc, ci = e - j * nc;
         a[j * C + ci] = t[(int64_t)j * nx + i0 + ci];
     }
+    Vy = stage_basis(lds + 2 * kFdmPlane, 0, Vy, ny);
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int j = e / nc, ci = e - j * nc;
@@ -382,6 +531,7 @@ __device__ __forceinline__ void fdm_zcol_body(double* lds, int nx, int ny, int n
         int k = e / nx, i = e - k * nx;
         a[e] = data[k * zs + (int64_t)j * nx + i];
     }
+    Vz = stage_basis(lds + 2 * kFdmPlane, 0, Vz, nz);
     __syncthreads();
     lds_axis(a, b, nx, nz, 1, 0, Vz);  // slice rows are z: the "y" axis of lds_axis
     __syncthreads();

@@ -60,14 +60,14 @@ __global__ __launch_bounds__(256) void k_apply_dot_nat(Dims D, Grid3 G, T* q, co
 }
 
 // q = A p with partial p.q ; red-black layout (thread = x-pair).
-template <typename T, int DIM>
+template <typename T, int DIM, int NZ>
 __global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, const T* p, T dt, double* part,
                                                       const SolverState* st) {
     if (st->done) return;
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    int m, y, zb, L;
+    tile_xyz(G, m, y, zb, L);
     double s = 0.0;
-    if (m < (D.w >> 1) && y < D.h) s = apply_pair<T, DIM>(D, q, p, dt, m, y, z);
+    if (m < (D.w >> 1) && y < D.h) s = apply_run<T, DIM, NZ>(D, q, p, dt, m, y, zb * NZ);
     double t = block_sum(s);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
 }
@@ -168,7 +168,7 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
         PANO_CHECK(mg_setup<T>(c, D, P->mg_levels, P->mg_nu1, P->dt, &S->mg));
     }
     S->nflat = (int)blocks_for(S->n, kFlatPer);
-    S->G = S->rb ? grid3(D.w >> 1, D.h, D.d) : grid3(D.w, D.h, D.d);
+    S->G = S->rb ? grid3_rb(D) : grid3(D.w, D.h, D.d);
     S->ntile = S->G.nb;
     int np = S->nflat > 2 * S->ntile ? S->nflat : 2 * S->ntile;
     void *st, *p0, *p1;
@@ -259,11 +259,18 @@ static pano_status solver_iteration(Solver<T>* S) {
     const SolverState* cst = S->st;
     // q = A p ; alpha   (pcg.rs:51-53)
     if (S->rb) {
-        if (D.d > 1)
-            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+        const int nz = run_len(D);
+        if (D.d == 1)
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2, 1>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->part0, cst);
+        else if (nz == 4)
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, 4>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+                    S->part0, cst);
+        else if (nz == 2)
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, 2>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
                     S->part0, cst);
         else
-            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, 1>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
                     S->part0, cst);
     } else {
         if (D.d > 1)
@@ -411,10 +418,17 @@ template pano_status pcg_solve<double>(Ctx*, const Dims&, const pano_pcg_params*
 // b = -div u written straight into the solver's layout (natural or red-black).
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_neg_div_to(Dims D, Grid3 G, T* b, Faces<const T> v) {
-    int x, y, z, L;
-    tile_xyz(G, x, y, z, L);
+    constexpr int NZ = kPlaneRun<DIM>;
+    int x, y, zb, L;
+    tile_xyz(G, x, y, zb, L);
     if (x >= D.w || y >= D.h) return;
-    b[lidx<L_>(D, z, y, x)] = -div_value<T, DIM>(D, v, z, y, x);
+    const int z0 = zb * NZ;
+    T out[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, imin(z0 + k, D.d - 1), y, x);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+        if (z0 + k < D.d) b[lidx<L_>(D, z0 + k, y, x)] = out[k];
 }
 
 template <typename T>
@@ -444,12 +458,13 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     // b = -div u
     Faces<const T> v = split_faces(D, (const T*)vel_out);
     Grid3 Gn = grid3(D.w, D.h, D.d);
+    Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
     if (S.rb) {
-        if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gn, D, Gn, S.r, v);
-        else L3((k_neg_div_to<T, 2, REDBLACK>), Gn, D, Gn, S.r, v);
+        if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gd, D, Gd, S.r, v);
+        else L3((k_neg_div_to<T, 2, REDBLACK>), Gd, D, Gd, S.r, v);
     } else {
-        if (D.d > 1) L3((k_neg_div_to<T, 3, NATURAL>), Gn, D, Gn, S.r, v);
-        else L3((k_neg_div_to<T, 2, NATURAL>), Gn, D, Gn, S.r, v);
+        if (D.d > 1) L3((k_neg_div_to<T, 3, NATURAL>), Gd, D, Gd, S.r, v);
+        else L3((k_neg_div_to<T, 2, NATURAL>), Gd, D, Gd, S.r, v);
     }
     PANO_HIP(hipGetLastError());
     PANO_CHECK(solver_init<T>(&S, pp.max_iterations));
@@ -496,21 +511,26 @@ static pano_status precond_apply(Ctx* c, const Dims& D, const pano_pcg_params* P
 template <int DIM>
 __global__ __launch_bounds__(256) void k_resid64_rb(Dims D, Grid3 G, float* r32, const double* x, const double* b,
                                                     double dt, double* pmax) {
-    int m, y, z, L;
-    tile_xyz(G, m, y, z, L);
+    int m, y, zb, L;
+    tile_xyz(G, m, y, zb, L);
     double mx = 0.0;
     if (m < (D.w >> 1) && y < D.h) {
-        RBRow R = rb_row<DIM>(D, z, y, m);
+        const int nz = run_len(D);
+        for (int kz = 0; kz < nz; ++kz) {
+            int z = zb * nz + kz;
+            if (z >= D.d) break;
+            RBRow R = rb_row<DIM>(D, z, y, m);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            int i = k == 0 ? R.i0 : R.i1;
-            int has = k == 0 ? R.has0 : R.has1;
-            double n[6];
-            rb_nbrs<double, DIM>(x, R, k, n);
-            double ax = poisson_from<double, DIM>(x[i], n[0], n[1], n[2], n[3], n[4], n[5], has, D.bc, dt);
-            double r = b[i] - ax;
-            r32[i] = (float)r;
-            mx = absmax_acc(mx, r);
+            for (int k = 0; k < 2; ++k) {
+                int i = k == 0 ? R.i0 : R.i1;
+                int has = k == 0 ? R.has0 : R.has1;
+                double n[6];
+                rb_nbrs<double, DIM>(x, R, k, n);
+                double ax = poisson_from<double, DIM>(x[i], n[0], n[1], n[2], n[3], n[4], n[5], has, D.bc, dt);
+                double r = b[i] - ax;
+                r32[i] = (float)r;
+                mx = absmax_acc(mx, r);
+            }
         }
     }
     double t = block_max(mx);
