@@ -303,6 +303,16 @@ pano_status pano_server_fetch(pano_server, int64_t id, float* faces, float* pres
 /* Frees a completed request's device memory and forgets it. */
 pano_status pano_server_release(pano_server, int64_t id);
 
+/* ---- field export (dec_fluid.rs:143-164) ------------------------------------ */
+/* Grey RGB8 image of a cell field (2D, or z-slice z of a 3D field): pixel
+ * value = imgproc::transfer(x, lower, upper) (panopaea_utils/src/imgproc.rs:2-5:
+ * clamp, scale to 255, truncate; computed in double), image rows top-down =
+ * field rows bottom-up as png::export (panopaea_utils/src/png.rs:6-18) lays
+ * them out. rgb: device buffer of ny*nx*3 bytes. The PNG container is the
+ * caller's (the package's png_export writes it). Asynchronous. */
+pano_status pano_field_rgb8(pano_ctx, const pano_grid*, const void* cells, int64_t z, double lower, double upper,
+                            uint8_t* rgb);
+
 /* ---- seeded synthetic inputs (host, deterministic) ------------------------- */
 /* Band-limited velocity: each face component is the sum of n_modes
  * sinusoids amplitude*sin(2*pi*(k.X)/N + phi), integer k with

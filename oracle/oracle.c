@@ -14,6 +14,16 @@
 
 static inline int64_t mini64(int64_t a, int64_t b) { return a < b ? a : b; }
 
+/* imgproc::transfer, panopaea_utils/src/imgproc.rs:2-5: x.min(upper).max(lower)
+ * (f64::min/max return the other operand for a NaN, as fmin/fmax do), then
+ * ((v - lower) / (upper - lower) * 255.0) as u8 (truncation). */
+static inline unsigned char or_transfer(double x, double lower, double upper)
+{
+    double v = fmax(fmin(x, upper), lower);
+    double t = (v - lower) / (upper - lower) * 255.0;
+    return (unsigned char)(int)t;
+}
+
 /* Solve / step parameters shared by both scalar instantiations. */
 typedef struct {
     int64_t d, h, w;        /* cells; d == 1 is 2D */

@@ -299,3 +299,12 @@ def step(P, rho, faces):
     _fn("or_step", rho.dtype)(ctypes.byref(P), _p(rho), _p(vz), _p(vy), _p(vx), _p(p),
                               ctypes.byref(info))
     return p, info
+
+
+def field_rgb8(h, w, plane, lower=-2.0, upper=2.0):
+    """Grey RGB8 pixels (h, w, 3) of a cell plane (dec_fluid.rs:143-164)."""
+    plane = np.ascontiguousarray(plane)
+    out = np.zeros((h, w, 3), np.uint8)
+    _fn("or_field_rgb8", plane.dtype)(c_i64(h), c_i64(w), _p(plane), ctypes.c_double(lower),
+                                      ctypes.c_double(upper), out.ctypes.data_as(ctypes.c_void_p))
+    return out

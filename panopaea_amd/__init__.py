@@ -19,6 +19,8 @@ kernel for gfx950. There is no CPU fallback: without libpano.so or a GPU the
 calls raise.
 """
 import ctypes
+import struct
+import zlib
 
 import numpy as np
 
@@ -29,7 +31,7 @@ from ._lib import (PanoError, PcgInfo, PcgParams, StepParams, GridDesc, PANO_OK,
 __all__ = [
     "Context", "default_context", "DeviceField", "Grid2d", "Grid3d", "Identity", "Jacobi", "Multigrid",
     "PoissonOperator", "precond_conjugate_gradient", "advect", "advect_mac", "neg_divergence", "project",
-    "fluid_step", "PanoError", "StepConfig", "Batch", "Server",
+    "fluid_step", "PanoError", "StepConfig", "Batch", "Server", "field_rgb8", "png_export", "png_bytes",
 ]
 
 
@@ -569,6 +571,45 @@ def fluid_step(grid, cfg, density_in, velocity_in, density_out, velocity_out, pr
                                           pressure.ptr if pressure is not None else None, ctypes.byref(info)),
                allow=(PANO_OK, PANO_NOT_CONVERGED))
     return info
+
+
+def field_rgb8(field, lower=-2.0, upper=2.0, z=0):
+    """Grey RGB8 pixels (ny, nx, 3) of a cell field (one z-slice in 3D):
+    imgproc::transfer per cell, image rows top-down = field rows bottom-up
+    (examples/dec_fluid.rs:143-164, panopaea_utils/src/png.rs:6-18)."""
+    g = field.grid
+    h, w = g.dim()[-2:]
+    out = DeviceField(g, h * w * 3, "rgb8", ctx=field.ctx, dtype=np.uint8)
+    _lib.check(_lib.lib().pano_field_rgb8(field.ctx.handle, ctypes.byref(g.desc), field.ptr, int(z), float(lower),
+                                          float(upper), out.ptr))
+    rgb = out.download().reshape(h, w, 3)
+    out.free()
+    return rgb
+
+
+def _png_chunk(tag, data):
+    return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+
+def png_bytes(rgb):
+    """PNG file bytes of an (h, w, 3) uint8 image: 8-bit RGB, no interlace,
+    filter type 0 on every row, zlib-compressed image data."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    h, w, c = rgb.shape
+    if c != 3:
+        raise ValueError("expected an (h, w, 3) RGB array")
+    raw = np.zeros((h, 1 + 3 * w), np.uint8)
+    raw[:, 1:] = rgb.reshape(h, 3 * w)
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)
+    return (b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", zlib.compress(raw.tobytes(), 6)) +
+            _png_chunk(b"IEND", b""))
+
+
+def png_export(name, rgb):
+    """png::export (panopaea_utils/src/png.rs:6-18) for pixels from
+    field_rgb8: writes the PNG file `name`."""
+    with open(name, "wb") as f:
+        f.write(png_bytes(rgb))
 
 
 def gen_bandlimited_faces(grid, seed, n_modes, k_max, amplitude=1.0):

@@ -142,6 +142,7 @@ _SIGS = {
     "pano_server_status": ([c_void_p, c_i64, _P(ServerQuery)], c_i32),
     "pano_server_fetch": ([c_void_p, c_i64, c_void_p, c_void_p], c_i32),
     "pano_server_release": ([c_void_p, c_i64], c_i32),
+    "pano_field_rgb8": ([c_void_p, _P(GridDesc), c_void_p, c_i64, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_bandlimited_faces": ([_P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p], c_i32),
     "pano_gen_bandlimited_faces_device": ([c_void_p, _P(GridDesc), c_u64, c_i32, c_f64, c_f64, c_void_p],
                                           c_i32),

@@ -483,3 +483,57 @@ def test_step_projects_to_divergence_free(dims, bc, precond):
     assert np.abs(b_after).max() <= 2 * 1e-10 * info.b_linf + 1e-14
     if len(dims) == 3:
         assert rho.max() > 0.5  # the source was applied and advected
+
+
+# ---- field export: imgproc::transfer and png::export's row order --------------------------
+
+def test_transfer_closed_forms_and_row_flip():
+    lo, hi = -2.0, 2.0
+    step = (hi - lo) / 255.0
+    vals = np.array([lo, hi, 0.0, -5.0, 7.0, np.nan] + [lo + (k + 0.5) * step for k in range(0, 255, 17)])
+    want = [0, 255, 127, 0, 255, 255] + list(range(0, 255, 17))
+    h, w = 1, vals.size
+    img = o.field_rgb8(h, w, vals.astype(np.float64), lo, hi)
+    assert img.shape == (1, w, 3)
+    assert list(img[0, :, 0]) == want
+    assert np.array_equal(img[..., 0], img[..., 1]) and np.array_equal(img[..., 0], img[..., 2])
+    # image row y shows field row h-1-y (png.rs:11)
+    h, w = 5, 3
+    plane = np.repeat(np.linspace(-1.9, 1.9, h), w).reshape(h, w)
+    img = o.field_rgb8(h, w, plane, lo, hi)
+    rows = [o.field_rgb8(1, w, plane[r:r + 1].copy(), lo, hi)[0, :, 0] for r in range(h)]
+    for y in range(h):
+        assert np.array_equal(img[y, :, 0], rows[h - 1 - y])
+    # f32 input converts to double before the transfer
+    p32 = np.float32([0.1, -0.3, 1.7])
+    assert np.array_equal(o.field_rgb8(1, 3, p32, lo, hi), o.field_rgb8(1, 3, p32.astype(np.float64), lo, hi))
+
+
+def _png_decode(data):
+    """Minimal PNG reader for the writer's output (8-bit RGB, filter 0)."""
+    import struct
+    import zlib
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        tag = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+        assert crc == zlib.crc32(tag + body) & 0xFFFFFFFF
+        if tag == b"IHDR":
+            w, h, depth, ctype, _, _, _ = struct.unpack(">IIBBBBB", body)
+            assert (depth, ctype) == (8, 2)
+        elif tag == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 3 * w)
+    assert (raw[:, 0] == 0).all()
+    return raw[:, 1:].reshape(h, w, 3)
+
+
+def test_png_writer_round_trip():
+    import panopaea_amd as pa
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, size=(7, 11, 3), dtype=np.uint8)
+    assert np.array_equal(_png_decode(pa.png_bytes(img)), img)
