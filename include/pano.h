@@ -109,6 +109,11 @@ pano_status pano_derivative_1_dual(pano_ctx, const pano_grid*, void* vertices, c
 /* ---- fluid-step operators (2D and 3D) ----------------------------------- */
 /* b = -div u: hodge_1_dual -> derivative_1_primal -> negate, dec_fluid.rs:68-83. */
 pano_status pano_neg_divergence(pano_ctx, const pano_grid*, void* b, const void* faces);
+/* The same with the reference scene's obstacle (dec_fluid.rs:68-83): faces
+ * whose face_mask byte (device, one per face) is 0 are zeroed after
+ * hodge_1_dual. 2D grids with closed walls. */
+pano_status pano_neg_divergence_masked(pano_ctx, const pano_grid*, void* b, const void* faces,
+                                       const uint8_t* face_mask);
 /* q = A p, the operator closure dec_fluid.rs:100-119 (hodge_2_primal ->
  * derivative_0_dual -> [obstacle mask] -> hodge_1_dual -> derivative_1_primal
  * -> *dt). face_mask: optional device byte per face (0 = closed), 2D only. */

@@ -329,10 +329,16 @@ def advect_mac(dst, src, timestep, vel):
                                           float(timestep), vel.ptr))
 
 
-def neg_divergence(b, faces):
-    """b = -div u (examples/dec_fluid.rs:68-83)."""
+def neg_divergence(b, faces, face_mask=None):
+    """b = -div u (examples/dec_fluid.rs:68-83); with a face mask (from
+    face_mask()), masked faces are zeroed after hodge_1_dual as the
+    reference's obstacle does."""
     g = b.grid
-    _lib.check(_lib.lib().pano_neg_divergence(g._ctx().handle, ctypes.byref(g.desc), b.ptr, faces.ptr))
+    if face_mask is None:
+        _lib.check(_lib.lib().pano_neg_divergence(g._ctx().handle, ctypes.byref(g.desc), b.ptr, faces.ptr))
+    else:
+        _lib.check(_lib.lib().pano_neg_divergence_masked(g._ctx().handle, ctypes.byref(g.desc), b.ptr, faces.ptr,
+                                                         face_mask.ptr))
 
 
 def project(faces, p, dt):

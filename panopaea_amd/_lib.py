@@ -117,6 +117,7 @@ _SIGS = {
     "pano_new_simplex_1": ([c_void_p, _P(GridDesc), _P(c_void_p)], c_i32),
     "pano_new_simplex_2": ([c_void_p, _P(GridDesc), _P(c_void_p)], c_i32),
     "pano_neg_divergence": ([c_void_p, _P(GridDesc), c_void_p, c_void_p], c_i32),
+    "pano_neg_divergence_masked": ([c_void_p, _P(GridDesc), c_void_p, c_void_p, c_void_p], c_i32),
     "pano_poisson_apply": ([c_void_p, _P(GridDesc), c_f64, c_void_p, c_void_p, c_void_p], c_i32),
     "pano_project": ([c_void_p, _P(GridDesc), c_f64, c_void_p, c_void_p], c_i32),
     "pano_advect": ([c_void_p, _P(GridDesc), c_void_p, c_void_p, c_f64, c_void_p], c_i32),

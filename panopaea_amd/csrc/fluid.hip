@@ -121,6 +121,31 @@ __global__ __launch_bounds__(256) void k_poisson_masked2(Dims D, Grid3 G, T* q, 
     q[(int64_t)y * w + x] = r * dt;
 }
 
+// b = -derivative_1_primal(mask(hodge_1_dual(u))), the divergence of the
+// reference's obstacle scene (dec_fluid.rs:68-83): masked faces are zeroed
+// after the hodge star. 2D only; literal operation order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_neg_div_masked2(Dims D, Grid3 G, T* b, const T* faces,
+                                                         const uint8_t* mask) {
+    int x, y, z, L;
+    tile_xyz(G, x, y, z, L);
+    if (x >= D.w || y >= D.h) return;
+    const int h = D.h, w = D.w;
+    const T zero = (T)0;
+    const T* vy = faces;                           // (h+1) x w
+    const T* vx = faces + (int64_t)(h + 1) * w;    // h x (w+1)
+    const uint8_t* my = mask;
+    const uint8_t* mx = mask + (int64_t)(h + 1) * w;
+    int64_t it = (int64_t)y * w + x, ib = (int64_t)(y + 1) * w + x;
+    int64_t il = (int64_t)y * (w + 1) + x, ir = il + 1;
+    T top = my[it] ? -vy[it] : zero;     // hodge_1_dual negates the vy part
+    T bottom = my[ib] ? -vy[ib] : zero;
+    T left = mx[il] ? vx[il] : zero;
+    T right = mx[ir] ? vx[ir] : zero;
+    T d = -bottom + top - left + right;  // derivative_1_primal, grid.rs:295-305
+    b[(int64_t)y * w + x] = -d;
+}
+
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt) {
     constexpr int NZ = kPlaneRun<DIM>;
@@ -240,6 +265,14 @@ pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces) {
 }
 
 template <typename T>
+static pano_status launch_neg_div_masked(Ctx* c, const Dims& D, T* b, const T* faces, const uint8_t* mask) {
+    if (D.d != 1 || D.bc != 0) return fail(PANO_ERR_BAD_SHAPE, "face_mask needs a 2D grid with closed walls");
+    Grid3 G = grid3(D.w, D.h, 1);
+    LAUNCH3((k_neg_div_masked2<T>), G, D, G, b, faces, mask);
+    return PANO_OK;
+}
+
+template <typename T>
 pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t* mask, T* q,
                                  const T* p) {
     Grid3 G = grid3(D.w, D.h, D.d);
@@ -318,6 +351,17 @@ pano_status pano_neg_divergence(pano_ctx ctx, const pano_grid* g, void* b, const
     Dims D = dims_of(g);
     return g->dtype == PANO_F64 ? launch_neg_div<double>(c, D, (double*)b, (const double*)faces)
                                 : launch_neg_div<float>(c, D, (float*)b, (const float*)faces);
+}
+
+pano_status pano_neg_divergence_masked(pano_ctx ctx, const pano_grid* g, void* b, const void* faces,
+                                       const uint8_t* face_mask) {
+    Ctx* c;
+    PANO_CHECK(pre(ctx, g, &c));
+    if (!b || !faces || !face_mask || b == faces) return fail(PANO_ERR_INVALID, "null or aliased field");
+    Dims D = dims_of(g);
+    if (g->dtype == PANO_F64)
+        return launch_neg_div_masked<double>(c, D, (double*)b, (const double*)faces, face_mask);
+    return launch_neg_div_masked<float>(c, D, (float*)b, (const float*)faces, face_mask);
 }
 
 pano_status pano_poisson_apply(pano_ctx ctx, const pano_grid* g, double dt, const uint8_t* mask,
