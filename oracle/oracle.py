@@ -308,3 +308,96 @@ def field_rgb8(h, w, plane, lower=-2.0, upper=2.0):
     _fn("or_field_rgb8", plane.dtype)(c_i64(h), c_i64(w), _p(plane), ctypes.c_double(lower),
                                       ctypes.c_double(upper), out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+# ---- assembled operators: the reference's CSR matrix --------------------------------------
+
+class SparseMatrix:
+    """Compressed sparse rows, panopaea/src/sparse.rs:41-129: insert keeps
+    each row sorted by column (binary search; a repeated (row, col)
+    overwrites), mul_vec folds each row from zero in column order
+    (acc + a_ij * x_j, rounded in the matrix's dtype)."""
+
+    def __init__(self, dim, dtype=np.float64):
+        self.dim = dim
+        self.dtype = np.dtype(dtype)
+        self.data, self.col = [], []
+        self.row = [0] * (dim[0] + 1)
+
+    def insert(self, i, j, v):  # sparse.rs:67-86
+        import bisect
+        start, end = self.row[i], self.row[i + 1]
+        pos = bisect.bisect_left(self.col, j, start, end)
+        if pos < end and self.col[pos] == j:
+            self.data[pos] = self.dtype.type(v)
+            return
+        self.data.insert(pos, self.dtype.type(v))
+        self.col.insert(pos, j)
+        for k in range(i + 1, len(self.row)):
+            self.row[k] += 1
+
+    def mul_vec(self, x):  # sparse.rs:113-121
+        x = np.asarray(x, self.dtype)
+        out = np.zeros(self.dim[0], self.dtype)
+        zero = self.dtype.type(0)
+        for i in range(self.dim[0]):
+            acc = zero
+            for k in range(self.row[i], self.row[i + 1]):
+                acc = acc + self.data[k] * x[self.col[k]]
+            out[i] = acc
+        return out
+
+    def dense(self):
+        m = np.zeros(self.dim, self.dtype)
+        for i in range(self.dim[0]):
+            for k in range(self.row[i], self.row[i + 1]):
+                m[i, self.col[k]] = self.data[k]
+        return m
+
+
+def d0_primal_matrix(h, w, dtype=np.float64):
+    """The incidence matrix of derivative_0_primal, assembled as the
+    reference's commented derivative_0_primal_matrix (dec/grid.rs:373-407):
+    horizontal edges (y in 0..h+1, x in 0..w) v[y,x+1] - v[y,x], then
+    vertical edges (y in 0..h, x in 0..w+1) v[y+1,x] - v[y,x]."""
+    m = SparseMatrix(((h + 1) * w + h * (w + 1), (h + 1) * (w + 1)), dtype)
+    idx = 0
+    for y in range(h + 1):
+        for x in range(w):
+            v = y * (w + 1) + x
+            m.insert(idx, v, -1)
+            m.insert(idx, v + 1, 1)
+            idx += 1
+    for y in range(h):
+        for x in range(w + 1):
+            v = y * (w + 1) + x
+            m.insert(idx, v, -1)
+            m.insert(idx, v + w + 1, 1)
+            idx += 1
+    return m
+
+
+def poisson_matrix(P, dtype=np.float64):
+    """A = dt (n_c I - adjacency) as a CSR matrix for P's grid (closed walls
+    drop a neighbour, open sides keep it in n_c with p = 0 outside)."""
+    d, h, w = P.d, P.h, P.w
+    n = d * h * w
+    m = SparseMatrix((n, n), dtype)
+    dt = dtype(P.dt)
+    for z in range(d):
+        for y in range(h):
+            for x in range(w):
+                c = (z * h + y) * w + x
+                nc = 0
+                sides = [(x > 0, P.bc[0], c - 1), (x < w - 1, P.bc[1], c + 1), (y > 0, P.bc[2], c - w),
+                         (y < h - 1, P.bc[3], c + w)]
+                if d > 1:
+                    sides += [(z > 0, P.bc[4], c - h * w), (z < d - 1, P.bc[5], c + h * w)]
+                for inside, open_side, j in sides:
+                    if inside:
+                        m.insert(c, j, -dt)
+                        nc += 1
+                    elif open_side:
+                        nc += 1
+                m.insert(c, c, dt * nc)
+    return m

@@ -537,3 +537,40 @@ def test_png_writer_round_trip():
     rng = np.random.default_rng(3)
     img = rng.integers(0, 256, size=(7, 11, 3), dtype=np.uint8)
     assert np.array_equal(_png_decode(pa.png_bytes(img)), img)
+
+
+# ---- assembled operators (sparse.rs CSR) against the matrix-free operators ------------------
+
+def test_csr_insert_and_mul_vec_semantics():
+    m = o.SparseMatrix((3, 4))
+    m.insert(1, 3, 2.0)
+    m.insert(1, 0, 5.0)
+    m.insert(0, 2, 1.0)
+    m.insert(1, 3, 7.0)  # overwrite
+    assert m.row == [0, 1, 3, 3] and m.col == [2, 0, 3] and m.data == [1.0, 5.0, 7.0]
+    x = np.array([1.0, 2.0, 3.0, 4.0])
+    assert list(m.mul_vec(x)) == [3.0, 33.0, 0.0]
+    assert np.array_equal(m.dense() @ x, m.mul_vec(x))
+
+
+def test_d0_primal_matrix_equals_the_operator():
+    h, w = 5, 7
+    rng = np.random.default_rng(1)
+    for dt in (np.float32, np.float64):
+        v = rng.standard_normal((h + 1) * (w + 1)).astype(dt)
+        e = np.zeros((h + 1) * w + h * (w + 1), dt)
+        o.dec2("d0_primal", h, w, e, v)
+        assert np.array_equal(o.d0_primal_matrix(h, w, dt).mul_vec(v), e)  # -v0 + v1 == v1 - v0 exactly
+
+
+@pytest.mark.parametrize("dims,bc", [((6, 5), (0,) * 6), ((6, 5), (0, 1, 0, 0, 0, 0)), ((3, 4, 5), (0, 0, 0, 0, 0, 1))])
+def test_poisson_matrix_equals_matrix_free(dims, bc):
+    P = o.params(dims, bc=bc, dt=0.25)
+    A = o.poisson_matrix(P)
+    n = int(np.prod(dims))
+    Ad = A.dense()
+    assert np.array_equal(Ad, Ad.T)
+    p = np.random.default_rng(2).integers(-50, 50, n).astype(np.float64)  # integer data: exact in any order
+    assert np.array_equal(A.mul_vec(p), o.poisson(P, p))
+    if not any(bc):
+        assert np.abs(Ad.sum(axis=1)).max() == 0.0
