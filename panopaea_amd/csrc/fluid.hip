@@ -21,12 +21,11 @@ namespace pano {
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_cell(Dims D, Grid3 G, T* dst, const T* q, T dt,
                                                      Faces<const T> v, Sphere S) {
-    constexpr int NZ = kPlaneRun<DIM>;
+    constexpr int NZ = kAdvectRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
     T out[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) out[k] = advect_cell_value<T, DIM>(D, q, S, dt, v, PLANE_Z(k, D.d), y, x);
+    advect_cell_run<T, DIM, NZ>(D, q, S, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
         if (z0 + k < D.d) dst[cell_index(D, z0 + k, y, x)] = out[k];
@@ -35,12 +34,11 @@ __global__ __launch_bounds__(256) void k_advect_cell(Dims D, Grid3 G, T* dst, co
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_fx(Dims D, Grid3 G, T* dst, const T* qx, T dt,
                                                    Faces<const T> v) {
-    constexpr int NZ = kPlaneRun<DIM>;
+    constexpr int NZ = kAdvectRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x > D.w || y >= D.h) return;
     T out[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) out[k] = advect_face_x<T, DIM>(D, qx, dt, v, PLANE_Z(k, D.d), y, x);
+    advect_face_run<T, DIM, 0, NZ>(D, qx, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
         if (z0 + k < D.d) dst[fx_idx(D, z0 + k, y, x)] = out[k];
@@ -49,12 +47,11 @@ __global__ __launch_bounds__(256) void k_advect_fx(Dims D, Grid3 G, T* dst, cons
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void k_advect_fy(Dims D, Grid3 G, T* dst, const T* qy, T dt,
                                                    Faces<const T> v) {
-    constexpr int NZ = kPlaneRun<DIM>;
+    constexpr int NZ = kAdvectRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y > D.h) return;
     T out[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) out[k] = advect_face_y<T, DIM>(D, qy, dt, v, PLANE_Z(k, D.d), y, x);
+    advect_face_run<T, DIM, 1, NZ>(D, qy, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
         if (z0 + k < D.d) dst[fy_idx(D, z0 + k, y, x)] = out[k];
@@ -64,13 +61,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_advect_fz(Dims D, Grid3 G, T* dst, const T* qz, T dt,
                                                    Faces<const T> v, T buoy_c, const T* rho,
                                                    Sphere S) {
-    constexpr int NZ = kPlaneRun<3>;
+    constexpr int NZ = kAdvectRun<3>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
     T out[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        out[k] = advect_face_z<T>(D, qz, dt, v, PLANE_Z(k, D.d + 1), y, x, buoy_c, rho, S);
+    advect_face_run<T, 3, 2, NZ>(D, qz, dt, v, z0, y, x, out, buoy_c, rho, &S);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
         if (z0 + k <= D.d) dst[fz_idx(D, z0 + k, y, x)] = out[k];
@@ -220,7 +215,7 @@ template <typename T>
 pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
                           const double* src_sphere) {
     Faces<const T> v = split_faces(D, vel);
-    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
+    const int nz = D.d > 1 ? kAdvectRun<3> : 1;
     Grid3 G = grid3_run(D.w, D.h, D.d, nz);
     Sphere S = make_sphere(src_sphere);
     const double cells = (double)D.d * D.h * D.w * sizeof(T);
@@ -241,7 +236,7 @@ pano_status launch_advect_mac(Ctx* c, const Dims& D, T* dst, const T* src, doubl
     Faces<const T> q = split_faces(D, src);
     Faces<T> o = split_faces(D, dst);
     Sphere S = make_sphere(src_sphere);
-    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
+    const int nz = D.d > 1 ? kAdvectRun<3> : 1;
     Grid3 Gx = grid3_run(D.w + 1, D.h, D.d, nz), Gy = grid3_run(D.w, D.h + 1, D.d, nz);
     if (D.d > 1) {
         Grid3 Gz = grid3_run(D.w, D.h, D.d + 1, nz);

@@ -125,32 +125,92 @@ __device__ __forceinline__ void cell_coord(T pp, int n, int& i0, int& i1, T& f) 
     f = p - (T)i;
 }
 
+// Runs of NZ values along z per thread (planes z0 + k, clamped to the last
+// plane for the loads; the caller stores those in range). Each body issues
+// all velocity loads of the run, then forms the departure points, then
+// issues all interpolation loads, then interpolates: the per-value
+// arithmetic is the single-value formula's.
+
+// Cell-centred values (density), dec_fluid.rs:173-211, with the source
+// sphere's reset applied to the interpolated neighbours.
+template <typename T, int DIM, int NZ>
+__device__ __forceinline__ void advect_cell_run(const Dims& D, const T* q, const Sphere& S, T dt,
+                                                const Faces<const T>& v, int z0, int y, int x, T* out) {
+    const T half = (T)0.5, two = (T)2.0, ndt = -dt;
+    T vx[NZ][2], vy[NZ][2], vz[NZ][2];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = imin(z0 + k, D.d - 1);
+        vx[k][0] = v.x[fx_idx(D, z, y, x)];
+        vx[k][1] = v.x[fx_idx(D, z, y, x + 1)];
+        vy[k][0] = v.y[fy_idx(D, z, y, x)];
+        vy[k][1] = v.y[fy_idx(D, z, y + 1, x)];
+        if constexpr (DIM == 3) {
+            vz[k][0] = v.z[fz_idx(D, z, y, x)];
+            vz[k][1] = v.z[fz_idx(D, z + 1, y, x)];
+        }
+    }
+    constexpr int NG = DIM == 3 ? 8 : 4;
+    int gx[NZ][2], gy[NZ][2], gz[NZ][2];
+    T sw[NZ], tw[NZ], uw[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = imin(z0 + k, D.d - 1);
+        T vcx = (vx[k][0] + vx[k][1]) / two;
+        T vcy = (vy[k][0] + vy[k][1]) / two;
+        T ppx = ((T)x + half) + ndt * vcx;
+        T ppy = ((T)y + half) + ndt * vcy;
+        cell_coord(ppx, D.w, gx[k][0], gx[k][1], sw[k]);
+        cell_coord(ppy, D.h, gy[k][0], gy[k][1], tw[k]);
+        if constexpr (DIM == 3) {
+            T vcz = (vz[k][0] + vz[k][1]) / two;
+            T ppz = ((T)z + half) + ndt * vcz;
+            cell_coord(ppz, D.d, gz[k][0], gz[k][1], uw[k]);
+        } else {
+            gz[k][0] = gz[k][1] = 0;
+            uw[k] = (T)0;
+        }
+    }
+    T g[NZ][NG];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k)
+#pragma unroll
+        for (int j = 0; j < NG; ++j)
+            g[k][j] = q[cell_index(D, gz[k][j >> 2], gy[k][(j >> 1) & 1], gx[k][j & 1])];
+    if (S.on) {  // the source reset of the gathered cells (rho_eff): squared
+                 // distances per axis, summed as in_sphere sums them
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+            double qx[2], qy[2], qz[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                double dx = (double)gx[k][a] + 0.5 - S.cx, dy = (double)gy[k][a] + 0.5 - S.cy;
+                double dz = D.d == 1 ? 0.0 : (double)gz[k][a] + 0.5 - S.cz;
+                qx[a] = dx * dx;
+                qy[a] = dy * dy;
+                qz[a] = dz * dz;
+            }
+#pragma unroll
+            for (int j = 0; j < NG; ++j)
+                if (qz[j >> 2] + qy[(j >> 1) & 1] + qx[j & 1] <= S.r2) g[k][j] = (T)1;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        if constexpr (DIM == 3)
+            out[k] = trilerp(g[k][0], g[k][1], g[k][2], g[k][3], g[k][4], g[k][5], g[k][6], g[k][7], sw[k], tw[k],
+                             uw[k]);
+        else
+            out[k] = bilerp(g[k][0], g[k][1], g[k][2], g[k][3], sw[k], tw[k]);
+    }
+}
+
 template <typename T, int DIM>
 __device__ __forceinline__ T advect_cell_value(const Dims& D, const T* q, const Sphere& S, T dt,
                                                const Faces<const T>& v, int z, int y, int x) {
-    const T half = (T)0.5, two = (T)2.0, ndt = -dt;
-    T vcx = (v.x[fx_idx(D, z, y, x)] + v.x[fx_idx(D, z, y, x + 1)]) / two;
-    T vcy = (v.y[fy_idx(D, z, y, x)] + v.y[fy_idx(D, z, y + 1, x)]) / two;
-    T ppx = ((T)x + half) + ndt * vcx;
-    T ppy = ((T)y + half) + ndt * vcy;
-    int x0, x1, y0, y1;
-    T s, t;
-    cell_coord(ppx, D.w, x0, x1, s);
-    cell_coord(ppy, D.h, y0, y1, t);
-    if constexpr (DIM == 3) {
-        T vcz = (v.z[fz_idx(D, z, y, x)] + v.z[fz_idx(D, z + 1, y, x)]) / two;
-        T ppz = ((T)z + half) + ndt * vcz;
-        int z0, z1;
-        T u;
-        cell_coord(ppz, D.d, z0, z1, u);
-        return trilerp(rho_eff(q, S, D, z0, y0, x0), rho_eff(q, S, D, z0, y0, x1),
-                       rho_eff(q, S, D, z0, y1, x0), rho_eff(q, S, D, z0, y1, x1),
-                       rho_eff(q, S, D, z1, y0, x0), rho_eff(q, S, D, z1, y0, x1),
-                       rho_eff(q, S, D, z1, y1, x0), rho_eff(q, S, D, z1, y1, x1), s, t, u);
-    } else {
-        return bilerp(rho_eff(q, S, D, 0, y0, x0), rho_eff(q, S, D, 0, y0, x1),
-                      rho_eff(q, S, D, 0, y1, x0), rho_eff(q, S, D, 0, y1, x1), s, t);
-    }
+    T o[1];
+    advect_cell_run<T, DIM, 1>(D, q, S, dt, v, z, y, x, o);
+    return o[0];
 }
 
 // ---- advect_mac (dec_fluid.rs:213-291) ----------------------------------------------
@@ -167,101 +227,160 @@ __device__ __forceinline__ void face_coord(T pp, T off, int nmax, int& i0, int& 
     f = tmax(tmin(pp - off - (T)i, (T)1), (T)0);
 }
 
-// X faces at (x, y+1/2, z+1/2).
-template <typename T, int DIM>
-__device__ __forceinline__ T advect_face_x(const Dims& D, const T* qx, T dt, const Faces<const T>& v,
-                                           int z, int y, int x) {
+// One velocity component on its faces (axis 0: x faces at (x, y+1/2, z+1/2),
+// 1: y faces at (x+1/2, y, z+1/2), 2: z faces at (x+1/2, y+1/2, z)); z
+// faces also get the smoke buoyancy buoy_c*(rho[z-1] + rho[z]) on interior
+// and open faces.
+template <typename T, int DIM, int AXIS, int NZ>
+__device__ __forceinline__ void advect_face_run(const Dims& D, const T* qc, T dt, const Faces<const T>& v, int z0,
+                                                int y, int x, T* out, T buoy_c = (T)0, const T* rho = nullptr,
+                                                const Sphere* S = nullptr) {
     const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
-    int xa = imin(x, D.w - 1), xb = x > 0 ? x - 1 : 0;
-    T velx = v.x[fx_idx(D, z, y, x)];
-    T vely = (v.y[fy_idx(D, z, y, xa)] + v.y[fy_idx(D, z, y + 1, xa)] + v.y[fy_idx(D, z, y, xb)] +
-              v.y[fy_idx(D, z, y + 1, xb)]) / four;
-    T ppx = ((T)x + zero) + ndt * velx;
-    T ppy = ((T)y + half) + ndt * vely;
-    int x0, x1, y0, y1;
-    T s, t;
-    face_coord(ppx, zero, D.w, x0, x1, s);
-    face_coord(ppy, half, D.h - 1, y0, y1, t);
-    if constexpr (DIM == 3) {
-        T velz = (v.z[fz_idx(D, z, y, xa)] + v.z[fz_idx(D, z + 1, y, xa)] + v.z[fz_idx(D, z, y, xb)] +
-                  v.z[fz_idx(D, z + 1, y, xb)]) / four;
-        T ppz = ((T)z + half) + ndt * velz;
-        int z0, z1;
-        T u;
-        face_coord(ppz, half, D.d - 1, z0, z1, u);
-        return trilerp(qx[fx_idx(D, z0, y0, x0)], qx[fx_idx(D, z0, y0, x1)], qx[fx_idx(D, z0, y1, x0)],
-                       qx[fx_idx(D, z0, y1, x1)], qx[fx_idx(D, z1, y0, x0)], qx[fx_idx(D, z1, y0, x1)],
-                       qx[fx_idx(D, z1, y1, x0)], qx[fx_idx(D, z1, y1, x1)], s, t, u);
-    } else {
-        return bilerp(qx[fx_idx(D, 0, y0, x0)], qx[fx_idx(D, 0, y0, x1)], qx[fx_idx(D, 0, y1, x0)],
-                      qx[fx_idx(D, 0, y1, x1)], s, t);
-    }
-}
-
-// Y faces at (x+1/2, y, z+1/2).
-template <typename T, int DIM>
-__device__ __forceinline__ T advect_face_y(const Dims& D, const T* qy, T dt, const Faces<const T>& v,
-                                           int z, int y, int x) {
-    const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
-    int ya = imin(y, D.h - 1), yb = y > 0 ? y - 1 : 0;
-    T velx = (v.x[fx_idx(D, z, ya, x)] + v.x[fx_idx(D, z, ya, x + 1)] + v.x[fx_idx(D, z, yb, x)] +
-              v.x[fx_idx(D, z, yb, x + 1)]) / four;
-    T vely = v.y[fy_idx(D, z, y, x)];
-    T ppx = ((T)x + half) + ndt * velx;
-    T ppy = ((T)y + zero) + ndt * vely;
-    int x0, x1, y0, y1;
-    T s, t;
-    face_coord(ppx, half, D.w - 1, x0, x1, s);
-    face_coord(ppy, zero, D.h, y0, y1, t);
-    if constexpr (DIM == 3) {
-        T velz = (v.z[fz_idx(D, z, ya, x)] + v.z[fz_idx(D, z + 1, ya, x)] + v.z[fz_idx(D, z, yb, x)] +
-                  v.z[fz_idx(D, z + 1, yb, x)]) / four;
-        T ppz = ((T)z + half) + ndt * velz;
-        int z0, z1;
-        T u;
-        face_coord(ppz, half, D.d - 1, z0, z1, u);
-        return trilerp(qy[fy_idx(D, z0, y0, x0)], qy[fy_idx(D, z0, y0, x1)], qy[fy_idx(D, z0, y1, x0)],
-                       qy[fy_idx(D, z0, y1, x1)], qy[fy_idx(D, z1, y0, x0)], qy[fy_idx(D, z1, y0, x1)],
-                       qy[fy_idx(D, z1, y1, x0)], qy[fy_idx(D, z1, y1, x1)], s, t, u);
-    } else {
-        return bilerp(qy[fy_idx(D, 0, y0, x0)], qy[fy_idx(D, 0, y0, x1)], qy[fy_idx(D, 0, y1, x0)],
-                      qy[fy_idx(D, 0, y1, x1)], s, t);
-    }
-}
-
-// Z faces at (x+1/2, y+1/2, z) (3D only), plus the smoke buoyancy
-// buoy_c*(rho[z-1] + rho[z]) on interior and open faces.
-template <typename T>
-__device__ __forceinline__ T advect_face_z(const Dims& D, const T* qz, T dt, const Faces<const T>& v,
-                                           int z, int y, int x, T buoy_c, const T* rho,
-                                           const Sphere& S) {
-    const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
-    int za = imin(z, D.d - 1), zb = z > 0 ? z - 1 : 0;
-    T velx = (v.x[fx_idx(D, za, y, x)] + v.x[fx_idx(D, za, y, x + 1)] + v.x[fx_idx(D, zb, y, x)] +
-              v.x[fx_idx(D, zb, y, x + 1)]) / four;
-    T vely = (v.y[fy_idx(D, za, y, x)] + v.y[fy_idx(D, za, y + 1, x)] + v.y[fy_idx(D, zb, y, x)] +
-              v.y[fy_idx(D, zb, y + 1, x)]) / four;
-    T velz = v.z[fz_idx(D, z, y, x)];
-    T ppx = ((T)x + half) + ndt * velx;
-    T ppy = ((T)y + half) + ndt * vely;
-    T ppz = ((T)z + zero) + ndt * velz;
-    int x0, x1, y0, y1, z0, z1;
-    T s, t, u;
-    face_coord(ppx, half, D.w - 1, x0, x1, s);
-    face_coord(ppy, half, D.h - 1, y0, y1, t);
-    face_coord(ppz, zero, D.d, z0, z1, u);
-    T val = trilerp(qz[fz_idx(D, z0, y0, x0)], qz[fz_idx(D, z0, y0, x1)], qz[fz_idx(D, z0, y1, x0)],
-                    qz[fz_idx(D, z0, y1, x1)], qz[fz_idx(D, z1, y0, x0)], qz[fz_idx(D, z1, y0, x1)],
-                    qz[fz_idx(D, z1, y1, x0)], qz[fz_idx(D, z1, y1, x1)], s, t, u);
-    if (buoy_c != zero && rho) {
-        bool open_face = (z > 0 && z < D.d) || (z == 0 && (D.bc & 16)) || (z == D.d && (D.bc & 32));
-        if (open_face) {
-            T rlo = z > 0 ? rho_eff(rho, S, D, z - 1, y, x) : zero;
-            T rhi = z < D.d ? rho_eff(rho, S, D, z, y, x) : zero;
-            val = val + buoy_c * (rlo + rhi);
+    const int np = AXIS == 2 ? D.d + 1 : D.d;  // planes of this component
+    // own component and the four-face averages of the other components
+    T own[NZ], oa[NZ][4], ob[NZ][4];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = imin(z0 + k, np - 1);
+        if constexpr (AXIS == 0) {
+            int xa = imin(x, D.w - 1), xb = x > 0 ? x - 1 : 0;
+            own[k] = v.x[fx_idx(D, z, y, x)];
+            oa[k][0] = v.y[fy_idx(D, z, y, xa)];
+            oa[k][1] = v.y[fy_idx(D, z, y + 1, xa)];
+            oa[k][2] = v.y[fy_idx(D, z, y, xb)];
+            oa[k][3] = v.y[fy_idx(D, z, y + 1, xb)];
+            if constexpr (DIM == 3) {
+                ob[k][0] = v.z[fz_idx(D, z, y, xa)];
+                ob[k][1] = v.z[fz_idx(D, z + 1, y, xa)];
+                ob[k][2] = v.z[fz_idx(D, z, y, xb)];
+                ob[k][3] = v.z[fz_idx(D, z + 1, y, xb)];
+            }
+        } else if constexpr (AXIS == 1) {
+            int ya = imin(y, D.h - 1), yb = y > 0 ? y - 1 : 0;
+            oa[k][0] = v.x[fx_idx(D, z, ya, x)];
+            oa[k][1] = v.x[fx_idx(D, z, ya, x + 1)];
+            oa[k][2] = v.x[fx_idx(D, z, yb, x)];
+            oa[k][3] = v.x[fx_idx(D, z, yb, x + 1)];
+            own[k] = v.y[fy_idx(D, z, y, x)];
+            if constexpr (DIM == 3) {
+                ob[k][0] = v.z[fz_idx(D, z, ya, x)];
+                ob[k][1] = v.z[fz_idx(D, z + 1, ya, x)];
+                ob[k][2] = v.z[fz_idx(D, z, yb, x)];
+                ob[k][3] = v.z[fz_idx(D, z + 1, yb, x)];
+            }
+        } else {
+            int za = imin(z, D.d - 1), zb = z > 0 ? z - 1 : 0;
+            oa[k][0] = v.x[fx_idx(D, za, y, x)];
+            oa[k][1] = v.x[fx_idx(D, za, y, x + 1)];
+            oa[k][2] = v.x[fx_idx(D, zb, y, x)];
+            oa[k][3] = v.x[fx_idx(D, zb, y, x + 1)];
+            ob[k][0] = v.y[fy_idx(D, za, y, x)];
+            ob[k][1] = v.y[fy_idx(D, za, y + 1, x)];
+            ob[k][2] = v.y[fy_idx(D, zb, y, x)];
+            ob[k][3] = v.y[fy_idx(D, zb, y + 1, x)];
+            own[k] = v.z[fz_idx(D, z, y, x)];
         }
     }
-    return val;
+    constexpr int NG = DIM == 3 ? 8 : 4;
+    int gx[NZ][2], gy[NZ][2], gz[NZ][2];
+    T sw[NZ], tw[NZ], uw[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        int z = imin(z0 + k, np - 1);
+        gz[k][0] = gz[k][1] = 0;
+        uw[k] = (T)0;
+        if constexpr (AXIS == 0) {
+            T vely = (oa[k][0] + oa[k][1] + oa[k][2] + oa[k][3]) / four;
+            T ppx = ((T)x + zero) + ndt * own[k];
+            T ppy = ((T)y + half) + ndt * vely;
+            face_coord(ppx, zero, D.w, gx[k][0], gx[k][1], sw[k]);
+            face_coord(ppy, half, D.h - 1, gy[k][0], gy[k][1], tw[k]);
+            if constexpr (DIM == 3) {
+                T velz = (ob[k][0] + ob[k][1] + ob[k][2] + ob[k][3]) / four;
+                T ppz = ((T)z + half) + ndt * velz;
+                face_coord(ppz, half, D.d - 1, gz[k][0], gz[k][1], uw[k]);
+            }
+        } else if constexpr (AXIS == 1) {
+            T velx = (oa[k][0] + oa[k][1] + oa[k][2] + oa[k][3]) / four;
+            T ppx = ((T)x + half) + ndt * velx;
+            T ppy = ((T)y + zero) + ndt * own[k];
+            face_coord(ppx, half, D.w - 1, gx[k][0], gx[k][1], sw[k]);
+            face_coord(ppy, zero, D.h, gy[k][0], gy[k][1], tw[k]);
+            if constexpr (DIM == 3) {
+                T velz = (ob[k][0] + ob[k][1] + ob[k][2] + ob[k][3]) / four;
+                T ppz = ((T)z + half) + ndt * velz;
+                face_coord(ppz, half, D.d - 1, gz[k][0], gz[k][1], uw[k]);
+            }
+        } else {
+            T velx = (oa[k][0] + oa[k][1] + oa[k][2] + oa[k][3]) / four;
+            T vely = (ob[k][0] + ob[k][1] + ob[k][2] + ob[k][3]) / four;
+            T ppx = ((T)x + half) + ndt * velx;
+            T ppy = ((T)y + half) + ndt * vely;
+            T ppz = ((T)z + zero) + ndt * own[k];
+            face_coord(ppx, half, D.w - 1, gx[k][0], gx[k][1], sw[k]);
+            face_coord(ppy, half, D.h - 1, gy[k][0], gy[k][1], tw[k]);
+            face_coord(ppz, zero, D.d, gz[k][0], gz[k][1], uw[k]);
+        }
+    }
+    T g[NZ][NG];
+    T rlo[NZ], rhi[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+#pragma unroll
+        for (int j = 0; j < NG; ++j) {
+            int zz = gz[k][j >> 2], yy = gy[k][(j >> 1) & 1], xx = gx[k][j & 1];
+            int64_t i = AXIS == 0 ? fx_idx(D, zz, yy, xx) : (AXIS == 1 ? fy_idx(D, zz, yy, xx) : fz_idx(D, zz, yy, xx));
+            g[k][j] = qc[i];
+        }
+        if constexpr (AXIS == 2) {
+            if (buoy_c != zero && rho) {
+                int z = imin(z0 + k, np - 1);
+                int zl = z > 0 ? z - 1 : 0, zh = z < D.d ? z : D.d - 1;
+                T a = rho[cell_index(D, zl, y, x)], b = rho[cell_index(D, zh, y, x)];
+                rlo[k] = z > 0 ? ((S->on && in_sphere(*S, D, zl, y, x)) ? (T)1 : a) : zero;
+                rhi[k] = z < D.d ? ((S->on && in_sphere(*S, D, zh, y, x)) ? (T)1 : b) : zero;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        T val;
+        if constexpr (DIM == 3)
+            val = trilerp(g[k][0], g[k][1], g[k][2], g[k][3], g[k][4], g[k][5], g[k][6], g[k][7], sw[k], tw[k],
+                          uw[k]);
+        else
+            val = bilerp(g[k][0], g[k][1], g[k][2], g[k][3], sw[k], tw[k]);
+        if constexpr (AXIS == 2) {
+            if (buoy_c != zero && rho) {
+                int z = imin(z0 + k, np - 1);
+                bool open_face = (z > 0 && z < D.d) || (z == 0 && (D.bc & 16)) || (z == D.d && (D.bc & 32));
+                if (open_face) val = val + buoy_c * (rlo[k] + rhi[k]);
+            }
+        }
+        out[k] = val;
+    }
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T advect_face_x(const Dims& D, const T* qx, T dt, const Faces<const T>& v, int z, int y,
+                                           int x) {
+    T o[1];
+    advect_face_run<T, DIM, 0, 1>(D, qx, dt, v, z, y, x, o);
+    return o[0];
+}
+template <typename T, int DIM>
+__device__ __forceinline__ T advect_face_y(const Dims& D, const T* qy, T dt, const Faces<const T>& v, int z, int y,
+                                           int x) {
+    T o[1];
+    advect_face_run<T, DIM, 1, 1>(D, qy, dt, v, z, y, x, o);
+    return o[0];
+}
+template <typename T>
+__device__ __forceinline__ T advect_face_z(const Dims& D, const T* qz, T dt, const Faces<const T>& v, int z, int y,
+                                           int x, T buoy_c, const T* rho, const Sphere& S) {
+    T o[1];
+    advect_face_run<T, 3, 2, 1>(D, qz, dt, v, z, y, x, o, buoy_c, rho, &S);
+    return o[0];
 }
 
 // ---- divergence: hodge_1_dual + derivative_1_primal (grid.rs:223-238, 295-305) ----------
@@ -461,6 +580,9 @@ inline Grid3 grid3(int64_t W, int64_t H, int64_t Dz) {
 // kPlaneRun values along z (every load of the run first), then stores them.
 template <int DIM>
 constexpr int kPlaneRun = DIM == 3 ? 4 : 1;
+// runs of the advection kernels (more loads per value: shorter runs)
+template <int DIM>
+constexpr int kAdvectRun = DIM == 3 ? 2 : 1;
 // tiles of a W x H x P launch with runs of nz planes
 inline Grid3 grid3_run(int64_t W, int64_t H, int64_t P, int nz) { return grid3(W, H, (P + nz - 1) / nz); }
 __device__ __forceinline__ void tile_xyz(const Grid3& g, int& x, int& y, int& z, int& L) {
