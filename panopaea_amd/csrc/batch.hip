@@ -299,13 +299,20 @@ __global__ __launch_bounds__(256) void kb_restrict(BTab T, char* A,
                                                    const SolverState* st) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
-    int I, J, K;
-    local_xyz(E.G, lb, I, J, K);
+    int I, J, kb;
+    local_xyz(E.G, lb, I, J, kb);
     if (I >= E.Dc.w || J >= E.Dc.h) return;
     float* rc = fld<float>(A, E.o[0]);
     const float* res = fld<float>(A, E.o[1]);
-    if (E.D.d > 1) restrict_cell<float, 3, LC>(E.D, E.Dc, rc, res, I, J, K);
-    else restrict_cell<float, 2, LC>(E.D, E.Dc, rc, res, I, J, K);
+    if (E.D.d <= 1) {
+        restrict_run<float, 2, LC, 1>(E.D, E.Dc, rc, res, I, J, 0);
+    } else {
+        switch (restrict_run_len(E.Dc)) {
+            case 4: restrict_run<float, 3, LC, 4>(E.D, E.Dc, rc, res, I, J, kb * 4); break;
+            case 2: restrict_run<float, 3, LC, 2>(E.D, E.Dc, rc, res, I, J, kb * 2); break;
+            default: restrict_run<float, 3, LC, 1>(E.D, E.Dc, rc, res, I, J, kb); break;
+        }
+    }
 }
 
 // prolongation from level l+1 over prl_[l]: o[0] = z_l, o[1] = z_{l+1}
@@ -655,7 +662,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 pr.o[1] = g.o_zl[l + 1];
                 prl_[l].h.push_back(pr);
                 BEntry co = r;
-                co.G = grid3t(r.Dc.w, r.Dc.h, r.Dc.d, shapes.nat_x, shapes.nat_y);
+                int nk = restrict_run_len(r.Dc);  // coarse planes per thread
+                co.G = grid3t(r.Dc.w, r.Dc.h, (r.Dc.d + nk - 1) / nk, shapes.nat_x, shapes.nat_y);
                 co.start = sco[l];
                 sco[l] += co.G.nb;
                 co.o[0] = g.o_rl[l + 1];

@@ -73,15 +73,15 @@ __global__ __launch_bounds__(256) void k_mg_residual(Dims D, Grid3 G, T* res, co
     residual_run<T, DIM, NZ>(D, res, zv, r, s, m, y, z0);
 }
 
-// r_c = R res
-template <typename T, int DIM, int LC>
+// r_c = R res, NK coarse planes per thread
+template <typename T, int DIM, int LC, int NK>
 __global__ __launch_bounds__(256) void k_restrict(Dims Df, Dims Dc, Grid3 G, T* rc, const T* res,
                                                   const SolverState* st) {
     if (st->done) return;
-    int I, J, K, L;
-    tile_xyz(G, I, J, K, L);
+    int I, J, kb, L;
+    tile_xyz(G, I, J, kb, L);
     if (I >= Dc.w || J >= Dc.h) return;
-    restrict_cell<T, DIM, LC>(Df, Dc, rc, res, I, J, K);
+    restrict_run<T, DIM, LC, NK>(Df, Dc, rc, res, I, J, kb * NK);
 }
 
 // z_f += P z_c
@@ -471,14 +471,23 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     }
     klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM, NZ>, dim3(G.nb), B, D, G, m->res, zl[l],
             rl[l], s, st);
-    Grid3 Gc = grid3(Dc.w, Dc.h, Dc.d);
     const char* nr = l == 0 ? "restrict_l0" : "";
-    if (l + 1 == Lc)
-        klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, NATURAL>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
-                (const T*)m->res, st);
-    else
-        klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, REDBLACK>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
-                (const T*)m->res, st);
+    const int nk = restrict_run_len(Dc);
+    Grid3 Gc = grid3(Dc.w, Dc.h, (Dc.d + nk - 1) / nk);
+    auto restrict_into = [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        if (nk == 4)
+            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, LC, 4>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                    (const T*)m->res, st);
+        else if (nk == 2)
+            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, LC, 2>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                    (const T*)m->res, st);
+        else
+            klaunch(c, nr, (1 + cfrac) * unit, k_restrict<T, DIM, LC, 1>, dim3(Gc.nb), B, D, Dc, Gc, m->r[l + 1],
+                    (const T*)m->res, st);
+    };
+    if (l + 1 == Lc) restrict_into(std::integral_constant<int, NATURAL>{});
+    else restrict_into(std::integral_constant<int, REDBLACK>{});
 }
 
 // One level of the up leg: prolongation from level l+1, post-smoothing; at

@@ -62,47 +62,60 @@ __device__ __forceinline__ int cidx(const Dims& Dc, int K, int J, int I) {
     }
 }
 
-// r_c(I, J, K) = R res (x, then y, then z restriction of the fine residual in
-// red-black layout).
-template <typename T, int DIM, int LC>
-__device__ __forceinline__ void restrict_cell(const Dims& Df, const Dims& Dc, T* rc, const T* res, int I, int J,
-                                              int K) {
-    const T zero = (T)0;
-    bool hmI = I > 0, hpI = I < Dc.w - 1;
-    bool hmJ = J > 0, hpJ = J < Dc.h - 1;
+// Coarse planes per thread of the restriction into a level of dims Dc.
+__host__ __device__ inline int restrict_run_len(const Dims& Dc) {
+    if (Dc.d <= 1) return 1;
+    int64_t n = (int64_t)Dc.d * Dc.h * Dc.w;
+    return n >= (int64_t(1) << 20) ? 4 : (n >= (int64_t(1) << 17) ? 2 : 1);
+}
+
+// r_c = R res for the coarse cells (K0 + k, J, I), k < NK: the fine residual
+// (red-black layout) restricted along x, then y, then z with r1's weights.
+// Every fine value of the support is loaded first; rows and planes outside
+// the grid are read at a clamped, valid position and r1 discards them (its
+// hm/hp flags). Planes shared by neighbouring coarse cells are loaded and
+// reduced once.
+template <typename T, int DIM, int LC, int NK>
+__device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* rc, const T* res, int I, int J,
+                                             int K0) {
+    constexpr int NP = DIM == 3 ? 2 * NK + 2 : 1;  // fine planes 2K0-1 .. 2K0+2NK (2D: the one plane)
+    const bool hmI = I > 0, hpI = I < Dc.w - 1, hmJ = J > 0, hpJ = J < Dc.h - 1;
     const int wh = Df.w >> 1, hw = Df.h * wh, n2 = Df.d * hw;
-    // fine x = 2I-1, 2I, 2I+1, 2I+2 of row (z, y): colours 1-ce, ce, 1-ce, ce at
-    // compact I-1, I, I, I+1 with ce = (y+z)&1
-    auto tx = [&](int z, int y) -> T {
-        int row = z * hw + y * wh + I;
-        int ce = (y + z) & 1;
-        const T* pe = res + ce * n2 + row;
-        const T* po = res + (1 - ce) * n2 + row;
-        T vm = hmI ? po[-1] : zero;
-        T v0 = pe[0];
-        T v1 = po[0];
-        T vp = hpI ? pe[1] : zero;
-        return r1(vm, v0, v1, vp, hmI, hpI);
-    };
-    auto ty = [&](int z) -> T {
-        T vm = hmJ ? tx(z, 2 * J - 1) : zero;
-        T v0 = tx(z, 2 * J);
-        T v1 = tx(z, 2 * J + 1);
-        T vp = hpJ ? tx(z, 2 * J + 2) : zero;
-        return r1(vm, v0, v1, vp, hmJ, hpJ);
-    };
-    T out;
-    if constexpr (DIM == 3) {
-        bool hmK = K > 0, hpK = K < Dc.d - 1;
-        T vm = hmK ? ty(2 * K - 1) : zero;
-        T v0 = ty(2 * K);
-        T v1 = ty(2 * K + 1);
-        T vp = hpK ? ty(2 * K + 2) : zero;
-        out = r1(vm, v0, v1, vp, hmK, hpK);
-    } else {
-        out = ty(0);
+    T v[NP][4][4];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int z = DIM == 3 ? imin(imax(2 * K0 - 1 + p, 0), Df.d - 1) : 0;
+            int y = imin(imax(2 * J - 1 + q, 0), Df.h - 1);
+            int row = z * hw + y * wh + I;
+            int ce = (y + z) & 1;  // colour of the even fine x = 2I
+            const T* pe = res + ce * n2 + row;
+            const T* po = res + (1 - ce) * n2 + row;
+            v[p][q][0] = po[hmI ? -1 : 0];  // x = 2I-1
+            v[p][q][1] = pe[0];             // x = 2I
+            v[p][q][2] = po[0];             // x = 2I+1
+            v[p][q][3] = pe[hpI ? 1 : 0];   // x = 2I+2
+        }
+    T ty[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        T t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = r1(v[p][q][0], v[p][q][1], v[p][q][2], v[p][q][3], hmI, hpI);
+        ty[p] = r1(t[0], t[1], t[2], t[3], hmJ, hpJ);
     }
-    rc[cidx<LC>(Dc, K, J, I)] = out;
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            int K = K0 + k;
+            if (K < Dc.d)
+                rc[cidx<LC>(Dc, K, J, I)] =
+                    r1(ty[2 * k], ty[2 * k + 1], ty[2 * k + 2], ty[2 * k + 3], K > 0, K < Dc.d - 1);
+        }
+    } else {
+        rc[cidx<LC>(Dc, 0, J, I)] = ty[0];
+    }
 }
 
 // Prolongation weights for fine index i over a coarse axis of n cells.
