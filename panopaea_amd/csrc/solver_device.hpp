@@ -199,34 +199,54 @@ __device__ __forceinline__ double flat_dot(int lb, int64_t n, const T* a, const 
 }
 
 // x += alpha p ; r -= alpha q ; block max|r| (and r.r when want_rr)
+// x, r, p, q are distinct arrays; all loads of the block's items are issued
+// before its stores.
 template <typename T>
 __device__ __forceinline__ void flat_update(int lb, int64_t n, T* x, T* r, const T* p, const T* q, double alpha,
                                             bool want_rr, double* mx, double* rr) {
     int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    T xv[kItems], rv[kItems], pv[kItems], qv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        int64_t j = i < n ? i : n - 1;
+        xv[k] = x[j];
+        rv[k] = r[j];
+        pv[k] = p[j];
+        qv[k] = q[j];
+    }
     double m = 0.0, s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         int64_t i = base + (int64_t)k * kBlock;
         if (i < n) {
-            x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-            T rv = (T)((double)r[i] - alpha * (double)q[i]);
-            r[i] = rv;
-            m = absmax_acc(m, (double)rv);
-            s += (double)rv * (double)rv;
+            x[i] = (T)((double)xv[k] + alpha * (double)pv[k]);
+            T rn = (T)((double)rv[k] - alpha * (double)qv[k]);
+            r[i] = rn;
+            m = absmax_acc(m, (double)rn);
+            s += (double)rn * (double)rn;
         }
     }
     *mx = block_max(m);
     *rr = want_rr ? block_sum(s) : 0.0;
 }
 
-// p = z + beta p
+// p = z + beta p (p and z distinct; loads first)
 template <typename T>
 __device__ __forceinline__ void flat_search(int lb, int64_t n, T* p, const T* z, double beta) {
     int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    T pv[kItems], zv[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+        int64_t j = i < n ? i : n - 1;
+        pv[k] = p[j];
+        zv[k] = z[j];
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) p[i] = (T)((double)zv[k] + beta * (double)pv[k]);
     }
 }
 
