@@ -507,10 +507,10 @@ __device__ __forceinline__ void rb_nbrs(const T* v, const RBRow& R, int k, T* n)
 // Number of non-wall neighbours from the existence bits: an existing
 // neighbour or an open side counts. The diagonal of the operator is dt*nc.
 template <int DIM>
-__device__ __forceinline__ int ncount_bits(int has, int bc) {
+__host__ __device__ __forceinline__ int ncount_bits(int has, int bc) {
     int m = has | bc;
     if constexpr (DIM == 2) m &= 0xF;
-    return __popc(m & 0x3F);
+    return __builtin_popcount((unsigned)(m & 0x3F));
 }
 template <int DIM>
 __device__ __forceinline__ int ncount(const Dims& D, int z, int y, int x) {

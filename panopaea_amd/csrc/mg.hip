@@ -15,9 +15,11 @@
 #include <cmath>
 #include <cstring>
 #include <type_traits>
+#include <utility>
 
 #include "mg.hpp"
 #include "mg_device.hpp"
+#include "mg_tb.hpp"
 
 namespace pano {
 
@@ -95,12 +97,12 @@ __global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* z
 }
 
 // ---- 3D f32 levels: prolongation from LDS ------------------------------------------------
-// z_f += P z_c (3D) on the pair tiles of grid3_rb, with the workgroup's coarse
-// neighbourhood ((kTX+2) x (kTY/2+2) x ((NZ+1)/2+2) cells) copied to LDS
-// first; the arithmetic of prolong_run.
+// z_o = z_f + P z_c (3D; z_o may be z_f) on the pair tiles of grid3_rb, with
+// the workgroup's coarse neighbourhood ((kTX+2) x (kTY/2+2) x ((NZ+1)/2+2)
+// cells) copied to LDS first; the arithmetic of prolong_run.
 template <int LC, int NZ>
-__global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G, float* zf, const float* zc,
-                                                      const SolverState* st) {
+__global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G, float* zo, const float* zf,
+                                                      const float* zc, const SolverState* st) {
     if (st->done) return;
     constexpr int CX = kTX + 2, CY = kTY / 2 + 2, CZ = (NZ + 1) / 2 + 2;
     __shared__ float ct[CZ * CY * CX];
@@ -165,8 +167,55 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
             }
         float v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
         float v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
-        zf[i0[k]] = f0[k] + v0;
-        zf[i1[k]] = f1[k] + v1;
+        zo[i0[k]] = f0[k] + v0;
+        zo[i1[k]] = f1[k] + v1;
+    }
+}
+
+// ---- 3D f32 levels: the pre- and post-smoothing as one pass each ----------------------
+constexpr int kTbTXP = 32, kTbTY = 32, kTbNT = 768;  // tile: 32 x-pairs by 32 rows; threads per workgroup
+// (two workgroups per CU: 79 KiB of LDS and at most 80 VGPRs each)
+constexpr int kTbWant = 512, kTbZmin = 8;
+
+// One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
+// after each, end.
+template <int J, typename TH, int... H>
+__device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
+    th.template begin_step<J>(t);
+    ((th.template stage<H, J>(t), __syncthreads()), ...);
+    th.template end_step<J>(t);
+}
+// Steps t .. t+SLOTS-1 (up to t1), then t advanced past them.
+template <int NS, typename TH, int... J>
+__device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_sequence<int, J...>) {
+    ((t <= t1 ? (tb_step<J>(th, t, std::make_integer_sequence<int, NS>{}), ++t) : t), ...);
+}
+
+// NS half-sweeps from xin into xout (TbThread); with DOT the workgroup's sums
+// of z.r over its tile for the last two half-sweeps go to part[L] and
+// part[nb + L].
+template <int NS, int CF, bool ZERO_X, bool DOT>
+__global__ __launch_bounds__(kTbNT) __attribute__((amdgpu_waves_per_eu(6))) void k_smooth_tb(Dims D, TbGrid g, const float* xin, float* xout, const float* r,
+                                                   float s, RInv<float> ri, double* part, const SolverState* st) {
+    if (st->done) return;
+    using TH = TbThread<NS, kTbTXP, kTbTY, kTbNT, CF, ZERO_X, DOT>;
+    __shared__ float lds[TH::S::LDS_FLOATS];
+    const int L = xcd_remap(blockIdx.x, g.nb);
+    const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
+    TH th;
+    th.init(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri, threadIdx.x);
+    th.prologue();
+    __syncthreads();
+    const int t1 = th.last_step();
+    int t = th.first_step();
+    while (t <= t1) tb_round<NS>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
+    if constexpr (DOT) {
+        const double a0 = block_sum(th.acc[0]);
+        const double a1 = block_sum(th.acc[1]);
+        if (threadIdx.x == 0) {
+            part[L] = a0;
+            part[g.nb + L] = a1;
+        }
     }
 }
 
@@ -298,6 +347,8 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
     PANO_CHECK(mg_check(D, levels));
     m->levels = levels;
     m->nu = nu;
+    const char* fused = getenv("PANO_MG_FUSED");
+    m->tb = !(fused && fused[0] == '0');
     double s = dt;
     for (int l = 0; l < levels; ++l) {
         Dims Dl = D;
@@ -452,9 +503,21 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     Grid3 G = grid3_rb(D);
     const double unit = (double)D.d * D.h * D.w * sizeof(T);
     const char* nh = l == 0 ? "smooth_l0" : "";
+    // 3D f32, nu = 2: the four half-sweeps from z = 0 in one pass (z is only
+    // written, so the pass may write it in place)
+    bool tb_done = false;
+    if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+        if (m->tb && m->nu == 2 && D.d >= 2) {
+            const TbGrid g = tb_grid<kTbTXP, kTbTY>(D, kTbWant, kTbZmin);
+            klaunch(c, l == 0 ? "smooth_pre_l0" : "", 2 * unit, k_smooth_tb<4, 0, true, false>, dim3(g.nb),
+                    dim3(kTbNT), D, g, (const float*)zl[l], zl[l], rl[l], s, ri, (double*)nullptr, st);
+            tb_done = true;
+        }
+    }
     // first sweep from z = 0: on bandwidth-bound levels red cells then the
     // black half-sweep (two light passes), elsewhere one fused launch
-    if (NZ == 4) {
+    if (tb_done) {
+    } else if (NZ == 4) {
         klaunch(c, l == 0 ? "red_init_l0" : "", unit, k_red_init<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s,
                 ri, st);
         klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
@@ -463,7 +526,7 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
         klaunch(c, l == 0 ? "smooth_first_l0" : "", 2 * unit, k_smooth_first<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l],
                 rl[l], s, ri, st);
     }
-    for (int k = 1; k < m->nu; ++k) {
+    for (int k = 1; k < m->nu && !tb_done; ++k) {
         klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 0,
                 (double*)nullptr, st);
         klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
@@ -507,12 +570,31 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
     const char* nh = l == 0 ? "smooth_l0" : "";
     const char* np = l == 0 ? "prolong_l0" : "";
     if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+        // 3D f32, nu = 2: prolongation into the residual scratch (free on the
+        // up leg), then the four post-smoothing half-sweeps from there back
+        // into z in one pass (out of place: workgroups read their halo from
+        // the input while others store)
+        const TbGrid g = tb_grid<kTbTXP, kTbTY>(D, kTbWant, kTbZmin);
+        const bool tb = m->tb && m->nu == 2 && D.d >= 2 && g.nb <= G.nb;
+        float* zo = tb ? (float*)m->res : zl[l];
         if (l + 1 == Lc)
-            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
-                    (const T*)m->z[l + 1], st);
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zo,
+                    (const T*)zl[l], (const T*)m->z[l + 1], st);
         else
-            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
-                    (const T*)m->z[l + 1], st);
+            klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zo,
+                    (const T*)zl[l], (const T*)m->z[l + 1], st);
+        if (tb) {
+            const char* npost = l == 0 ? "smooth_post_l0" : "";
+            if (l == 0 && dot_part) {
+                klaunch(c, npost, 3 * unit, k_smooth_tb<4, 1, false, true>, dim3(g.nb), dim3(kTbNT), D, g,
+                        (const float*)zo, zl[l], rl[l], s, ri, dot_part, st);
+                *dot_np = 2 * g.nb;
+            } else {
+                klaunch(c, npost, 3 * unit, k_smooth_tb<4, 1, false, false>, dim3(g.nb), dim3(kTbNT), D, g,
+                        (const float*)zo, zl[l], rl[l], s, ri, (double*)nullptr, st);
+            }
+            return;
+        }
     } else if (l + 1 == Lc) {
         klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
                 (const T*)m->z[l + 1], st);

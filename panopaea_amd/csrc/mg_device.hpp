@@ -27,7 +27,7 @@ inline RInv<T> make_rinv(T s) {
 
 // sum of the six neighbours in the fixed order ((x- + x+) + y-) + y+ (+ z-) + z+
 template <typename T, int DIM>
-__device__ __forceinline__ T nsum(const T* n) {
+__host__ __device__ __forceinline__ T nsum(const T* n) {
     T s = ((n[0] + n[1]) + n[2]) + n[3];
     if constexpr (DIM == 3) s = (s + n[4]) + n[5];
     return s;
@@ -35,7 +35,7 @@ __device__ __forceinline__ T nsum(const T* n) {
 
 // Smoothed value z = (r + s*sum) * rinv[nc].
 template <typename T>
-__device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RInv<T>& ri) {
+__host__ __device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RInv<T>& ri) {
     return nc ? (r + s * sum) * ri.v[nc] : (T)0;
 }
 

@@ -1,0 +1,300 @@
+// Temporally blocked red-black Gauss-Seidel on 3D f32 levels: NS consecutive
+// half-sweeps (colours CF, CF^1, CF, ...) in one pass over the field.
+//
+// A workgroup owns a tile of TXP x-pairs by TY rows and a chunk of planes
+// [zs, ze). It streams through the planes: at step t half-sweep h updates
+// plane t-h, so half-sweep h reads plane t-h-1 after half-sweep h-1 finished
+// it (one step earlier) and plane t-h+1 before half-sweep h+1 reaches it
+// (half-sweeps run in order h = 0..NS-1 within a step). In-plane, half-sweep h
+// covers the tile grown by NS-1-h cells (a halo of NS cells is loaded), and in
+// z the chunk grown by NS-1-h planes, so the tile's cells end with the values
+// of NS full-grid half-sweeps. Halo cells are recomputed by each neighbour
+// with the same arithmetic (gs_value of the same six neighbour values in the
+// same order), so every cell gets the bits of the unfused half-sweeps.
+//
+// LDS holds SLOTS planes of the tile + halo in the red-black compacted layout
+// of the field (plane slot, colour, row, pair); cells outside the grid hold 0,
+// which is the value a missing neighbour contributes to the sum. Plane z sits
+// in slot (z - t0 + 1) mod SLOTS; steps run in rounds of SLOTS, so every LDS
+// offset is a compile-time constant. b stays in registers: a ring of the NS
+// planes in flight, both colours.
+//
+// Thread positions are split by row parity (rows 2i+q, q = 0, 1), so the x
+// parity of the cell a half-sweep updates is the same across a position slot:
+// the activity test and the reciprocal 1/(s*nc) come from per-position
+// tables selected by that parity.
+//
+// Every phase touches only the thread's own positions plus LDS values written
+// in an earlier phase; the kernel puts a barrier after each half-sweep:
+// prologue | begin + half-sweep 0 | 1 | ... | NS-1 | end + begin of the next
+// step + half-sweep 0 | ...
+#pragma once
+
+#include "mg_device.hpp"
+
+namespace pano {
+
+template <int NS, int TXP, int TY, int NTHREADS>
+struct TbShape {
+    static_assert(NS % 2 == 0 && TY % 2 == 0, "even half-sweep count and tile height");
+    static constexpr int HP = NS / 2;                // halo pairs in x (2*HP = NS cells)
+    static constexpr int PW = TXP + 2 * HP;          // pairs per LDS row
+    static constexpr int PH = TY + 2 * NS;           // LDS rows
+    static constexpr int PLANE = PW * PH;            // cells of one colour of one plane
+    static constexpr int SLOTS = NS + 3;             // resident planes t-NS .. t+2 at step t
+    static constexpr int NT = NTHREADS;              // threads per workgroup
+    static constexpr int CPOS = (PH / 2) * PW;       // positions of one row parity
+    static constexpr int KC = (CPOS + NT - 1) / NT;  // position slots per thread and parity
+    static constexpr int KP = 2 * KC;
+    static constexpr int LDS_FLOATS = SLOTS * 2 * PLANE;
+};
+
+// Tiles and plane chunks of one launch: chunks of zc planes (zc >= zmin,
+// halved while the launch has fewer than `want` workgroups).
+struct TbGrid {
+    int gx, gy, zc, nch, nb;
+};
+template <int TXP, int TY>
+inline TbGrid tb_grid(const Dims& D, int want, int zmin) {
+    TbGrid g;
+    g.gx = ((D.w >> 1) + TXP - 1) / TXP;
+    g.gy = (D.h + TY - 1) / TY;
+    int tiles = g.gx * g.gy, zc = D.d;
+    while (zc > zmin && (int64_t)tiles * ((D.d + zc - 1) / zc) < want) zc = (zc + 1) / 2;
+    if (zc < zmin) zc = zmin < D.d ? zmin : D.d;
+    g.zc = zc;
+    g.nch = (D.d + zc - 1) / zc;
+    g.nb = tiles * g.nch;
+    return g;
+}
+
+// A register value the optimiser may not trace back to its memory (keeps a
+// select between two table entries a select of values, not a load from a
+// selected address).
+__device__ __forceinline__ float tb_opaque(float v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__host__ inline float tb_opaque(float v) { return v; }
+
+// Per-thread state and phases of one workgroup. xin: the field before the
+// half-sweeps (unused with ZERO_X: the sweeps start from 0); xout: the result
+// (may equal xin only with ZERO_X: other workgroups still read xin's halo
+// while this one stores). r: right-hand side. With DOT, acc[0] and acc[1]
+// collect sum(v*r) over the tile's cells of the last two half-sweeps.
+// Needs D.d >= 2 (every cell then has a z neighbour, so nc >= 1).
+template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT>
+struct TbThread {
+    using S = TbShape<NS, TXP, TY, NTHREADS>;
+    static constexpr int PW = S::PW, PH = S::PH, PLANE = S::PLANE, SLOTS = S::SLOTS, NT = S::NT, KC = S::KC,
+                         KP = S::KP, HP = S::HP, CPOS = S::CPOS;
+
+    float* lds;
+    Dims D;
+    const float* xin;
+    float* xout;
+    const float* r;
+    float s;
+    int tid, zs, ze, t0, hw, n2;
+    int e_[KP];              // LDS index of the position (row * PW + pair)
+    unsigned goff[KP];       // in-plane element offset y*wh + m (0 outside the grid)
+    int act[KP];             // bit h + 4p: half-sweep h updates the cell when its x parity is p
+    bool ind[KP], tile[KP];  // inside the grid; inside the tile
+    float rin[KP][2], rbd[KP][2];  // 1/(s*nc) for x parity p: interior plane / one z side missing
+    float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
+    bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
+    double acc[2];
+
+    static constexpr int mod(int a) { return ((a % SLOTS) + SLOTS) % SLOTS; }
+
+    __host__ __device__ __forceinline__ void init(float* lds_, const Dims& D_, int bx, int by, int bz, int zc,
+                                                  const float* xin_, float* xout_, const float* r_, float s_,
+                                                  const RInv<float>& ri, int tid_) {
+        lds = lds_;
+        D = D_;
+        xin = xin_;
+        xout = xout_;
+        r = r_;
+        s = s_;
+        tid = tid_;
+        const int wh = D.w >> 1;
+        hw = D.h * wh;
+        n2 = D.d * hw;
+        const int m0 = bx * TXP, y0 = by * TY;
+        zs = bz * zc;
+        ze = zs + zc < D.d ? zs + zc : D.d;
+        t0 = zs - NS + 1 > 0 ? zs - NS + 1 : 0;
+        acc[0] = acc[1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int q = k / KC, f = tid + (k % KC) * NT;
+            const bool valid = f < CPOS;
+            const int ff = valid ? f : 0;
+            const int row = 2 * (ff / PW) + q, pr = ff % PW;
+            const int y = y0 - NS + row, m = m0 - HP + pr;
+            e_[k] = row * PW + pr;
+            ind[k] = valid && y >= 0 && y < D.h && m >= 0 && m < wh;
+            tile[k] = ind[k] && row >= NS && row < NS + TY && pr >= HP && pr < HP + TXP;
+            goff[k] = ind[k] ? (unsigned)(y * wh + m) : 0u;
+            int a = 0;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int xl = 2 * pr + p, x = 2 * m + p;
+                for (int h = 0; h < NS; ++h)
+                    if (ind[k] && row >= h + 1 && row < PH - 1 - h && xl >= h + 1 && xl < 2 * PW - 1 - h)
+                        a |= 1 << (h + 4 * p);
+                const int hxy = (x > 0) | ((x < D.w - 1) << 1) | ((y > 0) << 2) | ((y < D.h - 1) << 3);
+                const int nxy = __builtin_popcount((unsigned)((hxy | D.bc) & 0xF));
+                rin[k][p] = ri.v[nxy + 2];
+                rbd[k][p] = ri.v[nxy + 1];
+            }
+            act[k] = a;
+        }
+    }
+
+    // x of plane z into xpf, b of plane z into v. The loads are unconditional
+    // (positions outside the grid and unused planes read a valid clamped
+    // address) and the values that must read as 0 are zeroed where they are
+    // consumed (fill, end_step): straight-line loads let each consumer wait
+    // for its own loads only, not for prefetches issued after them.
+    __host__ __device__ __forceinline__ const float* plane_ptr(const float* f, int z) const {
+        return f + (z < 0 ? 0 : (z >= D.d ? D.d - 1 : z)) * hw;
+    }
+    __host__ __device__ __forceinline__ void load_x(int z) {
+        xok = !ZERO_X && z >= 0 && z < D.d && z >= zs - NS && z < ze + NS;
+        if (ZERO_X) return;
+        const float* p0 = plane_ptr(xin, z);
+        const float* p1 = p0 + n2;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            xpf[0][k] = p0[goff[k]];
+            xpf[1][k] = p1[goff[k]];
+        }
+    }
+    __host__ __device__ __forceinline__ bool load_b(int z, float (&v)[2][KP]) const {
+        const float* p0 = plane_ptr(r, z);
+        const float* p1 = p0 + n2;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            v[0][k] = p0[goff[k]];
+            v[1][k] = p1[goff[k]];
+        }
+        return z >= 0 && z < D.d && z >= zs - NS + 1 && z < ze + NS - 1;
+    }
+    // xpf into slot SL
+    template <int SL>
+    __host__ __device__ __forceinline__ void fill() {
+        float* base = lds + SL * 2 * PLANE;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            if ((k % KC) * NT + tid < CPOS) {
+                const bool ok = xok && ind[k];
+                base[e_[k]] = ok ? xpf[0][k] : 0.0f;
+                base[PLANE + e_[k]] = ok ? xpf[1][k] : 0.0f;
+            }
+        }
+    }
+    __host__ __device__ __forceinline__ int first_step() const { return t0; }
+    __host__ __device__ __forceinline__ int last_step() const { return ze + NS - 2; }
+
+    // planes t0-1 .. t0+1 into slots 0..2; b of plane t0; prefetch x of t0+2, b of t0+1
+    __host__ __device__ __forceinline__ void prologue() {
+        load_x(t0 - 1);
+        fill<0>();
+        load_x(t0);
+        fill<1>();
+        load_x(t0 + 1);
+        fill<2>();
+#pragma unroll
+        for (int j = 1; j < NS; ++j)
+#pragma unroll
+            for (int k = 0; k < KP; ++k) bq[j][0][k] = bq[j][1][k] = 0.0f;
+        const bool ok0 = load_b(t0, bq[0]);
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const bool ok = ok0 && ind[k];
+            bq[0][0][k] = ok ? bq[0][0][k] : 0.0f;
+            bq[0][1][k] = ok ? bq[0][1][k] : 0.0f;
+        }
+        load_x(t0 + 2);
+        bok = load_b(t0 + 1, bpf);
+    }
+    // Step t with t - t0 = J (mod SLOTS): plane t+2 (prefetched) into its
+    // slot, which no phase of step t reads; prefetch plane t+3.
+    template <int J>
+    __host__ __device__ __forceinline__ void begin_step(int t) {
+        fill<mod(J + 3)>();
+        load_x(t + 3);
+    }
+    // half-sweep H on plane t-H (t - t0 = J mod SLOTS)
+    template <int H, int J>
+    __host__ __device__ __forceinline__ void stage(int t) {
+        const int z = t - H;
+        const int lo = zs - (NS - 1 - H) > 0 ? zs - (NS - 1 - H) : 0;
+        const int hi = ze + (NS - 1 - H) < D.d ? ze + (NS - 1 - H) : D.d;
+        if (z < lo || z >= hi) return;
+        constexpr int c = (CF + H) & 1;
+        constexpr int sz = mod(J - H + 1), sm = mod(J - H), sp = mod(J - H + 2);
+        const float* xo = lds + (sz * 2 + (1 - c)) * PLANE;
+        const float* xm = lds + (sm * 2 + (1 - c)) * PLANE;
+        const float* xp = lds + (sp * 2 + (1 - c)) * PLANE;
+        float* xc = lds + (sz * 2 + c) * PLANE;
+        // both z neighbours exist or their sides are open: interior reciprocals
+        const bool zin = (z > 0 || (D.bc & 16)) && (z < D.d - 1 || (D.bc & 32));
+        const bool own = DOT && H >= NS - 2 && z >= zs && z < ze;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int p = (c + k / KC + z) & 1;  // x parity of the colour-c cell (row parity k / KC)
+            if ((act[k] >> (H + 4 * p)) & 1) {
+                const int e = e_[k];
+                float n[6];
+                n[0] = xo[e - 1 + p];  // x-1
+                n[1] = xo[e + p];      // x+1
+                n[2] = xo[e - PW];     // y-1
+                n[3] = xo[e + PW];     // y+1
+                n[4] = xm[e];          // z-1
+                n[5] = xp[e];          // z+1
+                const float rv = bq[H][c][k];
+                const float a0 = tb_opaque(zin ? rin[k][0] : rbd[k][0]);
+                const float a1 = tb_opaque(zin ? rin[k][1] : rbd[k][1]);
+                const float v = (rv + s * nsum<float, 3>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                xc[e] = v;
+                if (own && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
+            }
+        }
+    }
+    // store plane t-NS+1 (final after half-sweep NS-1 of step t); advance the
+    // b ring; prefetch b of plane t+2
+    template <int J>
+    __host__ __device__ __forceinline__ void end_step(int t) {
+        const int zo = t - NS + 1;
+        if (zo >= zs && zo < ze) {
+            const float* base = lds + mod(J - NS + 2) * 2 * PLANE;
+            float* o0 = xout + zo * hw;
+            float* o1 = o0 + n2;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                if (tile[k]) {
+                    o0[goff[k]] = base[e_[k]];
+                    o1[goff[k]] = base[PLANE + e_[k]];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = NS - 1; j > 0; --j)
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                bq[j][0][k] = bq[j - 1][0][k];
+                bq[j][1][k] = bq[j - 1][1][k];
+            }
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const bool ok = bok && ind[k];
+            bq[0][0][k] = ok ? bpf[0][k] : 0.0f;
+            bq[0][1][k] = ok ? bpf[1][k] : 0.0f;
+        }
+        bok = load_b(t + 2, bpf);
+    }
+};
+
+}  // namespace pano

@@ -1,0 +1,65 @@
+"""The one-pass pre/post-smoothing of 3D f32 multigrid levels against the
+half-sweep kernels (PANO_MG_FUSED=0): both give the same V-cycle bit for bit
+(each cell is computed from the same six neighbour values in the same order).
+The PCG's z.r partial sums are added in another order, so solves agree to
+rounding and take the same number of iterations.
+"""
+import numpy as np
+import pytest
+
+import panopaea_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((16, 16, 16), (0,) * 6, 3),                 # one tile, two plane chunks
+         ((24, 40, 64), (0, 0, 0, 0, 0, 1), 3),       # open top, ragged tiles in y
+         ((40, 48, 96), (0, 1, 0, 0, 1, 1), 4),       # ragged plane chunks on levels 0 and 1
+         ((256, 256, 256), (0, 0, 0, 0, 0, 1), 4)]    # the 256^3 smoke-plume shape (32-plane chunks)
+
+
+def vcycle(g, r, levels, fused, monkeypatch):
+    if fused:
+        monkeypatch.delenv("PANO_MG_FUSED", raising=False)
+    else:
+        monkeypatch.setenv("PANO_MG_FUSED", "0")
+    z = g.new_simplex_2()
+    pa.precond_apply(pa.Multigrid(levels, 2), z, g.new_simplex_2().upload(r), dt=0.4)
+    return z.download()
+
+
+@pytest.mark.parametrize("dims,bc,levels", CASES)
+def test_gpu_fused_smoothing_bitwise(ctx, dims, bc, levels, monkeypatch):
+    print("fused smoothing", dims, bc, levels, flush=True)
+    g = pa.Grid3d(dims, np.float32, bc=bc)
+    r = np.random.default_rng(dims[0]).standard_normal(g.num_elem_2()).astype(np.float32)
+    if not any(bc):
+        r -= r.mean(dtype=np.float64).astype(np.float32)
+    zf = vcycle(g, r, levels, True, monkeypatch)
+    zu = vcycle(g, r, levels, False, monkeypatch)
+    assert np.all(np.isfinite(zf))
+    assert np.array_equal(zf, zu)
+
+
+@pytest.mark.parametrize("dims,bc,levels", CASES[1:3])
+def test_gpu_fused_smoothing_pcg(ctx, dims, bc, levels, monkeypatch):
+    print("fused smoothing pcg", dims, bc, levels, flush=True)
+    g = pa.Grid3d(dims, np.float32, bc=bc)
+    faces = pa.gen_bandlimited_faces(g, 5, 16, 8.0)
+    u = g.new_simplex_1().upload(faces)
+    b = g.new_simplex_2()
+    pa.neg_divergence(b, u)
+    out = {}
+    for fused in (True, False):
+        if fused:
+            monkeypatch.delenv("PANO_MG_FUSED", raising=False)
+        else:
+            monkeypatch.setenv("PANO_MG_FUSED", "0")
+        x, r, a, s = (g.new_simplex_2() for _ in range(4))
+        info = pa.precond_conjugate_gradient(pa.Multigrid(levels, 2), x, b, 200, 1e-6, r, a, s,
+                                             pa.PoissonOperator(g, 0.5 / dims[-1]), tol_kind="rel")
+        out[fused] = (x.download(), info)
+    (xf, inf_f), (xu, inf_u) = out[True], out[False]
+    assert inf_f.converged and inf_u.converged
+    assert abs(inf_f.iterations - inf_u.iterations) <= 1
+    scale = np.abs(xu).max()
+    assert np.abs(xf - xu).max() <= 1e-4 * scale
