@@ -173,9 +173,20 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
 }
 
 // ---- 3D f32 levels: the pre- and post-smoothing as one pass each ----------------------
-constexpr int kTbTXP = 32, kTbTY = 32, kTbNT = 768;  // tile: 32 x-pairs by 32 rows; threads per workgroup
-// (two workgroups per CU: 79 KiB of LDS and at most 80 VGPRs each)
-constexpr int kTbWant = 512, kTbZmin = 8;
+// Configurations by level size (measured on 256^3, 128^3, 64^3): tall tiles
+// on large levels (less halo work; two 768-thread workgroups per CU with 79
+// KiB of LDS and at most 80 VGPRs each), small tiles and short plane chunks on
+// small levels, so every CU still gets workgroups. WANT/ZMIN: workgroups
+// aimed for and the shortest chunk (tb_grid).
+struct TbLarge {  // >= 2^23 cells
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 512, ZMIN = 8;
+};
+struct TbMedium {  // >= 2^20 cells
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 1024, ZMIN = 4;
+};
+struct TbSmall {
+    static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2;
+};
 
 // One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
 // after each, end.
@@ -194,11 +205,12 @@ __device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_se
 // NS half-sweeps from xin into xout (TbThread); with DOT the workgroup's sums
 // of z.r over its tile for the last two half-sweeps go to part[L] and
 // part[nb + L].
-template <int NS, int CF, bool ZERO_X, bool DOT>
-__global__ __launch_bounds__(kTbNT) __attribute__((amdgpu_waves_per_eu(6))) void k_smooth_tb(Dims D, TbGrid g, const float* xin, float* xout, const float* r,
-                                                   float s, RInv<float> ri, double* part, const SolverState* st) {
+template <int TXP, int TY, int NT, int WPE, int NS, int CF, bool ZERO_X, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb(
+    Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
+    const SolverState* st) {
     if (st->done) return;
-    using TH = TbThread<NS, kTbTXP, kTbTY, kTbNT, CF, ZERO_X, DOT>;
+    using TH = TbThread<NS, TXP, TY, NT, CF, ZERO_X, DOT>;
     __shared__ float lds[TH::S::LDS_FLOATS];
     const int L = xcd_remap(blockIdx.x, g.nb);
     const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
@@ -217,6 +229,35 @@ __global__ __launch_bounds__(kTbNT) __attribute__((amdgpu_waves_per_eu(6))) void
             part[g.nb + L] = a1;
         }
     }
+}
+
+template <class C>
+static TbGrid tb_grid_of(const Dims& D) {
+    return tb_grid<C::TXP, C::TY>(D, C::WANT, C::ZMIN);
+}
+template <class F>
+static auto with_tb(const Dims& D, F&& f) {
+    const int64_t n = (int64_t)D.d * D.h * D.w;
+    if (n >= (int64_t(1) << 23)) return f(TbLarge{});
+    if (n >= (int64_t(1) << 20)) return f(TbMedium{});
+    return f(TbSmall{});
+}
+static int tb_blocks(const Dims& D) {
+    return with_tb(D, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(D).nb; });
+}
+// The four half-sweeps (colours CF, CF^1, ...) of a 3D f32 level in one pass;
+// with part, the z.r partials (2 per workgroup, *np set).
+template <int CF, bool ZERO_X, bool DOT>
+static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, const float* xin, float* xout,
+                      const float* r, float s, const RInv<float>& ri, double* part, int* np, const SolverState* st) {
+    with_tb(D, [&](auto cfg) {
+        using C = decltype(cfg);
+        const TbGrid g = tb_grid_of<C>(D);
+        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 4, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT), D, g, xin, xout, r,
+                s, ri, part, st);
+        if (np) *np = 2 * g.nb;
+        return 0;
+    });
 }
 
 // ---- coarsest level: fast diagonalisation in FP64 (natural layout) ---------------------
@@ -508,9 +549,8 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     bool tb_done = false;
     if constexpr (DIM == 3 && std::is_same<T, float>::value) {
         if (m->tb && m->nu == 2 && D.d >= 2) {
-            const TbGrid g = tb_grid<kTbTXP, kTbTY>(D, kTbWant, kTbZmin);
-            klaunch(c, l == 0 ? "smooth_pre_l0" : "", 2 * unit, k_smooth_tb<4, 0, true, false>, dim3(g.nb),
-                    dim3(kTbNT), D, g, (const float*)zl[l], zl[l], rl[l], s, ri, (double*)nullptr, st);
+            smooth_tb<0, true, false>(c, l == 0 ? "smooth_pre_l0" : "", 2 * unit, D, (const float*)zl[l], zl[l],
+                                      rl[l], s, ri, (double*)nullptr, (int*)nullptr, st);
             tb_done = true;
         }
     }
@@ -574,8 +614,7 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
         // up leg), then the four post-smoothing half-sweeps from there back
         // into z in one pass (out of place: workgroups read their halo from
         // the input while others store)
-        const TbGrid g = tb_grid<kTbTXP, kTbTY>(D, kTbWant, kTbZmin);
-        const bool tb = m->tb && m->nu == 2 && D.d >= 2 && g.nb <= G.nb;
+        const bool tb = m->tb && m->nu == 2 && D.d >= 2 && tb_blocks(D) <= G.nb;
         float* zo = tb ? (float*)m->res : zl[l];
         if (l + 1 == Lc)
             klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zo,
@@ -585,14 +624,12 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
                     (const T*)zl[l], (const T*)m->z[l + 1], st);
         if (tb) {
             const char* npost = l == 0 ? "smooth_post_l0" : "";
-            if (l == 0 && dot_part) {
-                klaunch(c, npost, 3 * unit, k_smooth_tb<4, 1, false, true>, dim3(g.nb), dim3(kTbNT), D, g,
-                        (const float*)zo, zl[l], rl[l], s, ri, dot_part, st);
-                *dot_np = 2 * g.nb;
-            } else {
-                klaunch(c, npost, 3 * unit, k_smooth_tb<4, 1, false, false>, dim3(g.nb), dim3(kTbNT), D, g,
-                        (const float*)zo, zl[l], rl[l], s, ri, (double*)nullptr, st);
-            }
+            if (l == 0 && dot_part)
+                smooth_tb<1, false, true>(c, npost, 3 * unit, D, (const float*)zo, zl[l], rl[l], s, ri, dot_part,
+                                          dot_np, st);
+            else
+                smooth_tb<1, false, false>(c, npost, 3 * unit, D, (const float*)zo, zl[l], rl[l], s, ri,
+                                           (double*)nullptr, (int*)nullptr, st);
             return;
         }
     } else if (l + 1 == Lc) {
