@@ -13,7 +13,7 @@ average launch duration measured with HIP events on the library's stream
 during the timed steps), and the CPU oracle timed on a bounded sample of the
 same workload on this host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel smooth_l0] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel smooth_post_l0] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--kernel", default="smooth_l0", help="kernel whose roofline is reported")
+    ap.add_argument("--kernel", default="smooth_post_l0", help="kernel whose roofline is reported")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--dims", default=None)
     ap.add_argument("--workload", default="smoke3d_256",

@@ -434,8 +434,11 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
 }
 
 // ---- coarsest level in LDS ---------------------------------------------------------------
+// One workgroup per plane (per y-row for the z pass): 1024 threads, so the
+// LDS-latency-bound dot products of a slice have 4 waves per SIMD to overlap.
+constexpr int kFdmThreads = 1024;
 template <typename Tin, typename Tout>
-__global__ __launch_bounds__(256) void k_fdm_plane(int nx, int ny, int mode, const Tin* in, Tout* out,
+__global__ __launch_bounds__(kFdmThreads) void k_fdm_plane(int nx, int ny, int mode, const Tin* in, Tout* out,
                                                    const double* Vx, const double* Vy, const double* lx,
                                                    const double* ly, double s, double eps,
                                                    const SolverState* st) {
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(256) void k_fdm2_cols(int nx, int ny, double* t, co
     fdm2_cols_body(fdm_lds, nx, ny, blockIdx.x, t, Vy, lx, ly, s, eps);
 }
 
-__global__ __launch_bounds__(256) void k_fdm_zcol(int nx, int ny, int nz, double* data, const double* Vz,
+__global__ __launch_bounds__(kFdmThreads) void k_fdm_zcol(int nx, int ny, int nz, double* data, const double* Vz,
                                                   const double* lx, const double* ly, const double* lz, double s,
                                                   double eps, const SolverState* st) {
     if (st->done) return;
@@ -496,10 +499,11 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     if (DIM == 3 && nx * ny <= kFdmPlane && nx * nz <= kFdmPlane) {
         size_t lds = kFdmLdsBytes;
         {
-            hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, Vy,
+            hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 0, r, t0, Vx, Vy,
                                (const double*)nullptr, (const double*)nullptr, s, eps, st);
-            hipLaunchKernelGGL(k_fdm_zcol, dim3(ny), dim3(256), lds, S, nx, ny, nz, t0, Vz, lx, ly, lz, s, eps, st);
-            hipLaunchKernelGGL((k_fdm_plane<double, T>), dim3(nz), dim3(256), lds, S, nx, ny, 1, t0, z, Vx, Vy,
+            hipLaunchKernelGGL(k_fdm_zcol, dim3(ny), dim3(kFdmThreads), lds, S, nx, ny, nz, t0, Vz, lx, ly, lz, s, eps,
+                               st);
+            hipLaunchKernelGGL((k_fdm_plane<double, T>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 1, t0, z, Vx, Vy,
                                (const double*)nullptr, (const double*)nullptr, s, eps, st);
         }
         PANO_HIP(hipGetLastError());
