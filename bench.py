@@ -125,6 +125,17 @@ def main():
                 "launches": launches.value, "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": bpl.value}
 
+    # the whole step against the same unit rule: model bytes of the reference
+    # method (phase-separated passes) over the measured step time
+    k_mean = float(np.mean(iters))
+    b_step = step_model_bytes(n, k_mean)
+    step_gbs = b_step * args.steps / elapsed / 1e9
+    step_roof = {"bound": "hbm", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4), "model_bytes_per_step": b_step,
+                 "cg_iterations_mean": k_mean,
+                 "model": "4N(8 + 4 + 7 + 18.54 + 29.54 K) bytes: advect 8, divergence 4, gradient 7, V-cycle "
+                          "18.54 and per CG iteration 11 + 18.54 f32 field passes"}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, dims, cfg, init_vel)
@@ -149,9 +160,20 @@ def main():
                    "bc": "closed walls, open top", "buoyancy": 1.0,
                    "cg_iterations_per_step": iters},
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def step_model_bytes(n, k):
+    """Algorithmic bytes of one 3D MG-PCG fluid step with k CG iterations, in
+    f32 field passes of n cells: advect (read u, v, w, rho; write them) 8,
+    divergence 4, gradient subtraction 7, the initial V-cycle 18.54 (16.25 per
+    level over the three smoothed levels, geometric factor 1/8), and per
+    iteration q = Ap with p.q 2, the x/r update with the norm 6, the search
+    update 3 and a V-cycle 18.54."""
+    return 4.0 * n * (8 + 4 + 7 + 18.54 + 29.54 * k)
 
 
 def bench_mixed(args):

@@ -549,11 +549,13 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     const double unit = (double)D.d * D.h * D.w * sizeof(T);
     const char* nh = l == 0 ? "smooth_l0" : "";
     // 3D f32, nu = 2: the four half-sweeps from z = 0 in one pass (z is only
-    // written, so the pass may write it in place)
+    // written, so the pass may write it in place). Timed bytes: the algorithmic
+    // bytes of the work, two red-black sweeps of 3 field passes each; the pass
+    // itself moves about 2 (b in, z out).
     bool tb_done = false;
     if constexpr (DIM == 3 && std::is_same<T, float>::value) {
         if (m->tb && m->nu == 2 && D.d >= 2) {
-            smooth_tb<0, true, false>(c, l == 0 ? "smooth_pre_l0" : "", 2 * unit, D, (const float*)zl[l], zl[l],
+            smooth_tb<0, true, false>(c, l == 0 ? "smooth_pre_l0" : "", 6 * unit, D, (const float*)zl[l], zl[l],
                                       rl[l], s, ri, (double*)nullptr, (int*)nullptr, st);
             tb_done = true;
         }
@@ -617,7 +619,8 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
         // 3D f32, nu = 2: prolongation into the residual scratch (free on the
         // up leg), then the four post-smoothing half-sweeps from there back
         // into z in one pass (out of place: workgroups read their halo from
-        // the input while others store)
+        // the input while others store). Timed bytes: two red-black sweeps of
+        // 3 field passes each; the pass moves about 3.
         const bool tb = m->tb && m->nu == 2 && D.d >= 2 && tb_blocks(D) <= G.nb;
         float* zo = tb ? (float*)m->res : zl[l];
         if (l + 1 == Lc)
@@ -629,10 +632,10 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
         if (tb) {
             const char* npost = l == 0 ? "smooth_post_l0" : "";
             if (l == 0 && dot_part)
-                smooth_tb<1, false, true>(c, npost, 3 * unit, D, (const float*)zo, zl[l], rl[l], s, ri, dot_part,
+                smooth_tb<1, false, true>(c, npost, 6 * unit, D, (const float*)zo, zl[l], rl[l], s, ri, dot_part,
                                           dot_np, st);
             else
-                smooth_tb<1, false, false>(c, npost, 3 * unit, D, (const float*)zo, zl[l], rl[l], s, ri,
+                smooth_tb<1, false, false>(c, npost, 6 * unit, D, (const float*)zo, zl[l], rl[l], s, ri,
                                            (double*)nullptr, (int*)nullptr, st);
             return;
         }
