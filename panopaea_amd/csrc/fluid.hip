@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void k_advect_cell(Dims D, Grid3 G, T* dst, co
     advect_cell_run<T, DIM, NZ>(D, q, S, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) dst[cell_index(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(dst, cell_index(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T, int DIM>
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_advect_fx(Dims D, Grid3 G, T* dst, cons
     advect_face_run<T, DIM, 0, NZ>(D, qx, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) dst[fx_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(dst, fx_idx(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T, int DIM>
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_advect_fy(Dims D, Grid3 G, T* dst, cons
     advect_face_run<T, DIM, 1, NZ>(D, qy, dt, v, z0, y, x, out);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) dst[fy_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(dst, fy_idx(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T>
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void k_advect_fz(Dims D, Grid3 G, T* dst, cons
     advect_face_run<T, 3, 2, NZ>(D, qz, dt, v, z0, y, x, out, buoy_c, rho, &S);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k <= D.d) dst[fz_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k <= D.d) stx(dst, fz_idx(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T, int DIM>
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_neg_div(Dims D, Grid3 G, T* b, Faces<co
     for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, PLANE_Z(k, D.d), y, x);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) b[cell_index(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(b, cell_index(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T, int DIM>
@@ -150,11 +150,11 @@ __global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = PLANE_Z(k, D.d);
-        out[k] = project_x<T, L_>(D, ux[fx_idx(D, z, y, x)], p, dt, z, y, x);
+        out[k] = project_x<T, L_>(D, ldx(ux, fx_idx(D, z, y, x)), p, dt, z, y, x);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) ux[fx_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(ux, fx_idx(D, z0 + k, y, x), out[k]);
 }
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const T* p, T dt) {
@@ -165,11 +165,11 @@ __global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = PLANE_Z(k, D.d);
-        out[k] = project_y<T, L_>(D, uy[fy_idx(D, z, y, x)], p, dt, z, y, x);
+        out[k] = project_y<T, L_>(D, ldx(uy, fy_idx(D, z, y, x)), p, dt, z, y, x);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) uy[fy_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(uy, fy_idx(D, z0 + k, y, x), out[k]);
 }
 template <typename T, int L_>
 __global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const T* p, T dt) {
@@ -180,11 +180,11 @@ __global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = PLANE_Z(k, D.d + 1);
-        out[k] = project_z<T, L_>(D, uz[fz_idx(D, z, y, x)], p, dt, z, y, x);
+        out[k] = project_z<T, L_>(D, ldx(uz, fz_idx(D, z, y, x)), p, dt, z, y, x);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k <= D.d) uz[fz_idx(D, z0 + k, y, x)] = out[k];
+        if (z0 + k <= D.d) stx(uz, fz_idx(D, z0 + k, y, x), out[k]);
 }
 
 static Sphere make_sphere(const double* s) {

@@ -13,17 +13,19 @@ namespace pano {
 // compact index (z, y, x>>1); requires an even x extent).
 enum CellLayout { NATURAL = 0, REDBLACK = 1 };
 
-__device__ __forceinline__ int64_t cell_index(const Dims& D, int z, int y, int x) {
-    return ((int64_t)z * D.h + y) * D.w + x;
+// Indices are 32-bit: check_grid caps a grid below 2^31 faces in total, so
+// every cell field and face component holds fewer than 2^30 elements.
+__device__ __forceinline__ int cell_index(const Dims& D, int z, int y, int x) {
+    return (z * D.h + y) * D.w + x;
 }
-__device__ __forceinline__ int64_t rb_index(const Dims& D, int z, int y, int x) {
+__device__ __forceinline__ int rb_index(const Dims& D, int z, int y, int x) {
     int wh = D.w >> 1;
-    int64_t n2 = (int64_t)D.d * D.h * wh;
+    int n2 = D.d * D.h * wh;
     int c = (x + y + z) & 1;
-    return c * n2 + ((int64_t)z * D.h + y) * wh + (x >> 1);
+    return c * n2 + (z * D.h + y) * wh + (x >> 1);
 }
 template <int L>
-__device__ __forceinline__ int64_t lidx(const Dims& D, int z, int y, int x) {
+__device__ __forceinline__ int lidx(const Dims& D, int z, int y, int x) {
     if constexpr (L == REDBLACK) return rb_index(D, z, y, x);
     else return cell_index(D, z, y, x);
 }
@@ -73,14 +75,29 @@ __host__ __device__ inline Faces<T> split_faces(const Dims& D, T* f) {
     }
     return r;
 }
-__device__ __forceinline__ int64_t fz_idx(const Dims& D, int z, int y, int x) {
-    return ((int64_t)z * D.h + y) * D.w + x;
+__device__ __forceinline__ int fz_idx(const Dims& D, int z, int y, int x) {
+    return (z * D.h + y) * D.w + x;
 }
-__device__ __forceinline__ int64_t fy_idx(const Dims& D, int z, int y, int x) {
-    return ((int64_t)z * (D.h + 1) + y) * D.w + x;
+__device__ __forceinline__ int fy_idx(const Dims& D, int z, int y, int x) {
+    return (z * (D.h + 1) + y) * D.w + x;
 }
-__device__ __forceinline__ int64_t fx_idx(const Dims& D, int z, int y, int x) {
-    return ((int64_t)z * D.h + y) * (D.w + 1) + x;
+__device__ __forceinline__ int fx_idx(const Dims& D, int z, int y, int x) {
+    return (z * D.h + y) * (D.w + 1) + x;
+}
+
+// Loads and stores of an element of one cell field or face component: f32
+// as the array's base (uniform, in scalar registers) plus a 32-bit byte
+// offset (fewer than 2^30 elements), with no per-access 64-bit address
+// arithmetic; f64 indexes normally.
+template <typename T>
+__device__ __forceinline__ T ldx(const T* p, int i) {
+    if constexpr (sizeof(T) == 4) return *(const T*)((const char*)p + ((unsigned)i << 2));
+    else return p[i];
+}
+template <typename T>
+__device__ __forceinline__ void stx(T* p, int i, T v) {
+    if constexpr (sizeof(T) == 4) *(T*)((char*)p + ((unsigned)i << 2)) = v;
+    else p[i] = v;
 }
 
 // ---- density source sphere ----------------------------------------------------------
@@ -141,13 +158,13 @@ __device__ __forceinline__ void advect_cell_run(const Dims& D, const T* q, const
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = imin(z0 + k, D.d - 1);
-        vx[k][0] = v.x[fx_idx(D, z, y, x)];
-        vx[k][1] = v.x[fx_idx(D, z, y, x + 1)];
-        vy[k][0] = v.y[fy_idx(D, z, y, x)];
-        vy[k][1] = v.y[fy_idx(D, z, y + 1, x)];
+        vx[k][0] = ldx(v.x, fx_idx(D, z, y, x));
+        vx[k][1] = ldx(v.x, fx_idx(D, z, y, x + 1));
+        vy[k][0] = ldx(v.y, fy_idx(D, z, y, x));
+        vy[k][1] = ldx(v.y, fy_idx(D, z, y + 1, x));
         if constexpr (DIM == 3) {
-            vz[k][0] = v.z[fz_idx(D, z, y, x)];
-            vz[k][1] = v.z[fz_idx(D, z + 1, y, x)];
+            vz[k][0] = ldx(v.z, fz_idx(D, z, y, x));
+            vz[k][1] = ldx(v.z, fz_idx(D, z + 1, y, x));
         }
     }
     constexpr int NG = DIM == 3 ? 8 : 4;
@@ -176,7 +193,7 @@ __device__ __forceinline__ void advect_cell_run(const Dims& D, const T* q, const
     for (int k = 0; k < NZ; ++k)
 #pragma unroll
         for (int j = 0; j < NG; ++j)
-            g[k][j] = q[cell_index(D, gz[k][j >> 2], gy[k][(j >> 1) & 1], gx[k][j & 1])];
+            g[k][j] = ldx(q, cell_index(D, gz[k][j >> 2], gy[k][(j >> 1) & 1], gx[k][j & 1]));
     if (S.on) {  // the source reset of the gathered cells (rho_eff): squared
                  // distances per axis, summed as in_sphere sums them
 #pragma unroll
@@ -244,41 +261,41 @@ __device__ __forceinline__ void advect_face_run(const Dims& D, const T* qc, T dt
         int z = imin(z0 + k, np - 1);
         if constexpr (AXIS == 0) {
             int xa = imin(x, D.w - 1), xb = x > 0 ? x - 1 : 0;
-            own[k] = v.x[fx_idx(D, z, y, x)];
-            oa[k][0] = v.y[fy_idx(D, z, y, xa)];
-            oa[k][1] = v.y[fy_idx(D, z, y + 1, xa)];
-            oa[k][2] = v.y[fy_idx(D, z, y, xb)];
-            oa[k][3] = v.y[fy_idx(D, z, y + 1, xb)];
+            own[k] = ldx(v.x, fx_idx(D, z, y, x));
+            oa[k][0] = ldx(v.y, fy_idx(D, z, y, xa));
+            oa[k][1] = ldx(v.y, fy_idx(D, z, y + 1, xa));
+            oa[k][2] = ldx(v.y, fy_idx(D, z, y, xb));
+            oa[k][3] = ldx(v.y, fy_idx(D, z, y + 1, xb));
             if constexpr (DIM == 3) {
-                ob[k][0] = v.z[fz_idx(D, z, y, xa)];
-                ob[k][1] = v.z[fz_idx(D, z + 1, y, xa)];
-                ob[k][2] = v.z[fz_idx(D, z, y, xb)];
-                ob[k][3] = v.z[fz_idx(D, z + 1, y, xb)];
+                ob[k][0] = ldx(v.z, fz_idx(D, z, y, xa));
+                ob[k][1] = ldx(v.z, fz_idx(D, z + 1, y, xa));
+                ob[k][2] = ldx(v.z, fz_idx(D, z, y, xb));
+                ob[k][3] = ldx(v.z, fz_idx(D, z + 1, y, xb));
             }
         } else if constexpr (AXIS == 1) {
             int ya = imin(y, D.h - 1), yb = y > 0 ? y - 1 : 0;
-            oa[k][0] = v.x[fx_idx(D, z, ya, x)];
-            oa[k][1] = v.x[fx_idx(D, z, ya, x + 1)];
-            oa[k][2] = v.x[fx_idx(D, z, yb, x)];
-            oa[k][3] = v.x[fx_idx(D, z, yb, x + 1)];
-            own[k] = v.y[fy_idx(D, z, y, x)];
+            oa[k][0] = ldx(v.x, fx_idx(D, z, ya, x));
+            oa[k][1] = ldx(v.x, fx_idx(D, z, ya, x + 1));
+            oa[k][2] = ldx(v.x, fx_idx(D, z, yb, x));
+            oa[k][3] = ldx(v.x, fx_idx(D, z, yb, x + 1));
+            own[k] = ldx(v.y, fy_idx(D, z, y, x));
             if constexpr (DIM == 3) {
-                ob[k][0] = v.z[fz_idx(D, z, ya, x)];
-                ob[k][1] = v.z[fz_idx(D, z + 1, ya, x)];
-                ob[k][2] = v.z[fz_idx(D, z, yb, x)];
-                ob[k][3] = v.z[fz_idx(D, z + 1, yb, x)];
+                ob[k][0] = ldx(v.z, fz_idx(D, z, ya, x));
+                ob[k][1] = ldx(v.z, fz_idx(D, z + 1, ya, x));
+                ob[k][2] = ldx(v.z, fz_idx(D, z, yb, x));
+                ob[k][3] = ldx(v.z, fz_idx(D, z + 1, yb, x));
             }
         } else {
             int za = imin(z, D.d - 1), zb = z > 0 ? z - 1 : 0;
-            oa[k][0] = v.x[fx_idx(D, za, y, x)];
-            oa[k][1] = v.x[fx_idx(D, za, y, x + 1)];
-            oa[k][2] = v.x[fx_idx(D, zb, y, x)];
-            oa[k][3] = v.x[fx_idx(D, zb, y, x + 1)];
-            ob[k][0] = v.y[fy_idx(D, za, y, x)];
-            ob[k][1] = v.y[fy_idx(D, za, y + 1, x)];
-            ob[k][2] = v.y[fy_idx(D, zb, y, x)];
-            ob[k][3] = v.y[fy_idx(D, zb, y + 1, x)];
-            own[k] = v.z[fz_idx(D, z, y, x)];
+            oa[k][0] = ldx(v.x, fx_idx(D, za, y, x));
+            oa[k][1] = ldx(v.x, fx_idx(D, za, y, x + 1));
+            oa[k][2] = ldx(v.x, fx_idx(D, zb, y, x));
+            oa[k][3] = ldx(v.x, fx_idx(D, zb, y, x + 1));
+            ob[k][0] = ldx(v.y, fy_idx(D, za, y, x));
+            ob[k][1] = ldx(v.y, fy_idx(D, za, y + 1, x));
+            ob[k][2] = ldx(v.y, fy_idx(D, zb, y, x));
+            ob[k][3] = ldx(v.y, fy_idx(D, zb, y + 1, x));
+            own[k] = ldx(v.z, fz_idx(D, z, y, x));
         }
     }
     constexpr int NG = DIM == 3 ? 8 : 4;
@@ -329,14 +346,14 @@ __device__ __forceinline__ void advect_face_run(const Dims& D, const T* qc, T dt
 #pragma unroll
         for (int j = 0; j < NG; ++j) {
             int zz = gz[k][j >> 2], yy = gy[k][(j >> 1) & 1], xx = gx[k][j & 1];
-            int64_t i = AXIS == 0 ? fx_idx(D, zz, yy, xx) : (AXIS == 1 ? fy_idx(D, zz, yy, xx) : fz_idx(D, zz, yy, xx));
-            g[k][j] = qc[i];
+            int i = AXIS == 0 ? fx_idx(D, zz, yy, xx) : (AXIS == 1 ? fy_idx(D, zz, yy, xx) : fz_idx(D, zz, yy, xx));
+            g[k][j] = ldx(qc, i);
         }
         if constexpr (AXIS == 2) {
             if (buoy_c != zero && rho) {
                 int z = imin(z0 + k, np - 1);
                 int zl = z > 0 ? z - 1 : 0, zh = z < D.d ? z : D.d - 1;
-                T a = rho[cell_index(D, zl, y, x)], b = rho[cell_index(D, zh, y, x)];
+                T a = ldx(rho, cell_index(D, zl, y, x)), b = ldx(rho, cell_index(D, zh, y, x));
                 rlo[k] = z > 0 ? ((S->on && in_sphere(*S, D, zl, y, x)) ? (T)1 : a) : zero;
                 rhi[k] = z < D.d ? ((S->on && in_sphere(*S, D, zh, y, x)) ? (T)1 : b) : zero;
             }
@@ -387,9 +404,9 @@ __device__ __forceinline__ T advect_face_z(const Dims& D, const T* qz, T dt, con
 // div = (vz[z+1]-vz[z]) + (((vy[y+1]-vy[y]) - vx[x]) + vx[x+1]).
 template <typename T, int DIM>
 __device__ __forceinline__ T div_value(const Dims& D, const Faces<const T>& v, int z, int y, int x) {
-    T d2 = ((v.y[fy_idx(D, z, y + 1, x)] - v.y[fy_idx(D, z, y, x)]) - v.x[fx_idx(D, z, y, x)]) +
-           v.x[fx_idx(D, z, y, x + 1)];
-    if constexpr (DIM == 3) return (v.z[fz_idx(D, z + 1, y, x)] - v.z[fz_idx(D, z, y, x)]) + d2;
+    T d2 = ((ldx(v.y, fy_idx(D, z, y + 1, x)) - ldx(v.y, fy_idx(D, z, y, x))) - ldx(v.x, fx_idx(D, z, y, x))) +
+           ldx(v.x, fx_idx(D, z, y, x + 1));
+    if constexpr (DIM == 3) return (ldx(v.z, fz_idx(D, z + 1, y, x)) - ldx(v.z, fz_idx(D, z, y, x))) + d2;
     else return d2;
 }
 
@@ -499,7 +516,7 @@ __device__ __forceinline__ void rb_nbrs(const T* v, const RBRow& R, int k, T* n)
     int has = k == 0 ? R.has0 : R.has1;
     T raw[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) raw[q] = (q < 4 || DIM == 3) ? v[j[q]] : (T)0;
+    for (int q = 0; q < 6; ++q) raw[q] = (q < 4 || DIM == 3) ? ldx(v, j[q]) : (T)0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) n[q] = ((has >> q) & 1) ? raw[q] : (T)0;
 }
@@ -524,13 +541,13 @@ template <typename T, int L>
 __device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
     const T zero = (T)0;
     T g;
-    if (x > 0 && x < D.w) g = p[lidx<L>(D, z, y, x - 1)] - p[lidx<L>(D, z, y, x)];
+    if (x > 0 && x < D.w) g = ldx(p, lidx<L>(D, z, y, x - 1)) - ldx(p, lidx<L>(D, z, y, x));
     else if (x == 0) {
         if (!(D.bc & 1)) return zero;
-        g = zero - p[lidx<L>(D, z, y, 0)];
+        g = zero - ldx(p, lidx<L>(D, z, y, 0));
     } else {
         if (!(D.bc & 2)) return zero;
-        g = p[lidx<L>(D, z, y, D.w - 1)] - zero;
+        g = ldx(p, lidx<L>(D, z, y, D.w - 1)) - zero;
     }
     return u + dt * g;
 }
@@ -538,13 +555,13 @@ template <typename T, int L>
 __device__ __forceinline__ T project_y(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
     const T zero = (T)0;
     T g;
-    if (y > 0 && y < D.h) g = -(p[lidx<L>(D, z, y, x)] - p[lidx<L>(D, z, y - 1, x)]);
+    if (y > 0 && y < D.h) g = -(ldx(p, lidx<L>(D, z, y, x)) - ldx(p, lidx<L>(D, z, y - 1, x)));
     else if (y == 0) {
         if (!(D.bc & 4)) return zero;
-        g = -(p[lidx<L>(D, z, 0, x)] - zero);
+        g = -(ldx(p, lidx<L>(D, z, 0, x)) - zero);
     } else {
         if (!(D.bc & 8)) return zero;
-        g = -(zero - p[lidx<L>(D, z, D.h - 1, x)]);
+        g = -(zero - ldx(p, lidx<L>(D, z, D.h - 1, x)));
     }
     return u + dt * g;
 }
@@ -552,13 +569,13 @@ template <typename T, int L>
 __device__ __forceinline__ T project_z(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
     const T zero = (T)0;
     T g;
-    if (z > 0 && z < D.d) g = -(p[lidx<L>(D, z, y, x)] - p[lidx<L>(D, z - 1, y, x)]);
+    if (z > 0 && z < D.d) g = -(ldx(p, lidx<L>(D, z, y, x)) - ldx(p, lidx<L>(D, z - 1, y, x)));
     else if (z == 0) {
         if (!(D.bc & 16)) return zero;
-        g = -(p[lidx<L>(D, 0, y, x)] - zero);
+        g = -(ldx(p, lidx<L>(D, 0, y, x)) - zero);
     } else {
         if (!(D.bc & 32)) return zero;
-        g = -(zero - p[lidx<L>(D, D.d - 1, y, x)]);
+        g = -(zero - ldx(p, lidx<L>(D, D.d - 1, y, x)));
     }
     return u + dt * g;
 }

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
         int I = imin(imax(Ib + ix, 0), Dc.w - 1);
         int J = imin(imax(Jb + iy, 0), Dc.h - 1);
         int K = imin(imax(Kb + iz, 0), Dc.d - 1);
-        vc[i] = zc[cidx<LC>(Dc, K, J, I)];
+        vc[i] = ldx(zc, cidx<LC>(Dc, K, J, I));
     }
 #pragma unroll
     for (int i = 0; i < NP; ++i)
@@ -144,8 +144,8 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
         RBRow R = rb_row<3>(Df, zclamp(Df, z0 + k), y, m);
         i0[k] = R.i0;
         i1[k] = R.i1;
-        f0[k] = zf[R.i0];
-        f1[k] = zf[R.i1];
+        f0[k] = ldx(zf, R.i0);
+        f1[k] = ldx(zf, R.i1);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
@@ -167,8 +167,8 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
             }
         float v0 = wz0 * (wy0 * px0[0][0] + wy1 * px0[0][1]) + wz1 * (wy0 * px0[1][0] + wy1 * px0[1][1]);
         float v1 = wz0 * (wy0 * px1[0][0] + wy1 * px1[0][1]) + wz1 * (wy0 * px1[1][0] + wy1 * px1[1][1]);
-        zo[i0[k]] = f0[k] + v0;
-        zo[i1[k]] = f1[k] + v1;
+        stx(zo, i0[k], f0[k] + v0);
+        stx(zo, i1[k], f1[k] + v1);
     }
 }
 

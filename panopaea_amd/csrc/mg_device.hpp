@@ -90,12 +90,11 @@ __device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* 
             int y = imin(imax(2 * J - 1 + q, 0), Df.h - 1);
             int row = z * hw + y * wh + I;
             int ce = (y + z) & 1;  // colour of the even fine x = 2I
-            const T* pe = res + ce * n2 + row;
-            const T* po = res + (1 - ce) * n2 + row;
-            v[p][q][0] = po[hmI ? -1 : 0];  // x = 2I-1
-            v[p][q][1] = pe[0];             // x = 2I
-            v[p][q][2] = po[0];             // x = 2I+1
-            v[p][q][3] = pe[hpI ? 1 : 0];   // x = 2I+2
+            const int ie = ce * n2 + row, io = (1 - ce) * n2 + row;
+            v[p][q][0] = ldx(res, io - (hmI ? 1 : 0));  // x = 2I-1
+            v[p][q][1] = ldx(res, ie);                  // x = 2I
+            v[p][q][2] = ldx(res, io);                  // x = 2I+1
+            v[p][q][3] = ldx(res, ie + (hpI ? 1 : 0));  // x = 2I+2
         }
     T ty[NP];
 #pragma unroll
@@ -110,11 +109,11 @@ __device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* 
         for (int k = 0; k < NK; ++k) {
             int K = K0 + k;
             if (K < Dc.d)
-                rc[cidx<LC>(Dc, K, J, I)] =
-                    r1(ty[2 * k], ty[2 * k + 1], ty[2 * k + 2], ty[2 * k + 3], K > 0, K < Dc.d - 1);
+                stx(rc, cidx<LC>(Dc, K, J, I),
+                    r1(ty[2 * k], ty[2 * k + 1], ty[2 * k + 2], ty[2 * k + 3], K > 0, K < Dc.d - 1));
         }
     } else {
-        rc[cidx<LC>(Dc, 0, J, I)] = ty[0];
+        stx(rc, cidx<LC>(Dc, 0, J, I), ty[0]);
     }
 }
 
@@ -178,7 +177,7 @@ __device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T*
         idx[k] = c == 0 ? R.i0 : R.i1;
         int has = c == 0 ? R.has0 : R.has1;
         rb_nbrs<T, DIM>(zv, R, c, nb[k]);
-        rv[k] = r[idx[k]];
+        rv[k] = ldx(r, idx[k]);
         nc[k] = ncount_bits<DIM>(has, D.bc);
     }
     double acc = 0.0;
@@ -186,7 +185,7 @@ __device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T*
     for (int k = 0; k < NZ; ++k) {
         if (z0 + k < D.d) {
             T v = gs_value(rv[k], s, nsum<T, DIM>(nb[k]), nc[k], ri);
-            zv[idx[k]] = v;
+            stx(zv, idx[k], v);
             acc += (double)v * (double)rv[k];
         }
     }
@@ -206,7 +205,7 @@ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T d
             idx[k][c] = c == 0 ? R.i0 : R.i1;
             hs[k][c] = c == 0 ? R.has0 : R.has1;
             rb_nbrs<T, DIM>(p, R, c, nb[k][c]);
-            pc[k][c] = p[idx[k][c]];
+            pc[k][c] = ldx(p, idx[k][c]);
         }
     }
     double s = 0.0;
@@ -217,7 +216,7 @@ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T d
             for (int c = 0; c < 2; ++c) {
                 const T* n = nb[k][c];
                 T v = poisson_from<T, DIM>(pc[k][c], n[0], n[1], n[2], n[3], n[4], n[5], hs[k][c], D.bc, dt);
-                q[idx[k][c]] = v;
+                stx(q, idx[k][c], v);
                 s += (double)pc[k][c] * (double)v;
             }
         }
@@ -239,11 +238,11 @@ __device__ __forceinline__ void red_init_run(const Dims& D, T* zv, const T* r, T
         int c0 = (y + z) & 1;  // the red (colour 0) cell of the pair is x0 when c0 == 0
         ir[k] = c0 == 0 ? R.i0 : R.i1;
         hr[k] = c0 == 0 ? R.has0 : R.has1;
-        rr[k] = r[ir[k]];
+        rr[k] = ldx(r, ir[k]);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) zv[ir[k]] = gs_value(rr[k], s, (T)0, ncount_bits<DIM>(hr[k], D.bc), ri);
+        if (z0 + k < D.d) stx(zv, ir[k], gs_value(rr[k], s, (T)0, ncount_bits<DIM>(hr[k], D.bc), ri));
 }
 
 // The whole first sweep from z = 0 for the x-pair run (both colours in one
@@ -266,8 +265,8 @@ __device__ __forceinline__ void smooth_first_run(const Dims& D, T* zv, const T* 
         ib[k] = kb == 0 ? R.i0 : R.i1;
         hr[k] = kr == 0 ? R.has0 : R.has1;
         hb[k] = kb == 0 ? R.has0 : R.has1;
-        rr[k] = r[ir[k]];
-        rbv[k] = r[ib[k]];
+        rr[k] = ldx(r, ir[k]);
+        rbv[k] = ldx(r, ib[k]);
         rb_nbrs<T, DIM>(r, R, kb, rn[k]);
     }
 #pragma unroll
@@ -276,7 +275,7 @@ __device__ __forceinline__ void smooth_first_run(const Dims& D, T* zv, const T* 
         if (z >= D.d) continue;
         int xb = 2 * m + (1 - ((y + z) & 1));
         int h = hb[k];
-        zv[ir[k]] = gs_value(rr[k], s, zero, ncount_bits<DIM>(hr[k], D.bc), ri);
+        stx(zv, ir[k], gs_value(rr[k], s, zero, ncount_bits<DIM>(hr[k], D.bc), ri));
         T n[6];
         n[0] = (h & 1) ? gs_value(rn[k][0], s, zero, ncount<DIM>(D, z, y, xb - 1), ri) : zero;
         n[1] = (h & 2) ? gs_value(rn[k][1], s, zero, ncount<DIM>(D, z, y, xb + 1), ri) : zero;
@@ -289,7 +288,7 @@ __device__ __forceinline__ void smooth_first_run(const Dims& D, T* zv, const T* 
             n[4] = zero;
             n[5] = zero;
         }
-        zv[ib[k]] = gs_value(rbv[k], s, nsum<T, DIM>(n), ncount_bits<DIM>(h, D.bc), ri);
+        stx(zv, ib[k], gs_value(rbv[k], s, nsum<T, DIM>(n), ncount_bits<DIM>(h, D.bc), ri));
     }
 }
 
@@ -307,8 +306,8 @@ __device__ __forceinline__ void residual_run(const Dims& D, T* res, const T* zv,
             idx[k][c] = c == 0 ? R.i0 : R.i1;
             hs[k][c] = c == 0 ? R.has0 : R.has1;
             rb_nbrs<T, DIM>(zv, R, c, nb[k][c]);
-            zc[k][c] = zv[idx[k][c]];
-            rc[k][c] = r[idx[k][c]];
+            zc[k][c] = ldx(zv, idx[k][c]);
+            rc[k][c] = ldx(r, idx[k][c]);
         }
     }
 #pragma unroll
@@ -317,7 +316,7 @@ __device__ __forceinline__ void residual_run(const Dims& D, T* res, const T* zv,
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 T az = s * ((T)ncount_bits<DIM>(hs[k][c], D.bc) * zc[k][c] - nsum<T, DIM>(nb[k][c]));
-                res[idx[k][c]] = rc[k][c] - az;
+                stx(res, idx[k][c], rc[k][c] - az);
             }
         }
     }
@@ -353,16 +352,16 @@ __device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* z
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 int jj = b == 0 ? J0 : J1;
-                cm[k][a][b] = zc[cidx<LC>(Dc, kk, jj, m)];
-                cl[k][a][b] = zc[cidx<LC>(Dc, kk, jj, Il)];
-                cr[k][a][b] = zc[cidx<LC>(Dc, kk, jj, Ir)];
+                cm[k][a][b] = ldx(zc, cidx<LC>(Dc, kk, jj, m));
+                cl[k][a][b] = ldx(zc, cidx<LC>(Dc, kk, jj, Il));
+                cr[k][a][b] = ldx(zc, cidx<LC>(Dc, kk, jj, Ir));
             }
         }
         RBRow R = rb_row<DIM>(Df, z, y, m);
         i0[k] = R.i0;
         i1[k] = R.i1;
-        f0[k] = zf[R.i0];
-        f1[k] = zf[R.i1];
+        f0[k] = ldx((const T*)zf, R.i0);
+        f1[k] = ldx((const T*)zf, R.i1);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
@@ -383,8 +382,8 @@ __device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* z
             v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
             v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
         }
-        zf[i0[k]] = f0[k] + v0;
-        zf[i1[k]] = f1[k] + v1;
+        stx(zf, i0[k], f0[k] + v0);
+        stx(zf, i1[k], f1[k] + v1);
     }
 }
 

@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void k_neg_div_to(Dims D, Grid3 G, T* b, Faces
     for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, imin(z0 + k, D.d - 1), y, x);
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) b[lidx<L_>(D, z0 + k, y, x)] = out[k];
+        if (z0 + k < D.d) stx(b, lidx<L_>(D, z0 + k, y, x), out[k]);
 }
 
 template <typename T>

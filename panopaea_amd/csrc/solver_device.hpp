@@ -128,13 +128,13 @@ static __global__ void k_state_init(SolverState* st, int max_iter) {
 // partial sum and max|.| of r (the right-hand side)
 template <typename T>
 __device__ __forceinline__ void flat_stats(int lb, int64_t n, const T* r, double* sum, double* mx) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     double s = 0.0, m = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
+        const int i = base + k * kBlock;
         if (i < n) {
-            double v = (double)r[i];
+            double v = (double)ldx(r, i);
             s += v;
             m = absmax_acc(m, v);
         }
@@ -145,14 +145,14 @@ __device__ __forceinline__ void flat_stats(int lb, int64_t n, const T* r, double
 
 template <typename T>
 __device__ __forceinline__ double flat_sub_mean(int lb, int64_t n, T* r, double mean) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     double m = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
+        const int i = base + k * kBlock;
         if (i < n) {
-            T v = (T)((double)r[i] - mean);
-            r[i] = v;
+            T v = (T)((double)ldx(r, i) - mean);
+            stx(r, i, v);
             m = absmax_acc(m, (double)v);
         }
     }
@@ -161,26 +161,26 @@ __device__ __forceinline__ double flat_sub_mean(int lb, int64_t n, T* r, double 
 
 template <typename T>
 __device__ __forceinline__ void flat_zero(int lb, int64_t n, T* x) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) x[i] = (T)0;
+        const int i = base + k * kBlock;
+        if (i < n) stx(x, i, (T)0);
     }
 }
 
 // p = z ; returns the block's z.r
 template <typename T>
 __device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T* z, const T* r) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
+        const int i = base + k * kBlock;
         if (i < n) {
-            T zv = z[i];
-            p[i] = zv;
-            s += (double)zv * (double)r[i];
+            T zv = ldx(z, i);
+            stx(p, i, zv);
+            s += (double)zv * (double)ldx(r, i);
         }
     }
     return block_sum(s);
@@ -188,12 +188,12 @@ __device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T
 
 template <typename T>
 __device__ __forceinline__ double flat_dot(int lb, int64_t n, const T* a, const T* b) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) s += (double)a[i] * (double)b[i];
+        const int i = base + k * kBlock;
+        if (i < n) s += (double)ldx(a, i) * (double)ldx(b, i);
     }
     return block_sum(s);
 }
@@ -204,25 +204,25 @@ __device__ __forceinline__ double flat_dot(int lb, int64_t n, const T* a, const 
 template <typename T>
 __device__ __forceinline__ void flat_update(int lb, int64_t n, T* x, T* r, const T* p, const T* q, double alpha,
                                             bool want_rr, double* mx, double* rr) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     T xv[kItems], rv[kItems], pv[kItems], qv[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        int64_t j = i < n ? i : n - 1;
-        xv[k] = x[j];
-        rv[k] = r[j];
-        pv[k] = p[j];
-        qv[k] = q[j];
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        xv[k] = ldx(x, j);
+        rv[k] = ldx(r, j);
+        pv[k] = ldx(p, j);
+        qv[k] = ldx(q, j);
     }
     double m = 0.0, s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
+        const int i = base + k * kBlock;
         if (i < n) {
-            x[i] = (T)((double)xv[k] + alpha * (double)pv[k]);
+            stx(x, i, (T)((double)xv[k] + alpha * (double)pv[k]));
             T rn = (T)((double)rv[k] - alpha * (double)qv[k]);
-            r[i] = rn;
+            stx(r, i, rn);
             m = absmax_acc(m, (double)rn);
             s += (double)rn * (double)rn;
         }
@@ -234,19 +234,19 @@ __device__ __forceinline__ void flat_update(int lb, int64_t n, T* x, T* r, const
 // p = z + beta p (p and z distinct; loads first)
 template <typename T>
 __device__ __forceinline__ void flat_search(int lb, int64_t n, T* p, const T* z, double beta) {
-    int64_t base = (int64_t)lb * kFlatPer + threadIdx.x;
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     T pv[kItems], zv[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        int64_t j = i < n ? i : n - 1;
-        pv[k] = p[j];
-        zv[k] = z[j];
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        pv[k] = ldx(p, j);
+        zv[k] = ldx(z, j);
     }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) p[i] = (T)((double)zv[k] + beta * (double)pv[k]);
+        const int i = base + k * kBlock;
+        if (i < n) stx(p, i, (T)((double)zv[k] + beta * (double)pv[k]));
     }
 }
 
