@@ -315,7 +315,9 @@ __global__ __launch_bounds__(256) void kb_restrict(BTab T, char* A,
     }
 }
 
-// prolongation from level l+1 over prl_[l]: o[0] = z_l, o[1] = z_{l+1}
+// prolongation from level l+1 over prl_[l]: o[2] = o[0] + P o[1], o[0] = z_l,
+// o[1] = z_{l+1}, o[2] = z_l (in place) or the residual scratch (the input of
+// the one-pass post-smoothing)
 template <int LC>
 __global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
                                                   const SolverState* st) {
@@ -324,11 +326,38 @@ __global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
     int m, y, z;
     local_xyz(E.G, lb, m, y, z);
     if (m >= (E.D.w >> 1) || y >= E.D.h) return;
-    float* zf = fld<float>(A, E.o[0]);
+    const float* zf = fld<float>(A, E.o[0]);
     const float* zc = fld<float>(A, E.o[1]);
+    float* zo = fld<float>(A, E.o[2]);
     with_run(E.D, [&](auto dc, auto rc) {
-        prolong_run<float, decltype(dc)::value, LC, decltype(rc)::value>(E.D, E.Dc, zf, zc, m, y, z * rc);
+        prolong_run<float, decltype(dc)::value, LC, decltype(rc)::value>(E.D, E.Dc, zo, zf, zc, m, y, z * rc);
     });
+}
+
+// one-pass smoothing (the four half-sweeps of a leg) over tb_[l][config]:
+// o[0] = input z (post: the prolonged z in the residual scratch; pre: unused,
+// the sweeps start from 0), o[1] = z_l (output), o[2] = r_l. Tile (bx, by, bz)
+// of E.G as in the single-grid kernel. With DOT, the tile's z.r sums of the
+// last two half-sweeps go to p0[base + b] and p2[base + b].
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb(
+    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base) {
+    BATCH_ENTRY(T);
+    if (st[E.sidx].done) return;
+    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
+    const int bx = lb % E.G.gx, q = lb / E.G.gx, by = q % E.G.gy, bz = q / E.G.gy;
+    double acc[2];
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, E.D, bx, by, bz, E.zc, fld<float>(A, E.o[0]),
+                                               fld<float>(A, E.o[1]), fld<float>(A, E.o[2]), (float)E.s, E.rinv,
+                                               acc);
+    if constexpr (DOT) {
+        const double a0 = block_sum(acc[0]);
+        const double a1 = block_sum(acc[1]);
+        if (threadIdx.x == 0) {
+            p0[base + b_] = a0;
+            p2[base + b_] = a1;
+        }
+    }
 }
 
 #define FDM_ENTRY(T) const BFdm& F = T.e[find_entry(T.start, T.n, blockIdx.x)];
@@ -556,8 +585,15 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     }
     for (int l = 0; l < 16; ++l) {
         reset(rb_[l]);
+        reset(rbs_[l]);
+        for (int k = 0; k < kTbConfigs; ++k) reset(tb_[l][k]);
         reset(co_[l]);
         reset(prl_[l]);
+    }
+    reset(dot_);
+    {
+        const char* fused = getenv("PANO_MG_FUSED");
+        tb_enabled_ = !(fused && fused[0] == '0');
     }
     plane_.h.clear();
     zcol_.h.clear();
@@ -567,6 +603,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     any3d_ = false;
     for (int l = 0; l < 16; ++l) split_first_[l] = false;
     int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
+    int srbs[16] = {0}, stb[16][kTbConfigs] = {};
+    std::vector<std::pair<int, int>> dot_src;  // per grid: (config + 1 or 0 for half-sweeps, entry index) at level 0
     int srow = 0, scol = 0;
     size_t maxp = 0;
     for (int id : ids) {
@@ -657,9 +695,38 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 r.o[2] = g.o_res;
                 r.o[3] = g.o_p;
                 rb_[l].h.push_back(r);
+                // smoothing: one pass per leg on 3D levels (the single-grid rule), else half-sweeps
+                const bool tb = tb_on(tb_enabled_, nu, L);
+                if (tb) {
+                    const int k = tb_config(L);
+                    BEntry t = r;
+                    with_tb(L, [&](auto cfg) {
+                        const TbGrid tg = tb_grid_of<decltype(cfg)>(L);
+                        t.G.gx = tg.gx;
+                        t.G.gy = tg.gy;
+                        t.G.gz = tg.nch;
+                        t.G.nb = tg.nb;
+                        t.zc = tg.zc;
+                        return 0;
+                    });
+                    t.start = stb[l][k];
+                    stb[l][k] += t.G.nb;
+                    t.o[0] = g.o_res;
+                    t.o[1] = g.o_zl[l];
+                    t.o[2] = g.o_rl[l];
+                    tb_[l][k].h.push_back(t);
+                    if (l == 0) dot_src.emplace_back(k + 1, (int)tb_[l][k].h.size() - 1);
+                } else {
+                    BEntry h = r;
+                    h.start = srbs[l];
+                    srbs[l] += h.G.nb;
+                    rbs_[l].h.push_back(h);
+                    if (l == 0) dot_src.emplace_back(0, (int)rbs_[l].h.size() - 1);
+                }
                 BEntry pr = r;
                 pr.o[0] = g.o_zl[l];
                 pr.o[1] = g.o_zl[l + 1];
+                pr.o[2] = tb ? g.o_res : g.o_zl[l];
                 prl_[l].h.push_back(pr);
                 BEntry co = r;
                 int nk = restrict_run_len(r.Dc);  // coarse planes per thread
@@ -713,13 +780,36 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         }
         maxp = std::max(maxp, (size_t)std::max({snat, sflat, srb[0]}));
     }
+    // per grid, the range of its level-0 z.r partials: half-sweep tiles first,
+    // then each one-pass configuration's launch
+    for (int l = 0; l < 16; ++l) {
+        rbs_[l].nbt = srbs[l];
+        for (int k = 0; k < kTbConfigs; ++k) tb_[l][k].nbt = stb[l][k];
+    }
+    {
+        int base = srbs[0];
+        for (int k = 0; k < kTbConfigs; ++k) {
+            tb_base_[k] = base;
+            base += stb[0][k];
+        }
+        maxp = std::max(maxp, (size_t)base);
+        for (const auto& ds : dot_src) {
+            BEntry d = ds.first == 0 ? rbs_[0].h[ds.second] : tb_[0][ds.first - 1].h[ds.second];
+            if (ds.first != 0) d.start += tb_base_[ds.first - 1];
+            dot_.h.push_back(d);
+        }
+    }
     fdm_lds_ = kFdmLdsBytes;
     // pack every table into the staging buffer, then one copy
     size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
                    table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
     for (int p = 0; p < 2; ++p)
         total += table_bytes(div_[p]) + table_bytes(prx_[p]) + table_bytes(pry_[p]) + table_bytes(prz_[p]);
-    for (int l = 0; l + 1 < levels_; ++l) total += table_bytes(rb_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]);
+    for (int l = 0; l + 1 < levels_; ++l) {
+        total += table_bytes(rb_[l]) + table_bytes(rbs_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]);
+        for (int k = 0; k < kTbConfigs; ++k) total += table_bytes(tb_[l][k]);
+    }
+    total += table_bytes(dot_);
     total = std::max<size_t>(total, 256);
     PANO_CHECK(grow(c, (void**)&tab_dev_, &tab_cap_, total));
     if (tab_host_cap_ < total) {
@@ -748,9 +838,12 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     }
     for (int l = 0; l + 1 < levels_; ++l) {
         PANO_CHECK(place(rb_[l]));
+        PANO_CHECK(place(rbs_[l]));
+        for (int k = 0; k < kTbConfigs; ++k) PANO_CHECK(place(tb_[l][k]));
         PANO_CHECK(place(co_[l]));
         PANO_CHECK(place(prl_[l]));
     }
+    PANO_CHECK(place(dot_));
     PANO_CHECK(place(plane_));
     PANO_CHECK(place(zcol_));
     PANO_CHECK(place(rows2_));
@@ -800,19 +893,44 @@ pano_status BatchCore::enqueue_project(int ping) {
 
 // z = M r at level 0 (one V-cycle from zero); with_dot: the last two level-0
 // half-sweeps write the partials of z.r (black into part0_, red into part2_).
+// One-pass smoothing launches of a level (pre: from z = 0; post: from the
+// prolonged z, with the z.r partials when dot) for each configuration present.
+void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
+    const SolverState* st = states;
+    auto go = [&](auto cfg, int k) {
+        using C = decltype(cfg);
+        const Table& t = tb_[l][k];
+        if (t.nbt == 0) return;
+        const dim3 g(t.nbt), b(C::NT);
+        if (!post)
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 0, true, false>), g, b, 0, c->stream,
+                               t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
+        else if (dot)
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, true>), g, b, 0, c->stream,
+                               t.view(), arena, st, part0_, part2_, tb_base_[k]);
+        else
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, false>), g, b, 0, c->stream,
+                               t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
+    };
+    go(TbLarge{}, 0);
+    go(TbMedium{}, 1);
+    go(TbSmall{}, 2);
+}
+
 pano_status BatchCore::vcycle(bool with_dot) {
     const int Lc = levels_ - 1;
     dim3 blk(shapes.rb_x, shapes.rb_y);
     for (int l = 0; l < Lc; ++l) {
+        smooth_tb_level(l, false, false);
         if (split_first_[l]) {  // bandwidth-bound levels: red cells, then the black half-sweep
-            BL(kb_red_init, rb_[l], arena, (const SolverState*)states);
-            BL(kb_smooth_half, rb_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
+            BL(kb_red_init, rbs_[l], arena, (const SolverState*)states);
+            BL(kb_smooth_half, rbs_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
         } else {  // latency-bound levels: one fused pass (same values)
-            BL(kb_smooth_first, rb_[l], arena, (const SolverState*)states);
+            BL(kb_smooth_first, rbs_[l], arena, (const SolverState*)states);
         }
         for (int k = 1; k < nu; ++k) {
-            BL(kb_smooth_half, rb_[l], arena, 0, (double*)nullptr, (const SolverState*)states);
-            BL(kb_smooth_half, rb_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rbs_[l], arena, 0, (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rbs_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
         }
         BL(kb_residual, rb_[l], arena, (const SolverState*)states);
         dim3 sblk = blk;
@@ -841,10 +959,11 @@ pano_status BatchCore::vcycle(bool with_dot) {
     for (int l = Lc - 1; l >= 0; --l) {
         if (l + 1 == Lc) BL(kb_prolong<NATURAL>, prl_[l], arena, (const SolverState*)states);
         else BL(kb_prolong<REDBLACK>, prl_[l], arena, (const SolverState*)states);
+        smooth_tb_level(l, true, l == 0 && with_dot);
         for (int k = 0; k < nu; ++k) {
             bool last = l == 0 && k == nu - 1 && with_dot;
-            BL(kb_smooth_half, rb_[l], arena, 1, last ? part0_ : (double*)nullptr, (const SolverState*)states);
-            BL(kb_smooth_half, rb_[l], arena, 0, last ? part2_ : (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rbs_[l], arena, 1, last ? part0_ : (double*)nullptr, (const SolverState*)states);
+            BL(kb_smooth_half, rbs_[l], arena, 0, last ? part2_ : (double*)nullptr, (const SolverState*)states);
         }
     }
     PANO_HIP(hipGetLastError());
@@ -882,7 +1001,7 @@ pano_status BatchCore::enqueue_iteration() {
     BL(kb_update, flat_, arena, (const SolverState*)states, part1_);
     fin(FIN_RNORM, part1_, nullptr, flat_, 1);
     PANO_CHECK(vcycle(true));
-    fin(FIN_BETA, part0_, part2_, rb_[0], 0);
+    fin(FIN_BETA, part0_, part2_, dot_, 0);
     blk = dim3(kBlock, 1);
     BL(kb_search, flat_, arena, (const SolverState*)states);
     PANO_HIP(hipGetLastError());

@@ -15,6 +15,7 @@
 
 #include "mg.hpp"
 #include "mg_device.hpp"
+#include "mg_tb.hpp"
 #include "solver_device.hpp"
 
 namespace pano {
@@ -29,6 +30,7 @@ struct BEntry {
     int64_t o[4];    // byte offsets in the arena of the fields the launch touches
     double s;        // dt, or the level scale dt/4^l
     float rinv[8];   // smoother reciprocals 1/(s*nc) at this level
+    int zc;          // one-pass smoothing tiles: planes per chunk (G: tiles x, y, chunks)
 };
 
 // One grid's coarsest-level solve (fast diagonalisation).
@@ -144,6 +146,9 @@ private:
     Table prx_[2], pry_[2], prz_[2];  // face tiles of the projection: o[0] = u[ping], o[1] = x
     Table flat_;           // flat vector blocks (2048 cells per block)
     Table rb_[16];         // red-black pair tiles per level
+    Table rbs_[16];        // the same, for the grids smoothed by half-sweeps at the level
+    Table tb_[16][kTbConfigs];  // one-pass smoothing tiles (3D levels, by configuration)
+    Table dot_;            // per grid: where the last post-smoothing left its z.r partials
     Table co_[16];         // coarse-cell tiles of the restriction into level l+1
     Table prl_[16];        // pair tiles of level l for the prolongation from l+1
     FdmTable plane_, zcol_;  // 3D coarsest solve
@@ -151,6 +156,8 @@ private:
     int levels_ = 0;
     bool any3d_ = false;
     bool split_first_[16] = {};  // first sweep as red cells + black half-sweep at this level
+    bool tb_enabled_ = true;      // one-pass smoothing on 3D levels (PANO_MG_FUSED=0 turns it off)
+    int tb_base_[kTbConfigs] = {};  // first partial slot of each configuration's level-0 launch
     size_t fdm_lds_ = 0;
     double* part0_ = nullptr;
     double* part1_ = nullptr;
@@ -167,6 +174,7 @@ private:
     pano_status ensure_partials(size_t n);
     void fin(int op, const double* p0, const double* p1, const Table& t, int use_max);
     pano_status vcycle(bool with_dot);
+    void smooth_tb_level(int l, bool post, bool dot);
 };
 
 }  // namespace pano

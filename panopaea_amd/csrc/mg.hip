@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* z
     if (st->done) return;
     RUN_TILE(G, NZ);
     if (m >= (Df.w >> 1) || y >= Df.h) return;
-    prolong_run<T, DIM, LC, NZ>(Df, Dc, zf, zc, m, y, z0);
+    prolong_run<T, DIM, LC, NZ>(Df, Dc, zf, (const T*)zf, zc, m, y, z0);
 }
 
 // ---- 3D f32 levels: prolongation from LDS ------------------------------------------------
@@ -173,57 +173,22 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
 }
 
 // ---- 3D f32 levels: the pre- and post-smoothing as one pass each ----------------------
-// Configurations by level size (measured on 256^3, 128^3, 64^3): tall tiles
-// on large levels (less halo work; two 768-thread workgroups per CU with 79
-// KiB of LDS and at most 80 VGPRs each), small tiles and short plane chunks on
-// small levels, so every CU still gets workgroups. WANT/ZMIN: workgroups
-// aimed for and the shortest chunk (tb_grid).
-struct TbLarge {  // >= 2^23 cells
-    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 512, ZMIN = 8;
-};
-struct TbMedium {  // >= 2^20 cells
-    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 1024, ZMIN = 4;
-};
-struct TbSmall {
-    static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2;
-};
-
-// One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
-// after each, end.
-template <int J, typename TH, int... H>
-__device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
-    th.template begin_step<J>(t);
-    ((th.template stage<H, J>(t), __syncthreads()), ...);
-    th.template end_step<J>(t);
-}
-// Steps t .. t+SLOTS-1 (up to t1), then t advanced past them.
-template <int NS, typename TH, int... J>
-__device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_sequence<int, J...>) {
-    ((t <= t1 ? (tb_step<J>(th, t, std::make_integer_sequence<int, NS>{}), ++t) : t), ...);
-}
-
 // NS half-sweeps from xin into xout (TbThread); with DOT the workgroup's sums
 // of z.r over its tile for the last two half-sweeps go to part[L] and
 // part[nb + L].
-template <int TXP, int TY, int NT, int WPE, int NS, int CF, bool ZERO_X, bool DOT>
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb(
     Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
     const SolverState* st) {
     if (st->done) return;
-    using TH = TbThread<NS, TXP, TY, NT, CF, ZERO_X, DOT>;
-    __shared__ float lds[TH::S::LDS_FLOATS];
+    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
     const int L = xcd_remap(blockIdx.x, g.nb);
     const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
-    TH th;
-    th.init(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri, threadIdx.x);
-    th.prologue();
-    __syncthreads();
-    const int t1 = th.last_step();
-    int t = th.first_step();
-    while (t <= t1) tb_round<NS>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
+    double acc[2];
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc);
     if constexpr (DOT) {
-        const double a0 = block_sum(th.acc[0]);
-        const double a1 = block_sum(th.acc[1]);
+        const double a0 = block_sum(acc[0]);
+        const double a1 = block_sum(acc[1]);
         if (threadIdx.x == 0) {
             part[L] = a0;
             part[g.nb + L] = a1;
@@ -231,20 +196,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
 }
 
-template <class C>
-static TbGrid tb_grid_of(const Dims& D) {
-    return tb_grid<C::TXP, C::TY>(D, C::WANT, C::ZMIN);
-}
-template <class F>
-static auto with_tb(const Dims& D, F&& f) {
-    const int64_t n = (int64_t)D.d * D.h * D.w;
-    if (n >= (int64_t(1) << 23)) return f(TbLarge{});
-    if (n >= (int64_t(1) << 20)) return f(TbMedium{});
-    return f(TbSmall{});
-}
-static int tb_blocks(const Dims& D) {
-    return with_tb(D, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(D).nb; });
-}
 // The four half-sweeps (colours CF, CF^1, ...) of a 3D f32 level in one pass;
 // with part, the z.r partials (2 per workgroup, *np set).
 template <int CF, bool ZERO_X, bool DOT>
@@ -253,7 +204,7 @@ static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, con
     with_tb(D, [&](auto cfg) {
         using C = decltype(cfg);
         const TbGrid g = tb_grid_of<C>(D);
-        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 4, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT), D, g, xin, xout, r,
+        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT), D, g, xin, xout, r,
                 s, ri, part, st);
         if (np) *np = 2 * g.nb;
         return 0;
@@ -554,7 +505,7 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     // itself moves about 2 (b in, z out).
     bool tb_done = false;
     if constexpr (DIM == 3 && std::is_same<T, float>::value) {
-        if (m->tb && m->nu == 2 && D.d >= 2) {
+        if (tb_on(m->tb, m->nu, D)) {
             smooth_tb<0, true, false>(c, l == 0 ? "smooth_pre_l0" : "", 6 * unit, D, (const float*)zl[l], zl[l],
                                       rl[l], s, ri, (double*)nullptr, (int*)nullptr, st);
             tb_done = true;
@@ -621,7 +572,7 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
         // into z in one pass (out of place: workgroups read their halo from
         // the input while others store). Timed bytes: two red-black sweeps of
         // 3 field passes each; the pass moves about 3.
-        const bool tb = m->tb && m->nu == 2 && D.d >= 2 && tb_blocks(D) <= G.nb;
+        const bool tb = tb_on(m->tb, m->nu, D);
         float* zo = tb ? (float*)m->res : zl[l];
         if (l + 1 == Lc)
             klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zo,

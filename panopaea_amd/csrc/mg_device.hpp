@@ -323,8 +323,9 @@ __device__ __forceinline__ void residual_run(const Dims& D, T* res, const T* zv,
 }
 
 template <typename T, int DIM, int LC, int NZ>
-__device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* zf, const T* zc, int m, int y,
-                                            int z0) {
+// zo = zf + P zc for the x-pair run (zo may be zf)
+__device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* zo, const T* zf, const T* zc, int m,
+                                            int y, int z0) {
     const T w34 = (T)0.75, w14 = (T)0.25, one = (T)1, zero = (T)0;
     int J0, J1;
     bool ey;
@@ -360,8 +361,8 @@ __device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* z
         RBRow R = rb_row<DIM>(Df, z, y, m);
         i0[k] = R.i0;
         i1[k] = R.i1;
-        f0[k] = ldx((const T*)zf, R.i0);
-        f1[k] = ldx((const T*)zf, R.i1);
+        f0[k] = ldx(zf, R.i0);
+        f1[k] = ldx(zf, R.i1);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
@@ -382,8 +383,8 @@ __device__ __forceinline__ void prolong_run(const Dims& Df, const Dims& Dc, T* z
             v0 = wy0 * px0[0][0] + wy1 * px0[0][1];
             v1 = wy0 * px1[0][0] + wy1 * px1[0][1];
         }
-        stx(zf, i0[k], f0[k] + v0);
-        stx(zf, i1[k], f1[k] + v1);
+        stx(zo, i0[k], f0[k] + v0);
+        stx(zo, i1[k], f1[k] + v1);
     }
 }
 

@@ -30,6 +30,8 @@
 // step + half-sweep 0 | ...
 #pragma once
 
+#include <utility>
+
 #include "mg_device.hpp"
 
 namespace pano {
@@ -55,7 +57,7 @@ struct TbGrid {
     int gx, gy, zc, nch, nb;
 };
 template <int TXP, int TY>
-inline TbGrid tb_grid(const Dims& D, int want, int zmin) {
+__host__ __device__ inline TbGrid tb_grid(const Dims& D, int want, int zmin) {
     TbGrid g;
     g.gx = ((D.w >> 1) + TXP - 1) / TXP;
     g.gy = (D.h + TY - 1) / TY;
@@ -107,9 +109,10 @@ struct TbThread {
 
     static constexpr int mod(int a) { return ((a % SLOTS) + SLOTS) % SLOTS; }
 
+    // rinv: the level's reciprocals 1/(s*nc), nc = 0..7 (read here only)
     __host__ __device__ __forceinline__ void init(float* lds_, const Dims& D_, int bx, int by, int bz, int zc,
                                                   const float* xin_, float* xout_, const float* r_, float s_,
-                                                  const RInv<float>& ri, int tid_) {
+                                                  const float* rinv, int tid_) {
         lds = lds_;
         D = D_;
         xin = xin_;
@@ -145,8 +148,8 @@ struct TbThread {
                         a |= 1 << (h + 4 * p);
                 const int hxy = (x > 0) | ((x < D.w - 1) << 1) | ((y > 0) << 2) | ((y < D.h - 1) << 3);
                 const int nxy = __builtin_popcount((unsigned)((hxy | D.bc) & 0xF));
-                rin[k][p] = ri.v[nxy + 2];
-                rbd[k][p] = ri.v[nxy + 1];
+                rin[k][p] = rinv[nxy + 2];
+                rbd[k][p] = rinv[nxy + 1];
             }
             act[k] = a;
         }
@@ -296,5 +299,80 @@ struct TbThread {
         bok = load_b(t + 2, bpf);
     }
 };
+
+// Configurations by level size (measured on 256^3, 128^3, 64^3): tall tiles
+// on large levels (less halo work; two 768-thread workgroups per CU with 79
+// KiB of LDS and at most 80 VGPRs each), small tiles and short plane chunks on
+// small levels, so every CU still gets workgroups. WANT/ZMIN: workgroups
+// aimed for and the shortest chunk (tb_grid).
+struct TbLarge {  // >= 2^23 cells
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 512, ZMIN = 8;
+};
+struct TbMedium {  // >= 2^20 cells
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 1024, ZMIN = 4;
+};
+struct TbSmall {
+    static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2;
+};
+constexpr int kTbConfigs = 3;  // index: 0 large, 1 medium, 2 small
+
+template <class C>
+inline TbGrid tb_grid_of(const Dims& D) {
+    return tb_grid<C::TXP, C::TY>(D, C::WANT, C::ZMIN);
+}
+// Calls f with the configuration of a level of dims D.
+template <class F>
+inline auto with_tb(const Dims& D, F&& f) {
+    const int64_t n = (int64_t)D.d * D.h * D.w;
+    if (n >= (int64_t(1) << 23)) return f(TbLarge{});
+    if (n >= (int64_t(1) << 20)) return f(TbMedium{});
+    return f(TbSmall{});
+}
+inline int tb_config(const Dims& D) {
+    const int64_t n = (int64_t)D.d * D.h * D.w;
+    return n >= (int64_t(1) << 23) ? 0 : (n >= (int64_t(1) << 20) ? 1 : 2);
+}
+inline int tb_blocks(const Dims& D) {
+    return with_tb(D, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(D).nb; });
+}
+// Whether a 3D f32 level smooths in one pass, both legs (the single-grid
+// V-cycle and the batched engine decide alike, so a grid gets the same bits
+// in both): when enabled, with nu = 2, and when the post-smoothing's z.r
+// partials (2 per workgroup) fit the buffer sized for the red-black tiles.
+inline bool tb_on(bool enabled, int nu, const Dims& D) {
+    return enabled && nu == 2 && D.d >= 2 && tb_blocks(D) <= grid3_rb(D).nb;
+}
+
+// One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
+// after each, end.
+template <int J, typename TH, int... H>
+__device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
+    th.template begin_step<J>(t);
+    ((th.template stage<H, J>(t), __syncthreads()), ...);
+    th.template end_step<J>(t);
+}
+// Steps t .. t+SLOTS-1 (up to t1), then t advanced past them.
+template <int NS, typename TH, int... J>
+__device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_sequence<int, J...>) {
+    ((t <= t1 ? (tb_step<J>(th, t, std::make_integer_sequence<int, NS>{}), ++t) : t), ...);
+}
+// One workgroup of the one-pass smoother (four half-sweeps) on tile
+// (bx, by, bz) of a level; acc: its tile's z.r sums of the last two
+// half-sweeps (with DOT).
+template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT>
+__device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, int by, int bz, int zc,
+                                             const float* xin, float* xout, const float* r, float s,
+                                             const float* rinv, double* acc) {
+    using TH = TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT>;
+    TH th;
+    th.init(lds, D, bx, by, bz, zc, xin, xout, r, s, rinv, threadIdx.x);
+    th.prologue();
+    __syncthreads();
+    const int t1 = th.last_step();
+    int t = th.first_step();
+    while (t <= t1) tb_round<4>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
+    acc[0] = th.acc[0];
+    acc[1] = th.acc[1];
+}
 
 }  // namespace pano
