@@ -334,6 +334,27 @@ __global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
     });
 }
 
+// the 2D form over tb_[l][3]: tile (bx, by) of E.G, fields as below
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb2(
+    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base) {
+    BATCH_ENTRY(T);
+    if (st[E.sidx].done) return;
+    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>::LDS_FLOATS];
+    const int bx = lb % E.G.gx, by = lb / E.G.gx;
+    double acc[2];
+    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, E.D, bx, by, fld<float>(A, E.o[0]), fld<float>(A, E.o[1]),
+                                                fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc);
+    if constexpr (DOT) {
+        const double a0 = block_sum(acc[0]);
+        const double a1 = block_sum(acc[1]);
+        if (threadIdx.x == 0) {
+            p0[base + b_] = a0;
+            p2[base + b_] = a1;
+        }
+    }
+}
+
 // one-pass smoothing (the four half-sweeps of a leg) over tb_[l][config]:
 // o[0] = input z (post: the prolonged z in the residual scratch; pre: unused,
 // the sweeps start from 0), o[1] = z_l (output), o[2] = r_l. Tile (bx, by, bz)
@@ -700,15 +721,13 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 if (tb) {
                     const int k = tb_config(L);
                     BEntry t = r;
-                    with_tb(L, [&](auto cfg) {
-                        const TbGrid tg = tb_grid_of<decltype(cfg)>(L);
-                        t.G.gx = tg.gx;
-                        t.G.gy = tg.gy;
-                        t.G.gz = tg.nch;
-                        t.G.nb = tg.nb;
-                        t.zc = tg.zc;
-                        return 0;
-                    });
+                    const TbGrid tg = L.d <= 1 ? tb2_grid(L)
+                                               : with_tb(L, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(L); });
+                    t.G.gx = tg.gx;
+                    t.G.gy = tg.gy;
+                    t.G.gz = tg.nch;
+                    t.G.nb = tg.nb;
+                    t.zc = tg.zc;
                     t.start = stb[l][k];
                     stb[l][k] += t.G.nb;
                     t.o[0] = g.o_res;
@@ -915,6 +934,20 @@ void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
     go(TbLarge{}, 0);
     go(TbMedium{}, 1);
     go(TbSmall{}, 2);
+    const Table& t2 = tb_[l][3];  // 2D grids
+    if (t2.nbt > 0) {
+        using C = Tb2D;
+        const dim3 g(t2.nbt), b(C::NT);
+        if (!post)
+            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 0, true, false>), g, b, 0, c->stream,
+                               t2.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
+        else if (dot)
+            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 1, false, true>), g, b, 0, c->stream,
+                               t2.view(), arena, st, part0_, part2_, tb_base_[3]);
+        else
+            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 1, false, false>), g, b, 0, c->stream,
+                               t2.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
+    }
 }
 
 pano_status BatchCore::vcycle(bool with_dot) {

@@ -86,14 +86,14 @@ __global__ __launch_bounds__(256) void k_restrict(Dims Df, Dims Dc, Grid3 G, T* 
     restrict_run<T, DIM, LC, NK>(Df, Dc, rc, res, I, J, kb * NK);
 }
 
-// z_f += P z_c
+// z_o = z_f + P z_c (z_o may be z_f)
 template <typename T, int DIM, int LC, int NZ>
-__global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* zf, const T* zc,
+__global__ __launch_bounds__(256) void k_prolong(Dims Df, Dims Dc, Grid3 G, T* zo, const T* zf, const T* zc,
                                                  const SolverState* st) {
     if (st->done) return;
     RUN_TILE(G, NZ);
     if (m >= (Df.w >> 1) || y >= Df.h) return;
-    prolong_run<T, DIM, LC, NZ>(Df, Dc, zf, (const T*)zf, zc, m, y, z0);
+    prolong_run<T, DIM, LC, NZ>(Df, Dc, zo, zf, zc, m, y, z0);
 }
 
 // ---- 3D f32 levels: prolongation from LDS ------------------------------------------------
@@ -196,11 +196,40 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
 }
 
+// The 2D form: one workgroup per tile of a 2D level.
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb2(
+    Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
+    const SolverState* st) {
+    if (st->done) return;
+    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>::LDS_FLOATS];
+    const int L = xcd_remap(blockIdx.x, g.nb);
+    const int bx = L % g.gx, by = L / g.gx;
+    double acc[2];
+    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, D, bx, by, xin, xout, r, s, ri.v, acc);
+    if constexpr (DOT) {
+        const double a0 = block_sum(acc[0]);
+        const double a1 = block_sum(acc[1]);
+        if (threadIdx.x == 0) {
+            part[L] = a0;
+            part[g.nb + L] = a1;
+        }
+    }
+}
+
 // The four half-sweeps (colours CF, CF^1, ...) of a 3D f32 level in one pass;
 // with part, the z.r partials (2 per workgroup, *np set).
 template <int CF, bool ZERO_X, bool DOT>
 static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, const float* xin, float* xout,
                       const float* r, float s, const RInv<float>& ri, double* part, int* np, const SolverState* st) {
+    if (D.d <= 1) {
+        using C = Tb2D;
+        const TbGrid g = tb2_grid(D);
+        klaunch(c, name, bytes, k_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT),
+                D, g, xin, xout, r, s, ri, part, st);
+        if (np) *np = 2 * g.nb;
+        return;
+    }
     with_tb(D, [&](auto cfg) {
         using C = decltype(cfg);
         const TbGrid g = tb_grid_of<C>(D);
@@ -504,7 +533,7 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     // bytes of the work, two red-black sweeps of 3 field passes each; the pass
     // itself moves about 2 (b in, z out).
     bool tb_done = false;
-    if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+    if constexpr (std::is_same<T, float>::value) {
         if (tb_on(m->tb, m->nu, D)) {
             smooth_tb<0, true, false>(c, l == 0 ? "smooth_pre_l0" : "", 6 * unit, D, (const float*)zl[l], zl[l],
                                       rl[l], s, ri, (double*)nullptr, (int*)nullptr, st);
@@ -566,20 +595,29 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
     const double unit = (double)D.d * D.h * D.w * sizeof(T);
     const char* nh = l == 0 ? "smooth_l0" : "";
     const char* np = l == 0 ? "prolong_l0" : "";
+    // f32, nu = 2: prolongation into the residual scratch (free on the up
+    // leg), then the four post-smoothing half-sweeps from there back into z in
+    // one pass (out of place: workgroups read their halo from the input while
+    // others store). Timed bytes: two red-black sweeps of 3 field passes each;
+    // the pass moves about 3.
+    bool tb = false;
+    if constexpr (std::is_same<T, float>::value) tb = tb_on(m->tb, m->nu, D);
+    T* zo = tb ? m->res : zl[l];
     if constexpr (DIM == 3 && std::is_same<T, float>::value) {
-        // 3D f32, nu = 2: prolongation into the residual scratch (free on the
-        // up leg), then the four post-smoothing half-sweeps from there back
-        // into z in one pass (out of place: workgroups read their halo from
-        // the input while others store). Timed bytes: two red-black sweeps of
-        // 3 field passes each; the pass moves about 3.
-        const bool tb = tb_on(m->tb, m->nu, D);
-        float* zo = tb ? (float*)m->res : zl[l];
         if (l + 1 == Lc)
             klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zo,
                     (const T*)zl[l], (const T*)m->z[l + 1], st);
         else
             klaunch(c, np, (2 + cfrac) * unit, k_prolong3_lds<REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zo,
                     (const T*)zl[l], (const T*)m->z[l + 1], st);
+    } else if (l + 1 == Lc) {
+        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zo,
+                (const T*)zl[l], (const T*)m->z[l + 1], st);
+    } else {
+        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zo,
+                (const T*)zl[l], (const T*)m->z[l + 1], st);
+    }
+    if constexpr (std::is_same<T, float>::value) {
         if (tb) {
             const char* npost = l == 0 ? "smooth_post_l0" : "";
             if (l == 0 && dot_part)
@@ -590,12 +628,6 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
                                            (double*)nullptr, (int*)nullptr, st);
             return;
         }
-    } else if (l + 1 == Lc) {
-        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, NATURAL, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
-                (const T*)m->z[l + 1], st);
-    } else {
-        klaunch(c, np, (2 + cfrac) * unit, k_prolong<T, DIM, REDBLACK, NZ>, dim3(G.nb), B, D, Dc, G, zl[l],
-                (const T*)m->z[l + 1], st);
     }
     for (int k = 0; k < m->nu; ++k) {
         bool last = l == 0 && k == m->nu - 1 && dot_part;

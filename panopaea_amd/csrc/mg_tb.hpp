@@ -300,6 +300,122 @@ struct TbThread {
     }
 };
 
+// The 2D form: one plane, so a workgroup loads its tile + halo (both colours)
+// into LDS once, runs the NS half-sweeps there (a barrier after each) and
+// stores the tile. Same positions, activity tables and arithmetic as the 3D
+// form with four neighbours (nsum over x-, x+, y-, y+). Every cell of a 2D
+// level with an even width has an x neighbour, so nc >= 1.
+template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT>
+struct Tb2Thread {
+    using S = TbShape<NS, TXP, TY, NTHREADS>;
+    static constexpr int PW = S::PW, PH = S::PH, PLANE = S::PLANE, NT = S::NT, KC = S::KC, KP = S::KP, HP = S::HP,
+                         CPOS = S::CPOS;
+    static constexpr int LDS_FLOATS = 2 * PLANE;
+
+    float* lds;
+    Dims D;
+    float s;
+    int tid, n2;
+    int e_[KP];
+    unsigned goff[KP];
+    int act[KP];
+    bool ind[KP], tile[KP];
+    float rin[KP][2];
+    float bv[2][KP];
+    double acc[2];
+
+    __host__ __device__ __forceinline__ void init(float* lds_, const Dims& D_, int bx, int by, const float* xin,
+                                                  const float* r, float s_, const float* rinv, int tid_) {
+        lds = lds_;
+        D = D_;
+        s = s_;
+        tid = tid_;
+        const int wh = D.w >> 1;
+        n2 = D.h * wh;
+        const int m0 = bx * TXP, y0 = by * TY;
+        acc[0] = acc[1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int q = k / KC, f = tid + (k % KC) * NT;
+            const bool valid = f < CPOS;
+            const int ff = valid ? f : 0;
+            const int row = 2 * (ff / PW) + q, pr = ff % PW;
+            const int y = y0 - NS + row, m = m0 - HP + pr;
+            e_[k] = row * PW + pr;
+            ind[k] = valid && y >= 0 && y < D.h && m >= 0 && m < wh;
+            tile[k] = ind[k] && row >= NS && row < NS + TY && pr >= HP && pr < HP + TXP;
+            goff[k] = ind[k] ? (unsigned)(y * wh + m) : 0u;
+            int a = 0;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int xl = 2 * pr + p, x = 2 * m + p;
+                for (int h = 0; h < NS; ++h)
+                    if (ind[k] && row >= h + 1 && row < PH - 1 - h && xl >= h + 1 && xl < 2 * PW - 1 - h)
+                        a |= 1 << (h + 4 * p);
+                const int hxy = (x > 0) | ((x < D.w - 1) << 1) | ((y > 0) << 2) | ((y < D.h - 1) << 3);
+                rin[k][p] = rinv[__builtin_popcount((unsigned)((hxy | D.bc) & 0xF))];
+            }
+            act[k] = a;
+        }
+        // all loads first (clamped, valid addresses), then the LDS fill and b
+        float xv[2][KP], b0[2][KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            xv[0][k] = xv[1][k] = 0.0f;
+            if (!ZERO_X) {
+                xv[0][k] = xin[goff[k]];
+                xv[1][k] = xin[n2 + goff[k]];
+            }
+            b0[0][k] = r[goff[k]];
+            b0[1][k] = r[n2 + goff[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const bool ok = ind[k];
+            if ((k % KC) * NT + tid < CPOS) {
+                lds[e_[k]] = ok ? xv[0][k] : 0.0f;
+                lds[PLANE + e_[k]] = ok ? xv[1][k] : 0.0f;
+            }
+            bv[0][k] = ok ? b0[0][k] : 0.0f;
+            bv[1][k] = ok ? b0[1][k] : 0.0f;
+        }
+    }
+    // half-sweep H (colour (CF + H) & 1)
+    template <int H>
+    __host__ __device__ __forceinline__ void stage() {
+        constexpr int c = (CF + H) & 1;
+        const float* xo = lds + (1 - c) * PLANE;
+        float* xc = lds + c * PLANE;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int p = (c + k / KC) & 1;  // x parity of the colour-c cell (row parity k / KC)
+            if (act[k] & (1 << (H + 4 * p))) {
+                const int e = e_[k];
+                float n[6];
+                n[0] = xo[e - 1 + p];  // x-1
+                n[1] = xo[e + p];      // x+1
+                n[2] = xo[e - PW];     // y-1
+                n[3] = xo[e + PW];     // y+1
+                n[4] = n[5] = 0.0f;
+                const float rv = bv[c][k];
+                const float a0 = tb_opaque(rin[k][0]), a1 = tb_opaque(rin[k][1]);
+                const float v = (rv + s * nsum<float, 2>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                xc[e] = v;
+                if (DOT && H >= NS - 2 && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
+            }
+        }
+    }
+    // the tile's cells, both colours
+    __host__ __device__ __forceinline__ void store(float* xout) const {
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+            if (tile[k]) {
+                xout[goff[k]] = lds[e_[k]];
+                xout[n2 + goff[k]] = lds[PLANE + e_[k]];
+            }
+    }
+};
+
 // Configurations by level size (measured on 256^3, 128^3, 64^3): tall tiles
 // on large levels (less halo work; two 768-thread workgroups per CU with 79
 // KiB of LDS and at most 80 VGPRs each), small tiles and short plane chunks on
@@ -314,7 +430,11 @@ struct TbMedium {  // >= 2^20 cells
 struct TbSmall {
     static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2;
 };
-constexpr int kTbConfigs = 3;  // index: 0 large, 1 medium, 2 small
+// 2D levels: 32 x-pairs by 32 rows, one plane per workgroup
+struct Tb2D {
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 8;
+};
+constexpr int kTbConfigs = 4;  // index: 0 large, 1 medium, 2 small (3D), 3 2D
 
 template <class C>
 inline TbGrid tb_grid_of(const Dims& D) {
@@ -329,18 +449,30 @@ inline auto with_tb(const Dims& D, F&& f) {
     return f(TbSmall{});
 }
 inline int tb_config(const Dims& D) {
+    if (D.d <= 1) return 3;
     const int64_t n = (int64_t)D.d * D.h * D.w;
     return n >= (int64_t(1) << 23) ? 0 : (n >= (int64_t(1) << 20) ? 1 : 2);
 }
+// 2D tiles of a level: x-pair tiles by row tiles (TbGrid with one chunk)
+inline TbGrid tb2_grid(const Dims& D) {
+    TbGrid g;
+    g.gx = ((D.w >> 1) + Tb2D::TXP - 1) / Tb2D::TXP;
+    g.gy = (D.h + Tb2D::TY - 1) / Tb2D::TY;
+    g.zc = 1;
+    g.nch = 1;
+    g.nb = g.gx * g.gy;
+    return g;
+}
 inline int tb_blocks(const Dims& D) {
+    if (D.d <= 1) return tb2_grid(D).nb;
     return with_tb(D, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(D).nb; });
 }
-// Whether a 3D f32 level smooths in one pass, both legs (the single-grid
+// Whether an f32 level smooths in one pass, both legs (the single-grid
 // V-cycle and the batched engine decide alike, so a grid gets the same bits
 // in both): when enabled, with nu = 2, and when the post-smoothing's z.r
 // partials (2 per workgroup) fit the buffer sized for the red-black tiles.
 inline bool tb_on(bool enabled, int nu, const Dims& D) {
-    return enabled && nu == 2 && D.d >= 2 && tb_blocks(D) <= grid3_rb(D).nb;
+    return enabled && nu == 2 && tb_blocks(D) <= grid3_rb(D).nb;
 }
 
 // One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
@@ -371,6 +503,27 @@ __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, 
     const int t1 = th.last_step();
     int t = th.first_step();
     while (t <= t1) tb_round<4>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
+    acc[0] = th.acc[0];
+    acc[1] = th.acc[1];
+}
+
+// One workgroup of the 2D one-pass smoother on tile (bx, by) of a level.
+template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT>
+__device__ __forceinline__ void tb2_workgroup(float* lds, const Dims& D, int bx, int by, const float* xin, float* xout,
+                                              const float* r, float s, const float* rinv, double* acc) {
+    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>;
+    TH th;
+    th.init(lds, D, bx, by, xin, r, s, rinv, threadIdx.x);
+    __syncthreads();
+    th.template stage<0>();
+    __syncthreads();
+    th.template stage<1>();
+    __syncthreads();
+    th.template stage<2>();
+    __syncthreads();
+    th.template stage<3>();
+    __syncthreads();
+    th.store(xout);
     acc[0] = th.acc[0];
     acc[1] = th.acc[1];
 }

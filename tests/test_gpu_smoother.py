@@ -1,4 +1,4 @@
-"""The one-pass pre/post-smoothing of 3D f32 multigrid levels against the
+"""The one-pass pre/post-smoothing of f32 multigrid levels (3D and 2D) against the
 half-sweep kernels (PANO_MG_FUSED=0): both give the same V-cycle bit for bit
 (each cell is computed from the same six neighbour values in the same order).
 The PCG's z.r partial sums are added in another order, so solves agree to
@@ -12,9 +12,11 @@ import panopaea_amd as pa
 pytestmark = pytest.mark.gpu
 
 CASES = [((16, 16, 16), (0,) * 6, 3),                 # one tile, two plane chunks
+         ((96, 128), (0, 1, 0, 0, 0, 0), 4),          # 2D: one plane per workgroup, open x+ side
+         ((40, 56), (0,) * 6, 3),                     # 2D, ragged tiles
          ((24, 40, 64), (0, 0, 0, 0, 0, 1), 3),       # open top, ragged tiles in y
          ((40, 48, 96), (0, 1, 0, 0, 1, 1), 4),       # ragged plane chunks on levels 0 and 1
-         ((256, 256, 256), (0, 0, 0, 0, 0, 1), 4)]    # the 256^3 smoke-plume shape (32-plane chunks)
+         ((256, 256, 256), (0, 0, 0, 0, 0, 1), 4)]    # the 256^3 smoke-plume shape (16-plane chunks)
 
 
 def vcycle(g, r, levels, fused, monkeypatch):
@@ -30,7 +32,7 @@ def vcycle(g, r, levels, fused, monkeypatch):
 @pytest.mark.parametrize("dims,bc,levels", CASES)
 def test_gpu_fused_smoothing_bitwise(ctx, dims, bc, levels, monkeypatch):
     print("fused smoothing", dims, bc, levels, flush=True)
-    g = pa.Grid3d(dims, np.float32, bc=bc)
+    g = (pa.Grid2d if len(dims) == 2 else pa.Grid3d)(dims, np.float32, bc=bc)
     r = np.random.default_rng(dims[0]).standard_normal(g.num_elem_2()).astype(np.float32)
     if not any(bc):
         r -= r.mean(dtype=np.float64).astype(np.float32)
@@ -40,10 +42,10 @@ def test_gpu_fused_smoothing_bitwise(ctx, dims, bc, levels, monkeypatch):
     assert np.array_equal(zf, zu)
 
 
-@pytest.mark.parametrize("dims,bc,levels", CASES[1:3])
+@pytest.mark.parametrize("dims,bc,levels", CASES[1:5])
 def test_gpu_fused_smoothing_pcg(ctx, dims, bc, levels, monkeypatch):
     print("fused smoothing pcg", dims, bc, levels, flush=True)
-    g = pa.Grid3d(dims, np.float32, bc=bc)
+    g = (pa.Grid2d if len(dims) == 2 else pa.Grid3d)(dims, np.float32, bc=bc)
     faces = pa.gen_bandlimited_faces(g, 5, 16, 8.0)
     u = g.new_simplex_1().upload(faces)
     b = g.new_simplex_2()
