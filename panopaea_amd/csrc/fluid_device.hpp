@@ -90,15 +90,42 @@ __device__ __forceinline__ int fx_idx(const Dims& D, int z, int y, int x) {
 // offset (fewer than 2^30 elements), with no per-access 64-bit address
 // arithmetic; f64 indexes normally.
 template <typename T>
-__device__ __forceinline__ T ldx(const T* p, int i) {
+__host__ __device__ __forceinline__ T ldx(const T* p, int i) {
     if constexpr (sizeof(T) == 4) return *(const T*)((const char*)p + ((unsigned)i << 2));
     else return p[i];
 }
 template <typename T>
-__device__ __forceinline__ void stx(T* p, int i, T v) {
+__host__ __device__ __forceinline__ void stx(T* p, int i, T v) {
     if constexpr (sizeof(T) == 4) *(T*)((char*)p + ((unsigned)i << 2)) = v;
     else p[i] = v;
 }
+
+// f32 stores that are always issued: a raw buffer store into a range of
+// `n` elements (the buffer resource's size), where a lane that must not
+// write passes an offset past the range and the hardware drops its store.
+// In a loop that also loads, a store under a branch is skipped on some
+// paths, so a later wait for a load issued before the store cannot tell
+// how many operations follow it and waits for every store in flight; an
+// always-issued store keeps such waits exact. Host builds (CPU emulation
+// of the kernels) store under the condition.
+#if defined(__HIP_DEVICE_COMPILE__)
+using OutRange = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ OutRange out_range(float* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, n << 2, 0x00020000);  // raw, 32-bit data format
+}
+__device__ __forceinline__ void st_if(OutRange o, int i, float v, bool on) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), o, on ? (unsigned)i << 2 : 0x7FFFFFF0u, 0, 0);
+}
+#else
+struct OutRange {
+    float* p;
+    int n;
+};
+inline OutRange out_range(float* p, int n) { return OutRange{p, n}; }
+inline void st_if(OutRange o, int i, float v, bool on) {
+    if (on && i >= 0 && i < o.n) o.p[i] = v;
+}
+#endif
 
 // ---- density source sphere ----------------------------------------------------------
 struct Sphere {

@@ -19,6 +19,7 @@
 
 #include "mg.hpp"
 #include "mg_device.hpp"
+#include "mg_rr.hpp"
 #include "mg_tb.hpp"
 
 namespace pano {
@@ -238,6 +239,34 @@ static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, con
         if (np) *np = 2 * g.nb;
         return 0;
     });
+}
+
+// Tile shape, plane chunks and load-ahead depth of the one-pass residual +
+// restriction. It streams plane by plane, so a workgroup's time is a chain of
+// load latencies: on levels below 2^23 cells the two separate kernels (more
+// workgroups, no chain) are faster.
+constexpr int kRrTX = 32, kRrTY = 8, kRrWant = 1024, kRrKmin = 2, kRrPD = 1;
+inline bool rr_on(bool enabled, const Dims& D) {
+    return enabled && D.d > 1 && (int64_t)D.d * D.h * D.w >= (int64_t(1) << 23);
+}
+
+// r_c = R (r - A z) of a 3D f32 level in one pass (no fine residual in HBM).
+template <int TXC, int TYC, int LC>
+__global__ __launch_bounds__(TXC * TYC) void k_resrestrict(Dims D, Dims Dc, RrGrid g, const float* z, const float* b,
+                                                           float* rc, float s, const SolverState* st) {
+    if (st->done) return;
+    __shared__ float lds[RrShape<TXC, TYC>::LDS_FLOATS];
+    const int L = xcd_remap(blockIdx.x, g.nb);
+    const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
+    rr_workgroup<TXC, TYC, LC, kRrPD>(lds, D, Dc, bx, by, bz, g.kc, z, b, rc, s);
+}
+
+template <int LC>
+static void resrestrict(Ctx* c, const char* name, double bytes, const Dims& D, const Dims& Dc, const float* z,
+                        const float* b, float* rc, float s, const SolverState* st) {
+    const RrGrid g = rr_grid<kRrTX, kRrTY>(Dc, kRrWant, kRrKmin);
+    klaunch(c, name, bytes, k_resrestrict<kRrTX, kRrTY, LC>, dim3(g.nb), dim3(kRrTX * kRrTY), D, Dc, g, z, b, rc, s,
+            st);
 }
 
 // ---- coarsest level: fast diagonalisation in FP64 (natural layout) ---------------------
@@ -557,6 +586,17 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
                 (double*)nullptr, st);
         klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
                 (double*)nullptr, st);
+    }
+    // 3D f32: residual and restriction in one pass (bytes: z and b in, r_c out)
+    if constexpr (std::is_same<T, float>::value && DIM == 3) {
+        if (rr_on(m->tb, D)) {
+            const char* nrr = l == 0 ? "resrestrict_l0" : "";
+            if (l + 1 == Lc)
+                resrestrict<NATURAL>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
+            else
+                resrestrict<REDBLACK>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
+            return;
+        }
     }
     klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM, NZ>, dim3(G.nb), B, D, G, m->res, zl[l],
             rl[l], s, st);

@@ -42,7 +42,7 @@ __host__ __device__ __forceinline__ T gs_value(T r, T s, T sum, int nc, const RI
 // 1D restriction weights: (2I-1)/8, (2I)*3/8, (2I+1)*3/8, (2I+2)/8, a missing
 // outer term moving its weight onto the boundary cell (3/8 -> 1/2).
 template <typename T>
-__device__ __forceinline__ T r1(T vm, T v0, T v1, T vp, bool hm, bool hp) {
+__host__ __device__ __forceinline__ T r1(T vm, T v0, T v1, T vp, bool hm, bool hp) {
     const T e8 = (T)0.125, t8 = (T)0.375, h2 = (T)0.5;
     T a = hm ? vm * e8 : (T)0;
     T b = v0 * (hm ? t8 : h2);
@@ -53,7 +53,7 @@ __device__ __forceinline__ T r1(T vm, T v0, T v1, T vp, bool hm, bool hp) {
 
 // Coarse-level index in natural or red-black layout.
 template <int LC>
-__device__ __forceinline__ int cidx(const Dims& Dc, int K, int J, int I) {
+__host__ __device__ __forceinline__ int cidx(const Dims& Dc, int K, int J, int I) {
     if constexpr (LC == REDBLACK) {
         int wh = Dc.w >> 1, hw = Dc.h * wh;
         return ((I + J + K) & 1) * (Dc.d * hw) + K * hw + J * wh + (I >> 1);

@@ -1,6 +1,8 @@
-"""The one-pass pre/post-smoothing of f32 multigrid levels (3D and 2D) against the
-half-sweep kernels (PANO_MG_FUSED=0): both give the same V-cycle bit for bit
-(each cell is computed from the same six neighbour values in the same order).
+"""The one-pass pre/post-smoothing of f32 multigrid levels (3D and 2D), and the
+one-pass residual + restriction of 3D levels of 2^23 cells and more, against the
+half-sweep, residual and restriction kernels (PANO_MG_FUSED=0): both give the
+same V-cycle bit for bit (each cell is computed from the same six neighbour
+values in the same order, each coarse value from the same 64 residuals).
 The PCG's z.r partial sums are added in another order, so solves agree to
 rounding and take the same number of iterations.
 """
@@ -16,7 +18,8 @@ CASES = [((16, 16, 16), (0,) * 6, 3),                 # one tile, two plane chun
          ((40, 56), (0,) * 6, 3),                     # 2D, ragged tiles
          ((24, 40, 64), (0, 0, 0, 0, 0, 1), 3),       # open top, ragged tiles in y
          ((40, 48, 96), (0, 1, 0, 0, 1, 1), 4),       # ragged plane chunks on levels 0 and 1
-         ((256, 256, 256), (0, 0, 0, 0, 0, 1), 4)]    # the 256^3 smoke-plume shape (16-plane chunks)
+         ((256, 256, 256), (0, 0, 0, 0, 0, 1), 4),    # the 256^3 smoke-plume shape (16-plane chunks)
+         ((128, 280, 240), (1, 0, 0, 1, 0, 1), 4)]    # one-pass residual + restriction, ragged x and y tiles
 
 
 def vcycle(g, r, levels, fused, monkeypatch):
