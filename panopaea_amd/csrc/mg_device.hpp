@@ -402,28 +402,74 @@ __device__ __forceinline__ const double* stage_basis(double* dst, int used, cons
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) dst[used + e] = V[e];
     return dst + used;
 }
+// The transpose of V (W[q*n+i] = V[i*n+q]) into `dst` beside `used` doubles,
+// or nullptr when it does not fit. An inverse transform reads V[i*n+q] with i
+// varying across the lanes, a stride of n doubles that maps the lanes to a few
+// LDS banks; the same sum read from W has consecutive lanes on consecutive
+// entries, and the same products in the same order.
+__device__ __forceinline__ const double* stage_basis_t(double* dst, int used, const double* V, int n) {
+    if (used + n * n > kFdmVStage) return nullptr;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int q = e / n, i = e - q * n;
+        dst[used + e] = V[i * n + q];
+    }
+    return dst + used;
+}
 
 // Transform the plane in `a` (ny rows of nx) along x (axis 0) or y (axis 1),
-// forward (coefficients = V^T v) or inverse (values = V c), into `b`.
+// forward (coefficients = V^T v) or inverse (values = V c), into `b`. Every
+// output is the sum over q = 0 .. n-1, in that order from 0, of one basis
+// entry times one plane entry. A thread computes the outputs of kAxisR
+// consecutive rows (axis 0) or coefficients (axis 1) of one column i: per q
+// it reads the entry those outputs share once and kAxisR entries common to
+// the lanes of its row block, and keeps kAxisR independent sums in flight.
+template <int R>
+__device__ __forceinline__ void lds_axis_r(const double* a, double* b, int nx, int ny, int axis, int inverse,
+                                           const double* V) {
+    const int nb = (ny + R - 1) / R;  // row / coefficient blocks
+    const int n = axis == 0 ? nx : ny;
+    for (int e = threadIdx.x; e < nx * nb; e += blockDim.x) {
+        const int jb = e / nx, i = e - jb * nx, j0 = jb * R;
+        double acc[R];
+        int jr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            acc[r] = 0.0;
+            jr[r] = j0 + r < ny ? j0 + r : ny - 1;  // rows past the plane: computed, not stored
+        }
+        if (axis == 0) {
+            // out[j][i] = sum_q W(q, i) a[j][q], W(q, i) = V[q*nx+i] (forward) or V[i*nx+q] (inverse)
+            const int ws = inverse ? 1 : nx, wb = inverse ? i * nx : i;
+#pragma unroll 4
+            for (int q = 0; q < n; ++q) {
+                const double w = V[wb + q * ws];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += w * a[jr[r] * nx + q];
+            }
+        } else {
+            // out[j][i] = sum_q W(q, j) a[q][i], W(q, j) = V[q*ny+j] (forward) or V[j*ny+q] (inverse)
+            const int ws = inverse ? 1 : ny;
+#pragma unroll 4
+            for (int q = 0; q < n; ++q) {
+                const double v = a[q * nx + i];
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += V[(inverse ? jr[r] * ny : jr[r]) + q * ws] * v;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (j0 + r < ny) b[(j0 + r) * nx + i] = acc[r];
+    }
+}
+
+// Block size by plane size: one output per thread while the plane has at
+// most one per thread of a 1024-thread workgroup, more on larger planes.
 __device__ __forceinline__ void lds_axis(const double* a, double* b, int nx, int ny, int axis, int inverse,
                                          const double* V) {
-    int P = nx * ny;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-        int j = e / nx, i = e - j * nx;
-        double acc = 0.0;
-        if (axis == 0) {
-            if (inverse)
-                for (int k = 0; k < nx; ++k) acc += V[i * nx + k] * a[j * nx + k];
-            else
-                for (int q = 0; q < nx; ++q) acc += V[q * nx + i] * a[j * nx + q];
-        } else {
-            if (inverse)
-                for (int k = 0; k < ny; ++k) acc += V[j * ny + k] * a[k * nx + i];
-            else
-                for (int q = 0; q < ny; ++q) acc += V[q * ny + j] * a[q * nx + i];
-        }
-        b[e] = acc;
-    }
+    const int P = nx * ny;
+    if (P > 2048) lds_axis_r<4>(a, b, nx, ny, axis, inverse, V);
+    else if (P > 1024) lds_axis_r<2>(a, b, nx, ny, axis, inverse, V);
+    else lds_axis_r<1>(a, b, nx, ny, axis, inverse, V);
 }
 
 // One z-plane of the coarsest level (lds: 2*nx*ny doubles):
@@ -439,13 +485,20 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
     int P = nx * ny;
     for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = (double)src[e];
     double* vs = lds + 2 * kFdmPlane;
+    const double* Vxg = Vx;
     Vx = stage_basis(vs, 0, Vx, nx);
-    Vy = stage_basis(vs, Vx == vs ? nx * nx : 0, Vy, ny);
+    int used = Vx == vs ? nx * nx : 0;
+    const double* Vys = stage_basis(vs, used, Vy, ny);
+    used += Vys != Vy ? ny * ny : 0;
+    Vy = Vys;
+    // inverse x from the staged transpose when it fits (the forward form over W)
+    const double* Wx = mode != 0 ? stage_basis_t(vs, used, Vxg, nx) : nullptr;
     __syncthreads();
     if (mode == 1) {
         lds_axis(a, b, nx, ny, 1, 1, Vy);
         __syncthreads();
-        lds_axis(b, a, nx, ny, 0, 1, Vx);
+        if (Wx) lds_axis(b, a, nx, ny, 0, 0, Wx);
+        else lds_axis(b, a, nx, ny, 0, 1, Vx);
     } else {
         lds_axis(a, b, nx, ny, 0, 0, Vx);
         __syncthreads();
@@ -460,7 +513,8 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
             __syncthreads();
             lds_axis(a, b, nx, ny, 1, 1, Vy);
             __syncthreads();
-            lds_axis(b, a, nx, ny, 0, 1, Vx);
+            if (Wx) lds_axis(b, a, nx, ny, 0, 0, Wx);
+            else lds_axis(b, a, nx, ny, 0, 1, Vx);
         }
     }
     __syncthreads();
@@ -486,13 +540,17 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     const int nr = min(R, ny - j0);
     const int P = nr * nx;
     for (int e = threadIdx.x; e < P; e += blockDim.x) lds[e] = (double)src[(int64_t)j0 * nx + e];
-    Vx = stage_basis(lds + 2 * kFdmPlane, 0, Vx, nx);
+    // inverse: the staged transpose W (W[k*nx+i] = Vx[i*nx+k]) when it fits
+    const double* W = inverse ? stage_basis_t(lds + 2 * kFdmPlane, 0, Vx, nx) : nullptr;
+    if (!W) Vx = stage_basis(lds + 2 * kFdmPlane, 0, Vx, nx);
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int jl = e / nx, i = e - jl * nx;
         const double* a = lds + jl * nx;
         double acc = 0.0;
-        if (inverse)
+        if (W)
+            for (int k = 0; k < nx; ++k) acc += W[k * nx + i] * a[k];
+        else if (inverse)
             for (int k = 0; k < nx; ++k) acc += Vx[i * nx + k] * a[k];
         else
             for (int q = 0; q < nx; ++q) acc += Vx[q * nx + i] * a[q];
