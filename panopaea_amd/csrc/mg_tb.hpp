@@ -49,6 +49,22 @@ struct TbShape {
     static constexpr int KC = (CPOS + NT - 1) / NT;  // position slots per thread and parity
     static constexpr int KP = 2 * KC;
     static constexpr int LDS_FLOATS = SLOTS * 2 * PLANE;
+    // Position f of a row parity -> (row pair rp, pair pr). The first
+    // (PH/2)*TXP positions are the tile's own pairs, TXP consecutive ones per
+    // row pair, so with TXP = 32 every half-wave reads and writes 32
+    // consecutive LDS words (32 distinct banks) and global addresses; the
+    // 2*HP halo pairs of each row pair follow.
+    static constexpr int MAINP = (PH / 2) * TXP;
+    __host__ __device__ static void pos_of(int f, int& rp, int& pr) {
+        if (f < MAINP) {
+            rp = f / TXP;
+            pr = HP + f % TXP;
+        } else {
+            const int g = f - MAINP, j = g % (2 * HP);
+            rp = g / (2 * HP);
+            pr = j < HP ? j : TXP + j;
+        }
+    }
 };
 
 // Tiles and plane chunks of one launch: chunks of zc planes (zc >= zmin,
@@ -133,7 +149,9 @@ struct TbThread {
             const int q = k / KC, f = tid + (k % KC) * NT;
             const bool valid = f < CPOS;
             const int ff = valid ? f : 0;
-            const int row = 2 * (ff / PW) + q, pr = ff % PW;
+            int rp, pr;
+            S::pos_of(ff, rp, pr);
+            const int row = 2 * rp + q;
             const int y = y0 - NS + row, m = m0 - HP + pr;
             e_[k] = row * PW + pr;
             ind[k] = valid && y >= 0 && y < D.h && m >= 0 && m < wh;
@@ -339,7 +357,9 @@ struct Tb2Thread {
             const int q = k / KC, f = tid + (k % KC) * NT;
             const bool valid = f < CPOS;
             const int ff = valid ? f : 0;
-            const int row = 2 * (ff / PW) + q, pr = ff % PW;
+            int rp, pr;
+            S::pos_of(ff, rp, pr);
+            const int row = 2 * rp + q;
             const int y = y0 - NS + row, m = m0 - HP + pr;
             e_[k] = row * PW + pr;
             ind[k] = valid && y >= 0 && y < D.h && m >= 0 && m < wh;
