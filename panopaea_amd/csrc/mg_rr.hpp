@@ -48,6 +48,21 @@ struct RrShape {
     static constexpr int ZS = KZ * NT;
     static constexpr int RS = KR * NT;
     static constexpr int LDS_FLOATS = 3 * 2 * ZS + 2 * RS;
+    // Position f -> (row, pair) of a region holding the TXC tile pairs at
+    // pr = H .. H+TXC-1: the tile pairs of every row first, TXC consecutive
+    // ones per row (with TXC = 32 a half-wave touches 32 consecutive LDS
+    // words, 32 distinct banks), then the 2H halo pairs of each row.
+    template <int H>
+    __host__ __device__ static void pos_of(int f, int rows, int& row, int& pr) {
+        if (f < rows * TXC) {
+            row = f / TXC;
+            pr = H + f % TXC;
+        } else {
+            const int g = f - rows * TXC, j = g % (2 * H);
+            row = g / (2 * H);
+            pr = j < H ? j : TXC + j;
+        }
+    }
 };
 
 // Tiles and coarse-plane chunks of one launch: chunks of kc coarse planes,
@@ -85,11 +100,13 @@ struct RrThread {
     int tid, hw, n2, K0, pf0, pf1;
     // z positions of this thread
     int zoff[KZ];
+    int zidx[KZ];  // LDS index of the position in a z colour plane
     bool zin[KZ];
     float zpf[PD][2][KZ];
     bool zok[PD];
     // residual positions
     int rzi[KR];   // index in a z colour plane
+    int ridx[KR];  // index in a residual colour plane
     int roff[KR];
     int rbits[KR]; // neighbour bits of x0 (bits 0..3) and x1 (bits 8..11), x and y only
     int ryp[KR];   // row parity
@@ -121,8 +138,11 @@ struct RrThread {
         pf1 = 2 * K1;  // last fine plane (inclusive)
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
+            // positions past the region: the pad beyond ZPLANE
             const int e = tid + k * NT;
-            const int row = e / ZW, pr = e - row * ZW;
+            int row, pr;
+            S::template pos_of<2>(e < ZPLANE ? e : 0, S::ZH, row, pr);
+            zidx[k] = e < ZPLANE ? row * ZW + pr : e;
             const int y = 2 * J0 - 2 + row, m = I0 - 2 + pr;
             zin[k] = e < ZPLANE && y >= 0 && y < D.h && m >= 0 && m < wh;
             zoff[k] = zin[k] ? y * wh + m : 0;
@@ -134,8 +154,9 @@ struct RrThread {
             // values and b at a valid address, which the restriction never
             // weighs
             const int e = tid + k * NT;
-            const int ee = e < RPLANE ? e : 0;
-            const int row = ee / RW, pr = ee - row * RW;
+            int row, pr;
+            S::template pos_of<1>(e < RPLANE ? e : 0, RH, row, pr);
+            ridx[k] = e < RPLANE ? row * RW + pr : e;
             const int y = 2 * J0 - 1 + row, m = I0 - 1 + pr;
             rzi[k] = (row + 1) * ZW + pr + 1;
             roff[k] = y >= 0 && y < D.h && m >= 0 && m < wh ? y * wh + m : 0;
@@ -178,8 +199,8 @@ struct RrThread {
 #pragma unroll
         for (int k = 0; k < KZ; ++k) {
             const bool ok = zok[Q] && zin[k];
-            base[tid + k * NT] = ok ? zpf[Q][0][k] : 0.0f;
-            base[ZS + tid + k * NT] = ok ? zpf[Q][1][k] : 0.0f;
+            base[zidx[k]] = ok ? zpf[Q][0][k] : 0.0f;
+            base[ZS + zidx[k]] = ok ? zpf[Q][1][k] : 0.0f;
         }
     }
     template <int Q>
@@ -266,8 +287,8 @@ struct RrThread {
                 const int h0 = (rbits[k] & 0xF) | zb, h1 = ((rbits[k] >> 8) & 0xF) | zb;
                 const float a0 = s * ((float)ncount_bits<3>(h0, D.bc) * z0 - nsum<float, 3>(n0));
                 const float a1 = s * ((float)ncount_bits<3>(h1, D.bc) * z1 - nsum<float, 3>(n1));
-                rl[c0 * RS + tid + k * NT] = b0 - a0;
-                rl[c1 * RS + tid + k * NT] = b1 - a1;
+                rl[c0 * RS + ridx[k]] = b0 - a0;
+                rl[c1 * RS + ridx[k]] = b1 - a1;
             }
         }
         load_b<Q>(p + PD);
