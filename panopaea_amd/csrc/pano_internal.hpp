@@ -53,7 +53,9 @@ struct SolverState {
     int32_t done;      // kernels of the loop return at once when set
     int32_t converged;
     int32_t nonfinite;
-    int32_t pad[3];
+    int32_t xit;       // updates already applied to x (the single-grid loop adds
+                       // alpha*p to x in the next search, so x lags while xit < iter)
+    int32_t pad[2];
 };
 
 // Times every launch of one named kernel with HIP events on the context's

@@ -281,10 +281,12 @@ static pano_status solver_iteration(Solver<T>* S) {
                     S->mask, S->part0, cst);
     }
     fin(S, FIN_ALPHA, S->part0, S->ntile, 0);
-    // x += alpha p ; r -= alpha q ; ||r||_inf   (pcg.rs:55-63)
+    // x += alpha p ; r -= alpha q ; ||r||_inf   (pcg.rs:55-63). The x step is
+    // applied by the next search (it reads p anyway) or, after the last
+    // update, by k_x_final at the end of the loop.
     bool ident = S->precond == PANO_PRECOND_IDENTITY;
-    klaunch(c, "update", 6 * unit, k_update<T>, dim3(S->nflat), dim3(kBlock), S->n, S->x, S->r, (const T*)S->p,
-            (const T*)S->q, S->part1, ident ? S->part0 : (double*)nullptr, cst);
+    klaunch(c, "update", 3 * unit, k_update_r<T>, dim3(S->nflat), dim3(kBlock), S->n, S->r, (const T*)S->q,
+            S->part1, ident ? S->part0 : (double*)nullptr, cst);
     fin(S, FIN_RNORM, S->part1, S->nflat, 1);
     // z = M r ; sigma' = z.r ; beta   (pcg.rs:65-68)
     T* z;
@@ -298,8 +300,8 @@ static pano_status solver_iteration(Solver<T>* S) {
         if (S->rb) fin(S, FIN_BETA, S->part0, np / 2, 0, S->part0 + np / 2);
         else fin(S, FIN_BETA, S->part0, np, 0);
     }
-    // p = z + beta p   (pcg.rs:72-77)
-    klaunch(c, "search", 3 * unit, k_search<T>, dim3(S->nflat), dim3(kBlock), S->n, S->p, (const T*)z, cst);
+    // x += alpha p (this iteration's step) ; p = z + beta p   (pcg.rs:55-56, 72-77)
+    klaunch(c, "search", 5 * unit, k_search_x<T>, dim3(S->nflat), dim3(kBlock), S->n, S->x, S->p, (const T*)z, cst);
     S->it += 1;
     PANO_HIP(hipGetLastError());
     return PANO_OK;
@@ -348,6 +350,10 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
             PANO_CHECK(enqueue(it != c->iter_hint.end() ? 1 : 2));
         }
     }
+    // the last update's x step when no search followed it
+    hipLaunchKernelGGL((k_x_final<T>), dim3(S->nflat), dim3(kBlock), 0, c->stream, S->n, S->x, (const T*)S->p,
+                       (const SolverState*)S->st);
+    PANO_HIP(hipGetLastError());
     PANO_HIP(hipStreamSynchronize(c->stream));
     PANO_HIP(hipMemcpy(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost));
     c->iter_hint[key] = hs[0].iter;

@@ -94,6 +94,8 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
             if (!(st->beta == st->beta)) {
                 st->nonfinite = 1;
                 st->done = 1;
+            } else {
+                st->xit = st->iter;  // the search that follows adds this update's step to x
             }
             break;
         }
@@ -231,6 +233,78 @@ __device__ __forceinline__ void flat_update(int lb, int64_t n, T* x, T* r, const
     *rr = want_rr ? block_sum(s) : 0.0;
 }
 
+// r -= alpha q with the partial max|r| (and r.r with want_rr): the residual
+// half of flat_update, for loops that add alpha*p to x in the next search
+template <typename T>
+__device__ __forceinline__ void flat_update_r(int lb, int64_t n, T* r, const T* q, double alpha, bool want_rr,
+                                              double* mx, double* rr) {
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T rv[kItems], qv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        rv[k] = ldx(r, j);
+        qv[k] = ldx(q, j);
+    }
+    double m = 0.0, s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        if (i < n) {
+            T rn = (T)((double)rv[k] - alpha * (double)qv[k]);
+            stx(r, i, rn);
+            m = absmax_acc(m, (double)rn);
+            s += (double)rn * (double)rn;
+        }
+    }
+    *mx = block_max(m);
+    *rr = want_rr ? block_sum(s) : 0.0;
+}
+
+// x += alpha p (the previous update's step), then p = z + beta p: the
+// expressions of flat_update and flat_search, on the same p
+template <typename T>
+__device__ __forceinline__ void flat_search_x(int lb, int64_t n, T* x, T* p, const T* z, double alpha, double beta) {
+    const int base = lb * kFlatPer + (int)threadIdx.x;
+    T xv[kItems], pv[kItems], zv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        xv[k] = ldx(x, j);
+        pv[k] = ldx(p, j);
+        zv[k] = ldx(z, j);
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        if (i < n) {
+            stx(x, i, (T)((double)xv[k] + alpha * (double)pv[k]));
+            stx(p, i, (T)((double)zv[k] + beta * (double)pv[k]));
+        }
+    }
+}
+
+// x += alpha p alone (the step of the last update when no search followed it)
+template <typename T>
+__device__ __forceinline__ void flat_axpy(int lb, int64_t n, T* x, const T* p, double alpha) {
+    const int base = lb * kFlatPer + (int)threadIdx.x;
+    T xv[kItems], pv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        xv[k] = ldx(x, j);
+        pv[k] = ldx(p, j);
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        if (i < n) stx(x, i, (T)((double)xv[k] + alpha * (double)pv[k]));
+    }
+}
+
 // p = z + beta p (p and z distinct; loads first)
 template <typename T>
 __device__ __forceinline__ void flat_search(int lb, int64_t n, T* p, const T* z, double beta) {
@@ -306,6 +380,33 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_search(int64_t n, T* p, const T* z, const SolverState* st) {
     if (st->done) return;
     flat_search(blockIdx.x, n, p, z, st->beta);
+}
+
+// The single-grid loop's update and search: x += alpha p moves from the
+// update (6 field passes) to the next search (3 -> 5 passes), which reads p
+// anyway; k_x_final applies the last step when no search followed it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_r(int64_t n, T* r, const T* q, double* pmax, double* prr,
+                                                  const SolverState* st) {
+    if (st->done) return;
+    double m, s;
+    flat_update_r(blockIdx.x, n, r, q, st->alpha, prr != nullptr, &m, &s);
+    if (threadIdx.x == 0) {
+        pmax[blockIdx.x] = m;
+        if (prr) prr[blockIdx.x] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_search_x(int64_t n, T* x, T* p, const T* z, const SolverState* st) {
+    if (st->done) return;
+    flat_search_x(blockIdx.x, n, x, p, z, st->alpha, st->beta);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_x_final(int64_t n, T* x, const T* p, const SolverState* st) {
+    if (st->xit >= st->iter) return;
+    flat_axpy(blockIdx.x, n, x, p, st->alpha);
 }
 
 }  // namespace pano
