@@ -245,7 +245,7 @@ __device__ __forceinline__ void flat_update_r(int lb, int64_t n, T* r, const T* 
         const int i = base + k * kBlock;
         const int j = i < n ? i : (int)n - 1;
         rv[k] = ldx(r, j);
-        qv[k] = ldx(q, j);
+        qv[k] = __builtin_nontemporal_load(q + j);  // last use of q
     }
     double m = 0.0, s = 0.0;
 #pragma unroll
@@ -263,7 +263,11 @@ __device__ __forceinline__ void flat_update_r(int lb, int64_t n, T* r, const T* 
 }
 
 // x += alpha p (the previous update's step), then p = z + beta p: the
-// expressions of flat_update and flat_search, on the same p
+// expressions of flat_update and flat_search, on the same p. x is next used
+// one iteration later and z is read here for the last time, so those accesses
+// are non-temporal: they do not displace from the caches the p just written,
+// which the operator reads next. (The old p is loaded normally: its lines
+// take the new p.)
 template <typename T>
 __device__ __forceinline__ void flat_search_x(int lb, int64_t n, T* x, T* p, const T* z, double alpha, double beta) {
     const int base = lb * kFlatPer + (int)threadIdx.x;
@@ -272,15 +276,15 @@ __device__ __forceinline__ void flat_search_x(int lb, int64_t n, T* x, T* p, con
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
         const int j = i < n ? i : (int)n - 1;
-        xv[k] = ldx(x, j);
+        xv[k] = __builtin_nontemporal_load(x + j);
         pv[k] = ldx(p, j);
-        zv[k] = ldx(z, j);
+        zv[k] = __builtin_nontemporal_load(z + j);
     }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
         if (i < n) {
-            stx(x, i, (T)((double)xv[k] + alpha * (double)pv[k]));
+            __builtin_nontemporal_store((T)((double)xv[k] + alpha * (double)pv[k]), x + i);
             stx(p, i, (T)((double)zv[k] + beta * (double)pv[k]));
         }
     }
