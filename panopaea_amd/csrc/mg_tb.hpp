@@ -268,17 +268,24 @@ struct TbThread {
             const int p = (c + k / KC + z) & 1;  // x parity of the colour-c cell (row parity k / KC)
             if ((act[k] >> (H + 4 * p)) & 1) {
                 const int e = e_[k];
-                float n[6];
-                n[0] = xo[e - 1 + p];  // x-1
-                n[1] = xo[e + p];      // x+1
-                n[2] = xo[e - PW];     // y-1
-                n[3] = xo[e + PW];     // y+1
-                n[4] = xm[e];          // z-1
-                n[5] = xp[e];          // z+1
                 const float rv = bq[H][c][k];
                 const float a0 = tb_opaque(zin ? rin[k][0] : rbd[k][0]);
                 const float a1 = tb_opaque(zin ? rin[k][1] : rbd[k][1]);
-                const float v = (rv + s * nsum<float, 3>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                float v;
+                if constexpr (ZERO_X && H == 0) {
+                    // first half-sweep from z = 0: every neighbour still holds +0, whose
+                    // sum is +0 (red_init_run's s*0)
+                    v = (rv + s * 0.0f) * (p ? a1 : a0);
+                } else {
+                    float n[6];
+                    n[0] = xo[e - 1 + p];  // x-1
+                    n[1] = xo[e + p];      // x+1
+                    n[2] = xo[e - PW];     // y-1
+                    n[3] = xo[e + PW];     // y+1
+                    n[4] = xm[e];          // z-1
+                    n[5] = xp[e];          // z+1
+                    v = (rv + s * nsum<float, 3>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                }
                 xc[e] = v;
                 if (own && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
             }
@@ -411,15 +418,20 @@ struct Tb2Thread {
             const int p = (c + k / KC) & 1;  // x parity of the colour-c cell (row parity k / KC)
             if (act[k] & (1 << (H + 4 * p))) {
                 const int e = e_[k];
-                float n[6];
-                n[0] = xo[e - 1 + p];  // x-1
-                n[1] = xo[e + p];      // x+1
-                n[2] = xo[e - PW];     // y-1
-                n[3] = xo[e + PW];     // y+1
-                n[4] = n[5] = 0.0f;
                 const float rv = bv[c][k];
                 const float a0 = tb_opaque(rin[k][0]), a1 = tb_opaque(rin[k][1]);
-                const float v = (rv + s * nsum<float, 2>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                float v;
+                if constexpr (ZERO_X && H == 0) {
+                    v = (rv + s * 0.0f) * (p ? a1 : a0);  // all neighbours still +0
+                } else {
+                    float n[6];
+                    n[0] = xo[e - 1 + p];  // x-1
+                    n[1] = xo[e + p];      // x+1
+                    n[2] = xo[e - PW];     // y-1
+                    n[3] = xo[e + PW];     // y+1
+                    n[4] = n[5] = 0.0f;
+                    v = (rv + s * nsum<float, 2>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                }
                 xc[e] = v;
                 if (DOT && H >= NS - 2 && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
             }
