@@ -252,45 +252,62 @@ struct RrThread {
         }
         load_ahead<0>();
     }
+    // residual of fine plane p into the residual plane rdst (both colours),
+    // from the z planes zc (p), zm (p-1), zp (p+1) in LDS and b of plane p
+    __host__ __device__ __forceinline__ void residual_plane(float* rdst, const float* zc, const float* zm,
+                                                            const float* zp, int p, const float (&bb)[2][KR]) const {
+        const int zb = ((p > 0) << 4) | ((p < D.d - 1) << 5);
+        const int cz = p & 1;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int c0 = cz ^ ryp[k];  // colour of the pair's even cell x0
+            const int c1 = 1 - c0;
+            const int e = rzi[k];
+            const float* o0 = zc + c1 * ZS;  // x0's neighbours in the plane, and x1
+            const float* o1 = zc + c0 * ZS;  // x1's neighbours in the plane, and x0
+            float n0[6], n1[6];
+            n0[0] = o0[e - 1];
+            n0[1] = o0[e];
+            n0[2] = o0[e - ZW];
+            n0[3] = o0[e + ZW];
+            n0[4] = zm[c1 * ZS + e];
+            n0[5] = zp[c1 * ZS + e];
+            n1[0] = o1[e];
+            n1[1] = o1[e + 1];
+            n1[2] = o1[e - ZW];
+            n1[3] = o1[e + ZW];
+            n1[4] = zm[c0 * ZS + e];
+            n1[5] = zp[c0 * ZS + e];
+            const float z0 = o1[e], z1 = o0[e];
+            const float b0 = c0 ? bb[1][k] : bb[0][k], b1 = c0 ? bb[0][k] : bb[1][k];
+            const int h0 = (rbits[k] & 0xF) | zb, h1 = ((rbits[k] >> 8) & 0xF) | zb;
+            const float a0 = s * ((float)ncount_bits<3>(h0, D.bc) * z0 - nsum<float, 3>(n0));
+            const float a1 = s * ((float)ncount_bits<3>(h1, D.bc) * z1 - nsum<float, 3>(n1));
+            rdst[c0 * RS + ridx[k]] = b0 - a0;
+            rdst[c1 * RS + ridx[k]] = b1 - a1;
+        }
+    }
+    // fine plane p of the residual plane rsrc restricted along x, then y, for
+    // the thread's coarse column
+    __host__ __device__ __forceinline__ float restrict_xy(const float* rsrc, int p) const {
+        float t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = rr0 + q;                 // fine row 2J-1+q
+            const int ce = (2 * J - 1 + q + p) & 1;  // colour of the fine x = 2I
+            const float* pe = rsrc + ce * RS + row * RW + rp;
+            const float* po = rsrc + (1 - ce) * RS + row * RW + rp;
+            t[q] = r1(po[-1], pe[0], po[0], pe[1], hmI, hpI);
+        }
+        return r1(t[0], t[1], t[2], t[3], hmJ, hpJ);
+    }
     // residual of plane p (b in register set Q) into LDS (nothing for planes
     // outside the grid); then b of plane p+PD into set Q
     template <int Q>
     __host__ __device__ __forceinline__ void phase1(int p) {
-        if (p >= 0 && p < D.d) {
-            const float* zc = zl + slot(p) * 2 * ZS;
-            const float* zm = zl + slot(p - 1) * 2 * ZS;
-            const float* zp = zl + slot(p + 1) * 2 * ZS;
-            const int zb = ((p > 0) << 4) | ((p < D.d - 1) << 5);
-            const int cz = p & 1;
-#pragma unroll
-            for (int k = 0; k < KR; ++k) {
-                const int c0 = cz ^ ryp[k];  // colour of the pair's even cell x0
-                const int c1 = 1 - c0;
-                const int e = rzi[k];
-                const float* o0 = zc + c1 * ZS;  // x0's neighbours in the plane, and x1
-                const float* o1 = zc + c0 * ZS;  // x1's neighbours in the plane, and x0
-                float n0[6], n1[6];
-                n0[0] = o0[e - 1];
-                n0[1] = o0[e];
-                n0[2] = o0[e - ZW];
-                n0[3] = o0[e + ZW];
-                n0[4] = zm[c1 * ZS + e];
-                n0[5] = zp[c1 * ZS + e];
-                n1[0] = o1[e];
-                n1[1] = o1[e + 1];
-                n1[2] = o1[e - ZW];
-                n1[3] = o1[e + ZW];
-                n1[4] = zm[c0 * ZS + e];
-                n1[5] = zp[c0 * ZS + e];
-                const float z0 = o1[e], z1 = o0[e];
-                const float b0 = c0 ? bpf[Q][1][k] : bpf[Q][0][k], b1 = c0 ? bpf[Q][0][k] : bpf[Q][1][k];
-                const int h0 = (rbits[k] & 0xF) | zb, h1 = ((rbits[k] >> 8) & 0xF) | zb;
-                const float a0 = s * ((float)ncount_bits<3>(h0, D.bc) * z0 - nsum<float, 3>(n0));
-                const float a1 = s * ((float)ncount_bits<3>(h1, D.bc) * z1 - nsum<float, 3>(n1));
-                rl[c0 * RS + ridx[k]] = b0 - a0;
-                rl[c1 * RS + ridx[k]] = b1 - a1;
-            }
-        }
+        if (p >= 0 && p < D.d)
+            residual_plane(rl, zl + slot(p) * 2 * ZS, zl + slot(p - 1) * 2 * ZS, zl + slot(p + 1) * 2 * ZS, p,
+                           bpf[Q]);
         load_b<Q>(p + PD);
     }
     // plane p restricted along x and y into the ring; r_c(K) once plane
@@ -301,19 +318,7 @@ struct RrThread {
         ty[0] = ty[1];
         ty[1] = ty[2];
         ty[2] = ty[3];
-        ty[3] = 0.0f;
-        if (p >= 0 && p < D.d) {
-            float t[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row = rr0 + q;                 // fine row 2J-1+q
-                const int ce = (2 * J - 1 + q + p) & 1;  // colour of the fine x = 2I
-                const float* pe = rl + ce * RS + row * RW + rp;
-                const float* po = rl + (1 - ce) * RS + row * RW + rp;
-                t[q] = r1(po[-1], pe[0], po[0], pe[1], hmI, hpI);
-            }
-            ty[3] = r1(t[0], t[1], t[2], t[3], hmJ, hpJ);
-        }
+        ty[3] = p >= 0 && p < D.d ? restrict_xy(rl, p) : 0.0f;
         // r_c(K) for p = 2K+2 (an always-issued store, dropped on other steps)
         const bool st = p >= 2 * K0 + 2 && !(p & 1);
         const int K = st ? p / 2 - 1 : K0;
