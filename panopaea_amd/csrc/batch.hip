@@ -315,6 +315,17 @@ __global__ __launch_bounds__(256) void kb_restrict(BTab T, char* A,
     }
 }
 
+// 2D levels, residual and restriction in one pass over rr2_[l]: o[0] = z_l,
+// o[1] = r_l, o[2] = r_{l+1}; tile (bx, by) of E.G's coarse-cell tiles
+template <int LC>
+__global__ __launch_bounds__(kRr2TX * kRr2TY) void kb_resrestrict2(BTab T, char* A, const SolverState* st) {
+    BATCH_ENTRY(T);
+    if (st[E.sidx].done) return;
+    __shared__ float lds[Rr2Thread<kRr2TX, kRr2TY, LC>::LDS_FLOATS];
+    rr2_workgroup<kRr2TX, kRr2TY, LC>(lds, E.D, E.Dc, lb % E.G.gx, lb / E.G.gx, fld<float>(A, E.o[0]),
+                                      fld<float>(A, E.o[1]), fld<float>(A, E.o[2]), (float)E.s);
+}
+
 // prolongation from level l+1 over prl_[l]: o[2] = o[0] + P o[1], o[0] = z_l,
 // o[1] = z_{l+1}, o[2] = z_l (in place) or the residual scratch (the input of
 // the one-pass post-smoothing)
@@ -609,6 +620,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         reset(rbs_[l]);
         for (int k = 0; k < kTbConfigs; ++k) reset(tb_[l][k]);
         reset(co_[l]);
+        reset(rsd_[l]);
+        reset(rr2_[l]);
         reset(prl_[l]);
     }
     reset(dot_);
@@ -625,6 +638,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     for (int l = 0; l < 16; ++l) split_first_[l] = false;
     int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
     int srbs[16] = {0}, stb[16][kTbConfigs] = {};
+    int srsd[16] = {0}, srr2[16] = {0};
     std::vector<std::pair<int, int>> dot_src;  // per grid: (config + 1 or 0 for half-sweeps, entry index) at level 0
     int srow = 0, scol = 0;
     size_t maxp = 0;
@@ -747,14 +761,34 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 pr.o[1] = g.o_zl[l + 1];
                 pr.o[2] = tb ? g.o_res : g.o_zl[l];
                 prl_[l].h.push_back(pr);
-                BEntry co = r;
-                int nk = restrict_run_len(r.Dc);  // coarse planes per thread
-                co.G = grid3t(r.Dc.w, r.Dc.h, (r.Dc.d + nk - 1) / nk, shapes.nat_x, shapes.nat_y);
-                co.start = sco[l];
-                sco[l] += co.G.nb;
-                co.o[0] = g.o_rl[l + 1];
-                co.o[1] = g.o_res;
-                co_[l].h.push_back(co);
+                // residual and restriction: one pass on 2D levels (the single-grid rule),
+                // else the residual over pair tiles and the restriction over coarse tiles
+                if (tb_enabled_ && L.d <= 1) {
+                    BEntry q = r;
+                    q.G.gx = (r.Dc.w + kRr2TX - 1) / kRr2TX;
+                    q.G.gy = (r.Dc.h + kRr2TY - 1) / kRr2TY;
+                    q.G.gz = 1;
+                    q.G.nb = q.G.gx * q.G.gy;
+                    q.start = srr2[l];
+                    srr2[l] += q.G.nb;
+                    q.o[0] = g.o_zl[l];
+                    q.o[1] = g.o_rl[l];
+                    q.o[2] = g.o_rl[l + 1];
+                    rr2_[l].h.push_back(q);
+                } else {
+                    BEntry rs = r;
+                    rs.start = srsd[l];
+                    srsd[l] += rs.G.nb;
+                    rsd_[l].h.push_back(rs);
+                    BEntry co = r;
+                    int nk = restrict_run_len(r.Dc);  // coarse planes per thread
+                    co.G = grid3t(r.Dc.w, r.Dc.h, (r.Dc.d + nk - 1) / nk, shapes.nat_x, shapes.nat_y);
+                    co.start = sco[l];
+                    sco[l] += co.G.nb;
+                    co.o[0] = g.o_rl[l + 1];
+                    co.o[1] = g.o_res;
+                    co_[l].h.push_back(co);
+                }
             } else {
                 BFdm F;
                 memset(&F, 0, sizeof(F));
@@ -825,7 +859,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     for (int p = 0; p < 2; ++p)
         total += table_bytes(div_[p]) + table_bytes(prx_[p]) + table_bytes(pry_[p]) + table_bytes(prz_[p]);
     for (int l = 0; l + 1 < levels_; ++l) {
-        total += table_bytes(rb_[l]) + table_bytes(rbs_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]);
+        total += table_bytes(rb_[l]) + table_bytes(rbs_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]) +
+                 table_bytes(rsd_[l]) + table_bytes(rr2_[l]);
         for (int k = 0; k < kTbConfigs; ++k) total += table_bytes(tb_[l][k]);
     }
     total += table_bytes(dot_);
@@ -860,6 +895,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         PANO_CHECK(place(rbs_[l]));
         for (int k = 0; k < kTbConfigs; ++k) PANO_CHECK(place(tb_[l][k]));
         PANO_CHECK(place(co_[l]));
+        PANO_CHECK(place(rsd_[l]));
+        PANO_CHECK(place(rr2_[l]));
         PANO_CHECK(place(prl_[l]));
     }
     PANO_CHECK(place(dot_));
@@ -965,11 +1002,14 @@ pano_status BatchCore::vcycle(bool with_dot) {
             BL(kb_smooth_half, rbs_[l], arena, 0, (double*)nullptr, (const SolverState*)states);
             BL(kb_smooth_half, rbs_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
         }
-        BL(kb_residual, rb_[l], arena, (const SolverState*)states);
+        BL(kb_residual, rsd_[l], arena, (const SolverState*)states);
         dim3 sblk = blk;
         blk = dim3(shapes.nat_x, shapes.nat_y);
         if (l + 1 == Lc) BL(kb_restrict<NATURAL>, co_[l], arena, (const SolverState*)states);
         else BL(kb_restrict<REDBLACK>, co_[l], arena, (const SolverState*)states);
+        blk = dim3(kRr2TX * kRr2TY, 1);
+        if (l + 1 == Lc) BL(kb_resrestrict2<NATURAL>, rr2_[l], arena, (const SolverState*)states);
+        else BL(kb_resrestrict2<REDBLACK>, rr2_[l], arena, (const SolverState*)states);
         blk = sblk;
     }
     if (!rows2_.h.empty()) {

@@ -15,6 +15,7 @@
 
 #include "mg.hpp"
 #include "mg_device.hpp"
+#include "mg_rr.hpp"
 #include "mg_tb.hpp"
 #include "solver_device.hpp"
 
@@ -150,6 +151,9 @@ private:
     Table tb_[16][kTbConfigs];  // one-pass smoothing tiles (3D levels, by configuration)
     Table dot_;            // per grid: where the last post-smoothing left its z.r partials
     Table co_[16];         // coarse-cell tiles of the restriction into level l+1
+    Table rsd_[16];        // pair tiles of the residual (the grids that restrict separately)
+    Table rr2_[16];        // 2D levels: residual + restriction in one pass, coarse-cell tiles
+                           // (o[0] = z_l, o[1] = r_l, o[2] = r_{l+1})
     Table prl_[16];        // pair tiles of level l for the prolongation from l+1
     FdmTable plane_, zcol_;  // 3D coarsest solve
     FdmTable rows2_, cols2_;  // 2D coarsest solve

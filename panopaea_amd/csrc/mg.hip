@@ -261,6 +261,25 @@ __global__ __launch_bounds__(TXC * TYC) void k_resrestrict(Dims D, Dims Dc, RrGr
     rr_workgroup<TXC, TYC, LC, kRrPD>(lds, D, Dc, bx, by, bz, g.kc, z, b, rc, s);
 }
 
+// The 2D form: one workgroup per tile of coarse cells, one plane.
+template <int LC>
+__global__ __launch_bounds__(kRr2TX * kRr2TY) void k_resrestrict2(Dims D, Dims Dc, int gx, int nb, const float* z,
+                                                                  const float* b, float* rc, float s,
+                                                                  const SolverState* st) {
+    if (st->done) return;
+    __shared__ float lds[Rr2Thread<kRr2TX, kRr2TY, LC>::LDS_FLOATS];
+    const int L = xcd_remap(blockIdx.x, nb);
+    rr2_workgroup<kRr2TX, kRr2TY, LC>(lds, D, Dc, L % gx, L / gx, z, b, rc, s);
+}
+
+template <int LC>
+static void resrestrict2(Ctx* c, const char* name, double bytes, const Dims& D, const Dims& Dc, const float* z,
+                         const float* b, float* rc, float s, const SolverState* st) {
+    const int gx = (Dc.w + kRr2TX - 1) / kRr2TX, gy = (Dc.h + kRr2TY - 1) / kRr2TY;
+    klaunch(c, name, bytes, k_resrestrict2<LC>, dim3(gx * gy), dim3(kRr2TX * kRr2TY), D, Dc, gx, gx * gy, z, b, rc,
+            s, st);
+}
+
 template <int LC>
 static void resrestrict(Ctx* c, const char* name, double bytes, const Dims& D, const Dims& Dc, const float* z,
                         const float* b, float* rc, float s, const SolverState* st) {
@@ -587,7 +606,8 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
         klaunch(c, nh, 1.5 * unit, k_smooth_half<T, DIM, NZ>, dim3(G.nb), B, D, G, zl[l], rl[l], s, ri, 1,
                 (double*)nullptr, st);
     }
-    // 3D f32: residual and restriction in one pass (bytes: z and b in, r_c out)
+    // f32: residual and restriction in one pass (bytes: z and b in, r_c out) on
+    // 3D levels of 2^23 cells and more and on every 2D level
     if constexpr (std::is_same<T, float>::value && DIM == 3) {
         if (rr_on(m->tb, D)) {
             const char* nrr = l == 0 ? "resrestrict_l0" : "";
@@ -595,6 +615,16 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
                 resrestrict<NATURAL>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
             else
                 resrestrict<REDBLACK>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
+            return;
+        }
+    }
+    if constexpr (std::is_same<T, float>::value && DIM == 2) {
+        if (m->tb) {
+            const char* nrr = l == 0 ? "resrestrict_l0" : "";
+            if (l + 1 == Lc)
+                resrestrict2<NATURAL>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
+            else
+                resrestrict2<REDBLACK>(c, nrr, (2 + cfrac) * unit, D, Dc, zl[l], rl[l], m->r[l + 1], s, st);
             return;
         }
     }

@@ -328,6 +328,85 @@ struct RrThread {
     }
 };
 
+// The 2D form: one plane, so a workgroup loads its z tile with the ring and
+// b once, computes the residual region in LDS and restricts it (x, then y:
+// restrict_run in 2D). Same positions as the 3D form; the residual is
+// residual_run's 2D expression (four neighbours).
+template <int TXC, int TYC, int LC>
+struct Rr2Thread : RrThread<TXC, TYC, LC, 1> {
+    using B = RrThread<TXC, TYC, LC, 1>;
+    using B::KR;
+    using B::KZ;
+    using B::RS;
+    using B::ZS;
+    using B::ZW;
+    static constexpr int LDS_FLOATS = 2 * ZS + 2 * RS;
+
+    __host__ __device__ __forceinline__ void init(float* lds, const Dims& D_, const Dims& Dc_, int bx, int by,
+                                                  const float* z_, const float* b_, float* rc_, float s_, int tid_) {
+        B::init(lds, D_, Dc_, bx, by, 0, 1, z_, b_, rc_, s_, tid_);
+        B::rl = lds + 2 * ZS;
+    }
+    // z (both colours) into LDS from its base, zero outside the grid; b into registers
+    __host__ __device__ __forceinline__ void load_fill() {
+        B::template load_z<0>(0);
+        B::template load_b<0>(0);
+#pragma unroll
+        for (int k = 0; k < KZ; ++k) {
+            const bool ok = B::zin[k];
+            B::zl[B::zidx[k]] = ok ? B::zpf[0][0][k] : 0.0f;
+            B::zl[ZS + B::zidx[k]] = ok ? B::zpf[0][1][k] : 0.0f;
+        }
+    }
+    __host__ __device__ __forceinline__ void residual() {
+        const float* zc = B::zl;
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int c0 = B::ryp[k];  // colour of the pair's even cell x0 (plane 0)
+            const int c1 = 1 - c0;
+            const int e = B::rzi[k];
+            const float* o0 = zc + c1 * ZS;
+            const float* o1 = zc + c0 * ZS;
+            float n0[4], n1[4];
+            n0[0] = o0[e - 1];
+            n0[1] = o0[e];
+            n0[2] = o0[e - ZW];
+            n0[3] = o0[e + ZW];
+            n1[0] = o1[e];
+            n1[1] = o1[e + 1];
+            n1[2] = o1[e - ZW];
+            n1[3] = o1[e + ZW];
+            const float z0 = o1[e], z1 = o0[e];
+            const float b0 = c0 ? B::bpf[0][1][k] : B::bpf[0][0][k], b1 = c0 ? B::bpf[0][0][k] : B::bpf[0][1][k];
+            const int h0 = B::rbits[k] & 0xF, h1 = (B::rbits[k] >> 8) & 0xF;
+            const float a0 = B::s * ((float)ncount_bits<2>(h0, B::D.bc) * z0 - nsum<float, 2>(n0));
+            const float a1 = B::s * ((float)ncount_bits<2>(h1, B::D.bc) * z1 - nsum<float, 2>(n1));
+            B::rl[c0 * RS + B::ridx[k]] = b0 - a0;
+            B::rl[c1 * RS + B::ridx[k]] = b1 - a1;
+        }
+    }
+    __host__ __device__ __forceinline__ void restrict_store() {
+        const float v = B::restrict_xy(B::rl, 0);
+        st_if(B::rco, B::own ? cidx<LC>(B::Dc, 0, B::J, B::I) : 0, v, B::own);
+    }
+};
+
+// Coarse tile of the 2D form (32 x 8 coarse cells, 256 threads).
+constexpr int kRr2TX = 32, kRr2TY = 8;
+
+// One workgroup of the 2D form: tile (bx, by) of coarse cells.
+template <int TXC, int TYC, int LC>
+__device__ __forceinline__ void rr2_workgroup(float* lds, const Dims& D, const Dims& Dc, int bx, int by,
+                                              const float* z, const float* b, float* rc, float s) {
+    Rr2Thread<TXC, TYC, LC> th;
+    th.init(lds, D, Dc, bx, by, z, b, rc, s, threadIdx.x);
+    th.load_fill();
+    __syncthreads();
+    th.residual();
+    __syncthreads();
+    th.restrict_store();
+}
+
 template <typename TH, int... Q>
 __device__ __forceinline__ void rr_steps(TH& th, int& p, int p1, std::integer_sequence<int, Q...>) {
     ((p <= p1 ? (th.template phase1<Q>(p), __syncthreads(), th.template phase2<Q>(p), __syncthreads(), ++p) : p),
