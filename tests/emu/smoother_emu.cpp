@@ -1,0 +1,203 @@
+// CPU emulation of the one-pass smoother (mg_tb.hpp): the device structs
+// TbThread / Tb2Thread built as host code, one workgroup at a time, every
+// thread running each phase before any runs the next (a barrier between
+// phases), compared bit for bit with full-grid red-black half-sweeps, and the
+// z.r partial sums with double sums of the same terms. Run under
+// AddressSanitizer by tests/test_emulation.py.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <random>
+#include "mg_tb.hpp"
+using namespace pano;
+
+static int rbi(const Dims& D, int z, int y, int x) {
+    int wh = D.w / 2, n2 = D.d * D.h * wh;
+    return ((x + y + z) & 1) * n2 + (z * D.h + y) * wh + (x >> 1);
+}
+
+// NS half-sweeps over the whole grid, colours CF, CF^1, ...
+static void ref_sweeps(const Dims& D, int NS, int CF, bool zero_x, std::vector<float>& x, const std::vector<float>& r,
+                       float s, const RInv<float>& ri, double* dots) {
+    if (zero_x) std::fill(x.begin(), x.end(), 0.0f);
+    for (int h = 0; h < NS; ++h) {
+        int c = (CF + h) & 1;
+        double acc = 0;
+        for (int z = 0; z < D.d; ++z)
+            for (int y = 0; y < D.h; ++y)
+                for (int xx = 0; xx < D.w; ++xx) {
+                    if (((xx + y + z) & 1) != c) continue;
+                    float n[6] = {0, 0, 0, 0, 0, 0};
+                    int has = 0;
+                    if (xx > 0) { n[0] = x[rbi(D, z, y, xx - 1)]; has |= 1; }
+                    if (xx < D.w - 1) { n[1] = x[rbi(D, z, y, xx + 1)]; has |= 2; }
+                    if (y > 0) { n[2] = x[rbi(D, z, y - 1, xx)]; has |= 4; }
+                    if (y < D.h - 1) { n[3] = x[rbi(D, z, y + 1, xx)]; has |= 8; }
+                    if (z > 0) { n[4] = x[rbi(D, z - 1, y, xx)]; has |= 16; }
+                    if (z < D.d - 1) { n[5] = x[rbi(D, z + 1, y, xx)]; has |= 32; }
+                    int i = rbi(D, z, y, xx);
+                    float v = gs_value(r[i], s, nsum<float, 3>(n), ncount_bits<3>(has, D.bc), ri);
+                    x[i] = v;
+                    acc += (double)v * (double)r[i];
+                }
+        if (h >= NS - 2) dots[h - (NS - 2)] = acc;
+    }
+}
+
+template <int NS, int J, int H = 0, typename V>
+static void stages(V& th, int t) {
+    if constexpr (H < NS) {
+        for (auto& tt : th) tt.template stage<H, J>(t);
+        stages<NS, J, H + 1>(th, t);
+    }
+}
+template <int NS, int SL, int J = 0, typename V>
+static void round_(V& th, int& t, int t1) {
+    if constexpr (J < SL) {
+        if (t > t1) return;
+        for (auto& tt : th) tt.template begin_step<J>(t);
+        stages<NS, J>(th, t);
+        for (auto& tt : th) tt.template end_step<J>(t);
+        ++t;
+        round_<NS, SL, J + 1>(th, t, t1);
+    }
+}
+
+template <int NS, int TXP, int TY, int CF, bool ZX, bool DOT, int NT = 512>
+static int run(const Dims& D, int want, int zmin, unsigned seed) {
+    using TH = TbThread<NS, TXP, TY, NT, CF, ZX, DOT>;
+    const int n = D.d * D.h * D.w;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(-1, 1);
+    std::vector<float> xin(n), r(n), xref, xout(n, 12345.0f);
+    for (auto& v : xin) v = U(rng);
+    for (auto& v : r) v = U(rng);
+    float s = 0.37f;
+    RInv<float> ri = make_rinv<float>(s);
+    xref = xin;
+    double dref[2] = {0, 0};
+    ref_sweeps(D, NS, CF, ZX, xref, r, s, ri, dref);
+    TbGrid g = tb_grid<TXP, TY>(D, want, zmin);
+    double dsum[2] = {0, 0};
+    for (int L = 0; L < g.nb; ++L) {
+        int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
+        if (bz * g.zc >= D.d) { printf("empty chunk\n"); return 1; }
+        std::vector<float> lds(TH::S::LDS_FLOATS, -777.0f);
+        std::vector<TH> th(TH::S::NT);
+        for (int i = 0; i < TH::S::NT; ++i)
+            th[i].init(lds.data(), D, bx, by, bz, g.zc, xin.data(), xout.data(), r.data(), s, ri.v, i);
+        for (auto& t : th) t.prologue();
+        int t = th[0].first_step(), t1 = th[0].last_step();
+        while (t <= t1) round_<NS, TH::SLOTS>(th, t, t1);
+        for (auto& tt : th) { dsum[0] += tt.acc[0]; dsum[1] += tt.acc[1]; }
+    }
+    int bad = 0;
+    for (int i = 0; i < n; ++i)
+        if (memcmp(&xout[i], &xref[i], 4) != 0) {
+            if (bad < 5) printf("  mismatch at %d: %.9g vs %.9g\n", i, xout[i], xref[i]);
+            ++bad;
+        }
+    double e0 = DOT ? fabs(dsum[0] - dref[0]) / (fabs(dref[0]) + 1e-30) : 0;
+    double e1 = DOT ? fabs(dsum[1] - dref[1]) / (fabs(dref[1]) + 1e-30) : 0;
+    printf("NS=%d CF=%d ZX=%d DOT=%d dims %dx%dx%d bc=%d grid gx=%d gy=%d zc=%d nb=%d: %d mismatches, dot rel err %.2e %.2e\n",
+           NS, CF, ZX, DOT, D.d, D.h, D.w, D.bc, g.gx, g.gy, g.zc, g.nb, bad, e0, e1);
+    return bad || e0 > 1e-12 || e1 > 1e-12;
+}
+
+
+// ---- 2D ----
+static void ref_sweeps2(const Dims& D, int NS, int CF, bool zero_x, std::vector<float>& x, const std::vector<float>& r,
+                        float s, const RInv<float>& ri, double* dots) {
+    if (zero_x) std::fill(x.begin(), x.end(), 0.0f);
+    for (int h = 0; h < NS; ++h) {
+        int c = (CF + h) & 1;
+        double acc = 0;
+        for (int y = 0; y < D.h; ++y)
+            for (int xx = 0; xx < D.w; ++xx) {
+                if (((xx + y) & 1) != c) continue;
+                float n[6] = {0, 0, 0, 0, 0, 0};
+                int has = 0;
+                if (xx > 0) { n[0] = x[rbi(D, 0, y, xx - 1)]; has |= 1; }
+                if (xx < D.w - 1) { n[1] = x[rbi(D, 0, y, xx + 1)]; has |= 2; }
+                if (y > 0) { n[2] = x[rbi(D, 0, y - 1, xx)]; has |= 4; }
+                if (y < D.h - 1) { n[3] = x[rbi(D, 0, y + 1, xx)]; has |= 8; }
+                int i = rbi(D, 0, y, xx);
+                float v = gs_value(r[i], s, nsum<float, 2>(n), ncount_bits<2>(has, D.bc), ri);
+                x[i] = v;
+                acc += (double)v * (double)r[i];
+            }
+        if (h >= NS - 2) dots[h - (NS - 2)] = acc;
+    }
+}
+
+template <int TXP, int TY, int NT, int CF, bool ZX, bool DOT>
+static int run2(const Dims& D, unsigned seed) {
+    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZX, DOT>;
+    const int n = D.h * D.w;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> U(-1, 1);
+    std::vector<float> xin(n), r(n), xref, xout(n, 12345.0f);
+    for (auto& v : xin) v = U(rng);
+    for (auto& v : r) v = U(rng);
+    float s = 0.37f;
+    RInv<float> ri = make_rinv<float>(s);
+    xref = xin;
+    double dref[2] = {0, 0};
+    ref_sweeps2(D, 4, CF, ZX, xref, r, s, ri, dref);
+    const int gx = ((D.w >> 1) + TXP - 1) / TXP, gy = (D.h + TY - 1) / TY;
+    double dsum[2] = {0, 0};
+    for (int L = 0; L < gx * gy; ++L) {
+        int bx = L % gx, by = L / gx;
+        std::vector<float> lds(TH::LDS_FLOATS, -777.0f);
+        std::vector<TH> th(NT);
+        for (int i = 0; i < NT; ++i) th[i].init(lds.data(), D, bx, by, xin.data(), r.data(), s, ri.v, i);
+        for (auto& t : th) t.template stage<0>();
+        for (auto& t : th) t.template stage<1>();
+        for (auto& t : th) t.template stage<2>();
+        for (auto& t : th) t.template stage<3>();
+        for (auto& t : th) t.store(xout.data());
+        for (auto& t : th) { dsum[0] += t.acc[0]; dsum[1] += t.acc[1]; }
+    }
+    int bad = 0;
+    for (int i = 0; i < n; ++i)
+        if (memcmp(&xout[i], &xref[i], 4) != 0) {
+            if (bad < 5) printf("  mismatch at %d: %.9g vs %.9g\n", i, xout[i], xref[i]);
+            ++bad;
+        }
+    double e0 = DOT ? fabs(dsum[0] - dref[0]) / (fabs(dref[0]) + 1e-30) : 0;
+    double e1 = DOT ? fabs(dsum[1] - dref[1]) / (fabs(dref[1]) + 1e-30) : 0;
+    printf("2D CF=%d ZX=%d DOT=%d dims %dx%d bc=%d tiles %dx%d: %d mismatches, dot rel err %.2e %.2e\n", CF, ZX, DOT,
+           D.h, D.w, D.bc, gx, gy, bad, e0, e1);
+    return bad || e0 > 1e-12 || e1 > 1e-12;
+}
+
+int main() {
+    int fails = 0;
+    const Dims cases[] = {{16, 16, 16, 0}, {9, 5, 6, 32 | 16}, {20, 36, 70, 0}, {3, 40, 140, 32},
+                          {32, 32, 128, 63}, {40, 17, 66, 2}};
+    unsigned seed = 1;
+    for (auto D : cases) {
+        for (int zmin : {8, 2, 1}) {
+            // the production tiles: 32 x 32 pairs / 768 threads (levels >= 2^20 cells), 32 x 8 / 320
+            fails += run<4, 32, 32, 0, true, false, 768>(D, 512, zmin, seed++);
+            fails += run<4, 32, 32, 1, false, true, 768>(D, 512, zmin, seed++);
+            fails += run<4, 32, 32, 1, false, false, 768>(D, 1024, zmin, seed++);
+            fails += run<4, 32, 8, 0, true, false, 320>(D, 2048, zmin, seed++);
+            fails += run<4, 32, 8, 1, false, true, 320>(D, 2048, zmin, seed++);
+            fails += run<4, 32, 8, 1, false, false, 320>(D, 2048, zmin, seed++);
+            // a small generic tile (several workgroups on every shape)
+            fails += run<4, 8, 4, 1, false, true, 64>(D, 64, zmin, seed++);
+        }
+    }
+    const Dims c2[] = {{1, 64, 64, 0}, {1, 40, 70, 2}, {1, 100, 200, 15}, {1, 8, 6, 4}, {1, 33, 130, 9}};
+    for (auto D : c2) {
+        fails += run2<32, 32, 768, 0, true, false>(D, seed++);
+        fails += run2<32, 32, 768, 1, false, true>(D, seed++);
+        fails += run2<32, 32, 768, 1, false, false>(D, seed++);
+        fails += run2<8, 4, 64, 1, false, true>(D, seed++);
+    }
+    printf("%s (%d failing cases)\n", fails ? "FAIL" : "OK", fails);
+    return fails != 0;
+}
