@@ -15,7 +15,7 @@ enum CellLayout { NATURAL = 0, REDBLACK = 1 };
 
 // Indices are 32-bit: check_grid caps a grid below 2^31 faces in total, so
 // every cell field and face component holds fewer than 2^30 elements.
-__device__ __forceinline__ int cell_index(const Dims& D, int z, int y, int x) {
+__host__ __device__ __forceinline__ int cell_index(const Dims& D, int z, int y, int x) {
     return (z * D.h + y) * D.w + x;
 }
 __device__ __forceinline__ int rb_index(const Dims& D, int z, int y, int x) {
@@ -32,26 +32,26 @@ __device__ __forceinline__ int lidx(const Dims& D, int z, int y, int x) {
 
 // ---- interpolation, math/interp.rs:7-36 -----------------------------------------
 template <typename T>
-__device__ __forceinline__ T lerp(T a0, T a1, T s) {
+__host__ __device__ __forceinline__ T lerp(T a0, T a1, T s) {
     return a0 * ((T)1 - s) + a1 * s;
 }
 template <typename T>
-__device__ __forceinline__ T bilerp(T a00, T a01, T a10, T a11, T s, T t) {
+__host__ __device__ __forceinline__ T bilerp(T a00, T a01, T a10, T a11, T s, T t) {
     return lerp(lerp(a00, a01, s), lerp(a10, a11, s), t);
 }
 template <typename T>
-__device__ __forceinline__ T trilerp(T a000, T a001, T a010, T a011, T a100, T a101, T a110,
-                                     T a111, T s, T t, T u) {
+__host__ __device__ __forceinline__ T trilerp(T a000, T a001, T a010, T a011, T a100, T a101, T a110, T a111,
+                                              T s, T t, T u) {
     return lerp(bilerp(a000, a001, a010, a011, s, t), bilerp(a100, a101, a110, a111, s, t), u);
 }
 template <typename T>
-__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T>
-__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
-__device__ __forceinline__ float tfloor(float v) { return floorf(v); }
-__device__ __forceinline__ double tfloor(double v) { return floor(v); }
-__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ float tfloor(float v) { return floorf(v); }
+__host__ __device__ __forceinline__ double tfloor(double v) { return floor(v); }
+__host__ __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 // ---- face layout views ---------------------------------------------------------------
 template <typename T>
@@ -75,13 +75,13 @@ __host__ __device__ inline Faces<T> split_faces(const Dims& D, T* f) {
     }
     return r;
 }
-__device__ __forceinline__ int fz_idx(const Dims& D, int z, int y, int x) {
+__host__ __device__ __forceinline__ int fz_idx(const Dims& D, int z, int y, int x) {
     return (z * D.h + y) * D.w + x;
 }
-__device__ __forceinline__ int fy_idx(const Dims& D, int z, int y, int x) {
+__host__ __device__ __forceinline__ int fy_idx(const Dims& D, int z, int y, int x) {
     return (z * (D.h + 1) + y) * D.w + x;
 }
-__device__ __forceinline__ int fx_idx(const Dims& D, int z, int y, int x) {
+__host__ __device__ __forceinline__ int fx_idx(const Dims& D, int z, int y, int x) {
     return (z * D.h + y) * (D.w + 1) + x;
 }
 
@@ -133,7 +133,7 @@ struct Sphere {
     int on;
     int lo[3], hi[3];  // (z, y, x) cell bounding box of the sphere (inclusive)
 };
-__device__ __forceinline__ bool in_sphere(const Sphere& S, const Dims& D, int z, int y, int x) {
+__host__ __device__ __forceinline__ bool in_sphere(const Sphere& S, const Dims& D, int z, int y, int x) {
     if ((D.d > 1 && (z < S.lo[0] || z > S.hi[0])) || y < S.lo[1] || y > S.hi[1] || x < S.lo[2] || x > S.hi[2])
         return false;
     double dz = (double)z + 0.5 - S.cz;
@@ -153,7 +153,7 @@ __device__ __forceinline__ T rho_eff(const T* rho, const Sphere& S, const Dims& 
 
 // ---- advect (dec_fluid.rs:173-211), per axis of the cell-centred field ----------------
 template <typename T>
-__device__ __forceinline__ void cell_coord(T pp, int n, int& i0, int& i1, T& f) {
+__host__ __device__ __forceinline__ void cell_coord(T pp, int n, int& i0, int& i1, T& f) {
     if (n == 1) {
         i0 = 0;
         i1 = 0;
@@ -178,8 +178,8 @@ __device__ __forceinline__ void cell_coord(T pp, int n, int& i0, int& i1, T& f) 
 // Cell-centred values (density), dec_fluid.rs:173-211, with the source
 // sphere's reset applied to the interpolated neighbours.
 template <typename T, int DIM, int NZ>
-__device__ __forceinline__ void advect_cell_run(const Dims& D, const T* q, const Sphere& S, T dt,
-                                                const Faces<const T>& v, int z0, int y, int x, T* out) {
+__host__ __device__ __forceinline__ void advect_cell_run(const Dims& D, const T* q, const Sphere& S, T dt,
+                                                         const Faces<const T>& v, int z0, int y, int x, T* out) {
     const T half = (T)0.5, two = (T)2.0, ndt = -dt;
     T vx[NZ][2], vy[NZ][2], vz[NZ][2];
 #pragma unroll
@@ -263,7 +263,7 @@ __device__ __forceinline__ T advect_cell_value(const Dims& D, const T* q, const 
 // nmax + 1 so the float-to-int conversion never overflows; this changes
 // nothing for departure points less than one cell beyond the far wall.
 template <typename T>
-__device__ __forceinline__ void face_coord(T pp, T off, int nmax, int& i0, int& i1, T& f) {
+__host__ __device__ __forceinline__ void face_coord(T pp, T off, int nmax, int& i0, int& i1, T& f) {
     T fi = tmin(tmax(tfloor(pp - off), (T)0), (T)(nmax + 1));
     int i = (int)fi;
     i0 = imin(i, nmax);
@@ -276,9 +276,9 @@ __device__ __forceinline__ void face_coord(T pp, T off, int nmax, int& i0, int& 
 // faces also get the smoke buoyancy buoy_c*(rho[z-1] + rho[z]) on interior
 // and open faces.
 template <typename T, int DIM, int AXIS, int NZ>
-__device__ __forceinline__ void advect_face_run(const Dims& D, const T* qc, T dt, const Faces<const T>& v, int z0,
-                                                int y, int x, T* out, T buoy_c = (T)0, const T* rho = nullptr,
-                                                const Sphere* S = nullptr) {
+__host__ __device__ __forceinline__ void advect_face_run(const Dims& D, const T* qc, T dt, const Faces<const T>& v,
+                                                         int z0, int y, int x, T* out, T buoy_c = (T)0,
+                                                         const T* rho = nullptr, const Sphere* S = nullptr) {
     const T half = (T)0.5, four = (T)4.0, zero = (T)0.0, ndt = -dt;
     const int np = AXIS == 2 ? D.d + 1 : D.d;  // planes of this component
     // own component and the four-face averages of the other components

@@ -143,7 +143,10 @@ def test_gpu_masked_operator_matches_literal_closure(ctx):
 
 
 @pytest.mark.parametrize("dtype", DT)
-@pytest.mark.parametrize("dims", [(20, 28), (64, 48), (9, 12, 10), (16, 16, 16)])
+# (7, 1), (5, 4, 1), (6, 2), (4, 3, 2): rows of one and two elements, where
+# the clamped neighbour indices of a gather coincide
+@pytest.mark.parametrize("dims", [(20, 28), (64, 48), (9, 12, 10), (16, 16, 16), (7, 1), (6, 2), (5, 4, 1),
+                                  (4, 3, 2)])
 @pytest.mark.parametrize("scale", [0.5, 40.0])
 def test_gpu_advection_bitwise(ctx, dtype, dims, scale):
     rng = np.random.default_rng(int(scale) + dims[0])
