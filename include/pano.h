@@ -209,8 +209,8 @@ typedef struct {
 
 /* source -> advect density and MAC velocity -> b = -div u -> PCG ->
  * projection. density_in/velocity_in are read, density_out/velocity_out
- * written (they must be distinct buffers); pressure (optional, may be NULL)
- * receives the solved pressure. Synchronous. */
+ * written (they must be distinct buffers); pressure (optional, may be NULL,
+ * distinct from both outputs) receives the solved pressure. Synchronous. */
 pano_status pano_fluid_step(pano_ctx, const pano_grid*, const pano_step_params*,
                             const void* density_in, const void* velocity_in, void* density_out,
                             void* velocity_out, void* pressure, pano_pcg_info* info);

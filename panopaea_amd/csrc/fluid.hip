@@ -141,20 +141,28 @@ __global__ __launch_bounds__(256) void k_neg_div_masked2(Dims D, Grid3 G, T* b, 
     b[(int64_t)y * w + x] = -d;
 }
 
+// pout (optional): the threads of faces x < w also copy cell (z, y, x) of p
+// into pout in natural layout, the pressure export without a pass of its own.
 template <typename T, int DIM, int L_>
-__global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt) {
+__global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt, T* pout) {
     constexpr int NZ = kPlaneRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x > D.w || y >= D.h) return;
-    T out[NZ];
+    T out[NZ], pc[NZ];
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = PLANE_Z(k, D.d);
-        out[k] = project_x<T, L_>(D, ldx(ux, fx_idx(D, z, y, x)), p, dt, z, y, x);
+        pc[k] = ldx(p, lidx<L_>(D, z, y, imin(x, D.w - 1)));
+        out[k] = project_x_pc<T, L_>(D, ldx(ux, fx_idx(D, z, y, x)), p, dt, z, y, x, pc[k]);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k)
         if (z0 + k < D.d) stx(ux, fx_idx(D, z0 + k, y, x), out[k]);
+    if (pout && x < D.w) {
+#pragma unroll
+        for (int k = 0; k < NZ; ++k)
+            if (z0 + k < D.d) stx(pout, cell_index(D, z0 + k, y, x), pc[k]);
+    }
 }
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const T* p, T dt) {
@@ -284,29 +292,29 @@ pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t
 }
 
 template <typename T, int L_>
-static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
+static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
     Faces<T> u = split_faces(D, faces);
     const int nz = D.d > 1 ? kPlaneRun<3> : 1;
     Grid3 Gx = grid3_run(D.w + 1, D.h, D.d, nz), Gy = grid3_run(D.w, D.h + 1, D.d, nz);
     if (D.d > 1) {
         Grid3 Gz = grid3_run(D.w, D.h, D.d + 1, nz);
-        LAUNCH3((k_project_x<T, 3, L_>), Gx, D, Gx, u.x, p, (T)dt);
+        LAUNCH3((k_project_x<T, 3, L_>), Gx, D, Gx, u.x, p, (T)dt, p_out);
         LAUNCH3((k_project_y<T, 3, L_>), Gy, D, Gy, u.y, p, (T)dt);
         LAUNCH3((k_project_z<T, L_>), Gz, D, Gz, u.z, p, (T)dt);
     } else {
-        LAUNCH3((k_project_x<T, 2, L_>), Gx, D, Gx, u.x, p, (T)dt);
+        LAUNCH3((k_project_x<T, 2, L_>), Gx, D, Gx, u.x, p, (T)dt, p_out);
         LAUNCH3((k_project_y<T, 2, L_>), Gy, D, Gy, u.y, p, (T)dt);
     }
     return PANO_OK;
 }
 
 template <typename T>
-pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
-    return launch_project_l<T, NATURAL>(c, D, dt, faces, p);
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
+    return launch_project_l<T, NATURAL>(c, D, dt, faces, p, p_out);
 }
 template <typename T>
-pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p) {
-    return launch_project_l<T, REDBLACK>(c, D, dt, faces, p);
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
+    return launch_project_l<T, REDBLACK>(c, D, dt, faces, p, p_out);
 }
 
 #define INST(T)                                                                                     \
@@ -317,8 +325,8 @@ pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const 
     template pano_status launch_neg_div<T>(Ctx*, const Dims&, T*, const T*);                         \
     template pano_status launch_poisson_apply<T>(Ctx*, const Dims&, double, const uint8_t*, T*,       \
                                                  const T*);                                         \
-    template pano_status launch_project<T>(Ctx*, const Dims&, double, T*, const T*);                 \
-    template pano_status launch_project_rb<T>(Ctx*, const Dims&, double, T*, const T*);
+    template pano_status launch_project<T>(Ctx*, const Dims&, double, T*, const T*, T*);             \
+    template pano_status launch_project_rb<T>(Ctx*, const Dims&, double, T*, const T*, T*);
 INST(float)
 INST(double)
 #undef INST
@@ -375,8 +383,8 @@ pano_status pano_project(pano_ctx ctx, const pano_grid* g, double dt, void* face
     PANO_CHECK(pre(ctx, g, &c));
     if (!faces || !p || faces == p) return fail(PANO_ERR_INVALID, "null or aliased field");
     Dims D = dims_of(g);
-    return g->dtype == PANO_F64 ? launch_project<double>(c, D, dt, (double*)faces, (const double*)p)
-                                : launch_project<float>(c, D, dt, (float*)faces, (const float*)p);
+    return g->dtype == PANO_F64 ? launch_project<double>(c, D, dt, (double*)faces, (const double*)p, nullptr)
+                                : launch_project<float>(c, D, dt, (float*)faces, (const float*)p, nullptr);
 }
 
 pano_status pano_advect(pano_ctx ctx, const pano_grid* g, void* dst, const void* src, double dt,

@@ -564,19 +564,26 @@ __device__ __forceinline__ int ncount(const Dims& D, int z, int y, int x) {
 // ---- projection (dec_fluid.rs:124-141) ----------------------------------------------------
 // u = u + dt*g, g = derivative_0_dual(p): x faces p[x-1]-p[x], y/z faces
 // -(p[+]-p[-]); closed-wall faces set to zero, open faces use p = 0 outside.
+// pc = p at (z, y, min(x, w-1)): the second operand of an interior face and
+// the only value an open boundary face reads (cell 0 at x = 0, cell w-1 at
+// x = w), passed in so a caller can reuse the load.
 template <typename T, int L>
-__device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
+__device__ __forceinline__ T project_x_pc(const Dims& D, T u, const T* p, T dt, int z, int y, int x, T pc) {
     const T zero = (T)0;
     T g;
-    if (x > 0 && x < D.w) g = ldx(p, lidx<L>(D, z, y, x - 1)) - ldx(p, lidx<L>(D, z, y, x));
+    if (x > 0 && x < D.w) g = ldx(p, lidx<L>(D, z, y, x - 1)) - pc;
     else if (x == 0) {
         if (!(D.bc & 1)) return zero;
-        g = zero - ldx(p, lidx<L>(D, z, y, 0));
+        g = zero - pc;
     } else {
         if (!(D.bc & 2)) return zero;
-        g = ldx(p, lidx<L>(D, z, y, D.w - 1)) - zero;
+        g = pc - zero;
     }
     return u + dt * g;
+}
+template <typename T, int L>
+__device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
+    return project_x_pc<T, L>(D, u, p, dt, z, y, x, ldx(p, lidx<L>(D, z, y, imin(x, D.w - 1))));
 }
 template <typename T, int L>
 __device__ __forceinline__ T project_y(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {

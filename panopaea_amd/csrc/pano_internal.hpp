@@ -207,10 +207,11 @@ pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces);
 template <typename T>
 pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t* mask, T* q,
                                  const T* p);
+// p_out (optional, may be null): a natural-layout copy of p written on the way
 template <typename T>
-pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p);
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out);
 template <typename T>
-pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p);
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out);
 template <typename T>
 pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
                           const double* src_sphere /* cz,cy,cx,r or null */);

@@ -463,7 +463,6 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
     // b = -div u
     Faces<const T> v = split_faces(D, (const T*)vel_out);
-    Grid3 Gn = grid3(D.w, D.h, D.d);
     Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
     if (S.rb) {
         if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gd, D, Gd, S.r, v);
@@ -475,13 +474,10 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     PANO_HIP(hipGetLastError());
     PANO_CHECK(solver_init<T>(&S, pp.max_iterations));
     PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, hint_key(D, &pp, sizeof(T))));
-    // u += dt d0_dual(p), closed walls zeroed
-    if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x));
-    else PANO_CHECK(launch_project<T>(c, D, P->dt, vel_out, S.x));
-    if (pressure) {
-        if (S.rb) L3((k_from_rb<T>), Gn, D, Gn, pressure, S.x);
-        else PANO_HIP(hipMemcpyAsync(pressure, S.x, nb, hipMemcpyDeviceToDevice, c->stream));
-    }
+    // u += dt d0_dual(p), closed walls zeroed; the x component's pass also
+    // writes the pressure out in natural layout
+    if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x, pressure));
+    else PANO_CHECK(launch_project<T>(c, D, P->dt, vel_out, S.x, pressure));
     PANO_HIP(hipStreamSynchronize(c->stream));
     return finish_info(c, info);
 }
@@ -701,6 +697,8 @@ pano_status pano_fluid_step(pano_ctx ctx, const pano_grid* g, const pano_step_pa
         return fail(PANO_ERR_INVALID, "null argument");
     if (density_in == density_out || velocity_in == velocity_out || density_out == velocity_out)
         return fail(PANO_ERR_INVALID, "step outputs must not alias its inputs");
+    if (pressure && (pressure == density_out || pressure == velocity_out))
+        return fail(PANO_ERR_INVALID, "pressure must not alias density_out or velocity_out");
     PANO_HIP(hipSetDevice(c->device));
     Dims D = dims_of(g);
     if (g->dtype == PANO_F64)

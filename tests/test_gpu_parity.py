@@ -306,13 +306,24 @@ def test_gpu_pcg_not_converged_and_nonfinite(ctx):
 
 # ---------------------------------------------------------------- the fluid step
 
+# The exported pressure is the solve's last iterate. GPU and oracle stop at
+# the same relative residual with different summation orders, so their
+# iterates differ by the solver's error, about cond(A) * tol (up to ~3e-4
+# relative on the 128^2 box), far above rounding but far below what a wrong
+# layout or a stale export would give.
+P_TOL = 2e-3
+
+
 def run_steps_gpu(g, cfg, rho, faces, steps):
+    """(density, faces, iterations, pressure) after each step; the pressure
+    is the step's optional natural-layout export."""
     r0, v0 = dev(g, rho), dev(g, faces, "simplex1")
     r1, v1 = g.new_simplex_2(), g.new_simplex_1()
+    p = g.new_simplex_2()
     out = []
     for _ in range(steps):
-        info = pa.fluid_step(g, cfg, r0, v0, r1, v1)
-        out.append((r1.download(), v1.download(), info.iterations))
+        info = pa.fluid_step(g, cfg, r0, v0, r1, v1, pressure=p)
+        out.append((r1.download(), v1.download(), info.iterations, p.download()))
         r0, r1, v0, v1 = r1, r0, v1, v0
     return out
 
@@ -331,12 +342,13 @@ def test_gpu_step_dec2d_128_matches_oracle(ctx):
     P64 = o.params(wl["dims"], dt=wl["dt"], precond="jacobi", tol=1e-6, max_iterations=500)
     r32, f32 = rho.copy(), faces.copy()
     r64, f64 = rho.astype(np.float64), faces.astype(np.float64)
-    for k, (gr, gf, it) in enumerate(got):
-        _, info = o.step(P, r32, f32)
+    for k, (gr, gf, it, gp) in enumerate(got):
+        p32, info = o.step(P, r32, f32)
         o.step(P64, r64, f64)
         assert abs(it - info.iterations) <= 1, k
         assert np.abs(gr - r32).max() <= 1e-5 * np.abs(r32).max(), k
         assert np.abs(gf - f32).max() <= 1e-5 * np.abs(f32).max(), k
+        assert np.abs(gp - p32).max() <= P_TOL * np.abs(p32).max(), k
         # error of the f32 GPU path against the f64 oracle, stated bound
         assert np.abs(gf - f64).max() <= 1e-4 * np.abs(f64).max(), k
 
@@ -353,11 +365,24 @@ def test_gpu_step_smoke3d_small_matches_oracle(ctx):
     P = o.params(wl["dims"], bc=wl["bc"], dt=cfg.dt, precond="mg", levels=3, tol=1e-6, max_iterations=500,
                  buoyancy=1.0, src_center=cfg.src_center, src_radius=cfg.src_radius)
     r32, f32 = rho.copy(), faces.copy()
-    for k, (gr, gf, it) in enumerate(got):
-        _, info = o.step(P, r32, f32)
+    for k, (gr, gf, it, gp) in enumerate(got):
+        p32, info = o.step(P, r32, f32)
         assert abs(it - info.iterations) <= 1, k
         assert np.abs(gr - r32).max() <= 1e-5 * max(1.0, np.abs(r32).max()), k
         assert np.abs(gf - f32).max() <= 1e-5 * np.abs(f32).max(), k
+        assert np.abs(gp - p32).max() <= P_TOL * np.abs(p32).max(), k
+
+
+def test_gpu_step_rejects_aliased_pressure(ctx):
+    from panopaea_amd import workloads as W
+    wl = dict(W.DEC2D_128, dims=(16, 16))
+    g = W.make_grid(wl)
+    cfg = W.step_config(wl)
+    r0, v0, r1, v1 = g.new_simplex_2(), g.new_simplex_1(), g.new_simplex_2(), g.new_simplex_1()
+    for alias in (r1, v1):
+        with pytest.raises(pa.PanoError) as e:
+            pa.fluid_step(g, cfg, r0, v0, r1, v1, pressure=alias)
+        assert e.value.status == pa._lib.PANO_ERR_INVALID
 
 
 def test_gpu_step_smoke3d_256_properties(ctx):
