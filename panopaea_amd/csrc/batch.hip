@@ -128,24 +128,6 @@ __global__ __launch_bounds__(256) void kb_project(BTab T, char* A, int axis) {
 
 // ---- solver kernels --------------------------------------------------------------------------
 
-__global__ void kb_state_init(BTab T, SolverState* st, int max_iter) {
-    if (threadIdx.x != 0) return;
-    SolverState s = {};
-    s.max_iter = max_iter;
-    st[T.e[blockIdx.x].sidx] = s;
-}
-
-template <typename Tr>
-__global__ void kb_round_threshold(BTab T, SolverState* st) {
-    if (threadIdx.x != 0) return;
-    SolverState* s = &st[T.e[blockIdx.x].sidx];
-    s->threshold = (double)(Tr)s->threshold;
-    if (!s->done && s->b_linf < s->threshold) {  // pcg.rs:35-38
-        s->converged = 1;
-        s->done = 1;
-    }
-}
-
 // One workgroup per grid: reduces the grid's partials of the launch `tab`
 // (blocks [start, start + G.nb)) and updates its state.
 __global__ __launch_bounds__(1024) void kb_finalize(int op, BTab T, const double* part0, const double* part1,
@@ -1047,15 +1029,12 @@ pano_status BatchCore::vcycle(bool with_dot) {
 // threshold from ||b||_inf, z = M r, p = z, sigma = z.r.
 pano_status BatchCore::enqueue_init() {
     if (flat_.h.empty()) return PANO_OK;
-    hipLaunchKernelGGL(kb_state_init, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream, flat_.view(), states,
-                       max_iterations);
+    // FIN_THRESH writes each grid's whole initial state (limit, rounded threshold, early out)
     dim3 blk(kBlock, 1);
     BL(kb_zero_stats, flat_, arena, part0_, part1_);
     fin(FIN_STATS, part0_, nullptr, flat_, 0);
     BL(kb_sub_mean, flat_, arena, (const SolverState*)states, part1_);
     fin(FIN_THRESH, part1_, nullptr, flat_, 1);
-    hipLaunchKernelGGL(kb_round_threshold<float>, dim3((unsigned)flat_.h.size()), dim3(64), 0, c->stream,
-                       flat_.view(), states);
     PANO_CHECK(vcycle(false));
     blk = dim3(kBlock, 1);
     BL(kb_copy_dot, flat_, arena, (const SolverState*)states, part0_);
@@ -1291,6 +1270,8 @@ pano_status pano_batch_create(pano_ctx ctx, const pano_batch_grid* grids, int32_
         cfg[i].tol = P->tol;
         cfg[i].n = hg.cells;
         cfg[i].remove_mean = hg.D.bc == 0;
+        cfg[i].max_iter = P->max_iterations;
+        cfg[i].round_f32 = 1;
     }
     B->arena_bytes = off;
     if (hipMalloc(&B->arena, (size_t)off) != hipSuccess) {

@@ -181,6 +181,8 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     S->cfg.tol_kind = P->tol_kind;
     S->cfg.tol = P->threshold;
     S->cfg.n = S->n;
+    S->cfg.max_iter = P->max_iterations;
+    S->cfg.round_f32 = sizeof(T) == 4;
     int rm = P->remove_mean;
     S->cfg.remove_mean = rm < 0 ? (D.bc == 0) : rm;
     return PANO_OK;
@@ -223,15 +225,16 @@ static pano_status solver_init(Solver<T>* S, int max_iterations) {
     c->timer.checks = 0;
     S->it = 0;
     if (!S->q) S->q = S->aux;
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, st, S->st, max_iterations);
-    hipLaunchKernelGGL((k_zero<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x);  // pcg.rs:32
-    hipLaunchKernelGGL((k_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->part0, S->part1);
+    // FIN_THRESH writes the whole initial state (limit, rounded threshold, early out)
+    S->cfg.max_iter = max_iterations;
+    S->cfg.round_f32 = sizeof(T) == 4;
+    hipLaunchKernelGGL((k_zero_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x, (const T*)S->r,
+                       S->part0, S->part1);
     if (S->cfg.remove_mean) {
         fin(S, FIN_STATS, S->part0, S->nflat, 0);
         hipLaunchKernelGGL((k_sub_mean<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->st, S->part1);
     }
     fin(S, FIN_THRESH, S->part1, S->nflat, 1);
-    hipLaunchKernelGGL((k_round_threshold<T>), dim3(1), dim3(64), 0, st, S->st);
     // z = M r ; p = z ; sigma = z.r   (pcg.rs:41-46)
     T* z = S->precond == PANO_PRECOND_IDENTITY ? S->r : S->aux;
     int np = 0;
@@ -584,11 +587,12 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     cfg2.tol = M->outer_tol;
     cfg2.n = n;
     cfg2.remove_mean = ip.remove_mean < 0 ? (D.bc == 0) : ip.remove_mean;
+    cfg2.max_iter = M->max_outer + 1;
+    cfg2.round_f32 = 0;
     Grid3 Gn = grid3(D.w, D.h, D.d);
     hipLaunchKernelGGL((k_to_rb<double>), dim3(Gn.nb), dim3(kTX, kTY), 0, s, D, Gn, bb, b);
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st2, M->max_outer + 1);
-    hipLaunchKernelGGL((k_zero<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, xx);
-    hipLaunchKernelGGL((k_stats<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, (const double*)bb, S.part0, part);
+    hipLaunchKernelGGL((k_zero_stats<double>), dim3(S.nflat), dim3(kBlock), 0, s, n, xx, (const double*)bb, S.part0,
+                       part);
     if (cfg2.remove_mean) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_STATS, (const double*)S.part0,
                            (const double*)nullptr, S.nflat,

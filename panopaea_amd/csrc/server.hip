@@ -171,6 +171,8 @@ static pano_status admit(Server* S, Request* r, std::vector<Request*>& fresh) {
     cf.remove_mean = r->g.D.bc == 0;
     cf.tol = S->P.tol;
     cf.n = r->g.cells;
+    cf.max_iter = S->P.max_iterations;
+    cf.round_f32 = 1;
     PANO_CHECK(stage_upload(S, S->core_all.cfgs + r->slot, &cf, sizeof(SolveCfg)));
     float* u = reinterpret_cast<float*>(S->arena + r->g.o_u[0]);
     if (!r->faces.empty()) {

@@ -19,6 +19,8 @@ struct SolveCfg {
     int32_t remove_mean;
     double tol;
     int64_t n;
+    int32_t max_iter;   // operator applications allowed
+    int32_t round_f32;  // the stop threshold is rounded to f32 (the solve's T is float)
 };
 
 // Reduces the FP64 partials part0[0..np) (+ part1[0..np) when part1 is not
@@ -48,20 +50,25 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
         case FIN_STATS:  // tot = sum(b): the mean to remove
             st->mean = tot / (double)cfg.n;
             break;
-        case FIN_THRESH: {  // tot = ||b||_inf (after mean removal)
-            st->b_linf = tot;
-            st->rnorm = tot;
+        case FIN_THRESH: {  // tot = ||b||_inf (after mean removal): the solve's initial state
+            SolverState s = {};
+            s.max_iter = cfg.max_iter;
+            s.mean = cfg.remove_mean ? st->mean : 0.0;  // FIN_STATS ran first when removing it
+            s.b_linf = tot;
+            s.rnorm = tot;
             double thr = cfg.tol_kind == PANO_TOL_ABS_LINF ? cfg.tol : cfg.tol * tot;
-            st->threshold = thr;
-            st->iter = 0;
-            st->done = 0;
-            st->converged = 0;
-            st->nonfinite = 0;
+            // the stop test compares T values: the threshold is rounded to T
+            // like the reference's `threshold: T`
+            s.threshold = cfg.round_f32 ? (double)(float)thr : thr;
             if (!(tot == tot) || tot > 1.7e308) {
-                st->nonfinite = 1;
-                st->done = 1;
+                s.nonfinite = 1;
+                s.done = 1;
+            } else if (s.b_linf < s.threshold) {  // pcg.rs:35-38
+                s.converged = 1;
+                s.done = 1;
             }
-            break;  // the early-out test runs after rounding thr to T
+            *st = s;
+            break;
         }
         case FIN_SIGMA0:
             st->sigma = tot;
@@ -102,18 +109,6 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
     }
 }
 
-// Threshold rounding: the stop test compares T values, so the threshold is
-// rounded to T like the reference's `threshold: T`.
-template <typename T>
-__global__ void k_round_threshold(SolverState* st) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->threshold = (double)(T)st->threshold;
-        if (!st->done && st->b_linf < st->threshold) {  // pcg.rs:35-38
-            st->converged = 1;
-            st->done = 1;
-        }
-    }
-}
 
 static __global__ void k_state_init(SolverState* st, int max_iter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -329,8 +324,10 @@ __device__ __forceinline__ void flat_search(int lb, int64_t n, T* p, const T* z,
 }
 
 // ---- single-grid wrappers ---------------------------------------------------------------
+// x = 0 (pcg.rs:32) and the partial sum and max|.| of r = b
 template <typename T>
-__global__ __launch_bounds__(256) void k_stats(int64_t n, const T* r, double* psum, double* pmax) {
+__global__ __launch_bounds__(256) void k_zero_stats(int64_t n, T* x, const T* r, double* psum, double* pmax) {
+    flat_zero(blockIdx.x, n, x);
     double s, m;
     flat_stats(blockIdx.x, n, r, &s, &m);
     if (threadIdx.x == 0) {
