@@ -158,14 +158,11 @@ __global__ __launch_bounds__(256) void kb_sub_mean(BTab T, char* A,
     double m = flat_sub_mean(lb, n, fld<float>(A, E.o[1]), st[E.sidx].mean);
     if (threadIdx.x == 0) pmax[b_] = m;
 }
-__global__ __launch_bounds__(256) void kb_copy_dot(BTab T, char* A,
-                                                   const SolverState* st, double* part) {
+__global__ __launch_bounds__(256) void kb_copy(BTab T, char* A, const SolverState* st) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
     int64_t n = (int64_t)E.D.d * E.D.h * E.D.w;
-    double t = flat_copy_dot(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]),
-                             (const float*)fld<float>(A, E.o[1]));
-    if (threadIdx.x == 0) part[b_] = t;
+    flat_copy(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]));
 }
 __global__ __launch_bounds__(256) void kb_update(BTab T, char* A, const SolverState* st,
                                                  double* pmax) {
@@ -1035,10 +1032,11 @@ pano_status BatchCore::enqueue_init() {
     fin(FIN_STATS, part0_, nullptr, flat_, 0);
     BL(kb_sub_mean, flat_, arena, (const SolverState*)states, part1_);
     fin(FIN_THRESH, part1_, nullptr, flat_, 1);
-    PANO_CHECK(vcycle(false));
+    // z = M r with its z.r partials (sigma reduced as the iterations reduce it), p = z
+    PANO_CHECK(vcycle(true));
+    fin(FIN_SIGMA0, part0_, part2_, dot_, 0);
     blk = dim3(kBlock, 1);
-    BL(kb_copy_dot, flat_, arena, (const SolverState*)states, part0_);
-    fin(FIN_SIGMA0, part0_, nullptr, flat_, 0);
+    BL(kb_copy, flat_, arena, (const SolverState*)states);
     PANO_HIP(hipGetLastError());
     return PANO_OK;
 }

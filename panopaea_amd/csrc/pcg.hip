@@ -236,19 +236,21 @@ static pano_status solver_init(Solver<T>* S, int max_iterations) {
     }
     fin(S, FIN_THRESH, S->part1, S->nflat, 1);
     // z = M r ; p = z ; sigma = z.r   (pcg.rs:41-46)
-    T* z = S->precond == PANO_PRECOND_IDENTITY ? S->r : S->aux;
-    int np = 0;
     if (S->precond == PANO_PRECOND_IDENTITY) {
         hipLaunchKernelGGL((k_copy_dot<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, S->r, S->r, S->part0,
                            S->st);
-        np = S->nflat;
+        fin(S, FIN_SIGMA0, S->part0, S->nflat, 0);
     } else {
-        PANO_CHECK(precond_dot<T>(S, z, S->r, S->part0, &np));
-        hipLaunchKernelGGL((k_copy_dot<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, z, S->r, S->part0,
-                           S->st);
-        np = S->nflat;
+        // z goes to aux and becomes p by exchanging the two buffers' roles (both
+        // are scratch of the solve; q shares the other one); sigma from the
+        // preconditioner's z.r partials, reduced as the iterations reduce them
+        int np = 0;
+        PANO_CHECK(precond_dot<T>(S, S->aux, S->r, S->part0, &np));
+        std::swap(S->p, S->aux);
+        S->q = S->aux;
+        if (S->rb) fin(S, FIN_SIGMA0, S->part0, np / 2, 0, S->part0 + np / 2);
+        else fin(S, FIN_SIGMA0, S->part0, np, 0);
     }
-    fin(S, FIN_SIGMA0, S->part0, np, 0);
     PANO_HIP(hipGetLastError());
     return PANO_OK;
 }

@@ -166,6 +166,23 @@ __device__ __forceinline__ void flat_zero(int lb, int64_t n, T* x) {
     }
 }
 
+// p = z
+template <typename T>
+__device__ __forceinline__ void flat_copy(int lb, int64_t n, T* p, const T* z) {
+    const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T v[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        if (i < n) v[k] = ldx(z, i);
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = base + k * kBlock;
+        if (i < n) stx(p, i, v[k]);
+    }
+}
+
 // p = z ; returns the block's z.r
 template <typename T>
 __device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T* z, const T* r) {
