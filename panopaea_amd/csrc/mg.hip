@@ -477,6 +477,25 @@ __global__ __launch_bounds__(kFdmThreads) void k_fdm_plane(int nx, int ny, int m
                               eps);
 }
 
+// The first pass of the 3D coarsest solve with its right-hand side restricted
+// from the fine residual (red-black layout, level dims Df) on the fly:
+// plane K's element (J, I) is restrict_value3, the value k_restrict stores,
+// so the coarse right-hand side is never written.
+__global__ __launch_bounds__(kFdmThreads) void k_fdm_plane_rc(Dims Df, Dims Dc, const float* res, double* out,
+                                                      const double* Vx, const double* Vy, double s, double eps,
+                                                      const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    const int nx = Dc.w, ny = Dc.h, K = blockIdx.x;
+    fdm_plane_body_src<double>(
+        fdm_lds, nx, ny, 0,
+        [&](int e) {
+            const int J = e / nx, I = e - J * nx;
+            return (double)restrict_value3<float>(Df, Dc, res, I, J, K);
+        },
+        out + (int64_t)K * nx * ny, Vx, Vy, (const double*)nullptr, (const double*)nullptr, s, eps);
+}
+
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(256) void k_fdm2_rows(int nx, int ny, int inverse, const Tin* in, Tout* out,
                                                    const double* Vx, const SolverState* st) {
@@ -500,6 +519,23 @@ __global__ __launch_bounds__(kFdmThreads) void k_fdm_zcol(int nx, int ny, int nz
     fdm_zcol_body(fdm_lds, nx, ny, nz, blockIdx.x, data, Vz, lx, ly, lz, s, eps);
 }
 
+// 3D f32 with the fused passes on: the restriction into the coarsest level is
+// done by the coarsest solve's first pass (k_fdm_plane_rc) when that level
+// takes the in-LDS plane path and the level above restricts a stored residual.
+template <typename T, int DIM>
+static bool coarse_rc(const MG<T>* m) {
+    if constexpr (!std::is_same<T, float>::value || DIM != 3) {
+        return false;
+    } else {
+        const int Lc = m->levels - 1;
+        const Dims& C = m->dims[Lc];
+        return m->tb && Lc >= 1 && !rr_on(m->tb, m->dims[Lc - 1]) && C.w * C.h <= kFdmPlane &&
+               C.w * C.d <= kFdmPlane;
+    }
+}
+
+// r: the coarsest right-hand side, or with coarse_rc the fine residual of
+// level levels-2 to restrict on the fly.
 template <typename T, int DIM>
 static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const SolverState* st) {
     const Dims& C = m->dims[m->levels - 1];
@@ -527,8 +563,17 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     if (DIM == 3 && nx * ny <= kFdmPlane && nx * nz <= kFdmPlane) {
         size_t lds = kFdmLdsBytes;
         {
-            hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 0, r, t0, Vx, Vy,
-                               (const double*)nullptr, (const double*)nullptr, s, eps, st);
+            if constexpr (std::is_same<T, float>::value) {
+                if (coarse_rc<T, DIM>(m))
+                    hipLaunchKernelGGL(k_fdm_plane_rc, dim3(nz), dim3(kFdmThreads), lds, S, m->dims[m->levels - 2],
+                                       C, (const float*)r, t0, Vx, Vy, s, eps, st);
+                else
+                    hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 0, r,
+                                       t0, Vx, Vy, (const double*)nullptr, (const double*)nullptr, s, eps, st);
+            } else {
+                hipLaunchKernelGGL((k_fdm_plane<T, double>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 0, r, t0,
+                                   Vx, Vy, (const double*)nullptr, (const double*)nullptr, s, eps, st);
+            }
             hipLaunchKernelGGL(k_fdm_zcol, dim3(ny), dim3(kFdmThreads), lds, S, nx, ny, nz, t0, Vz, lx, ly, lz, s, eps,
                                st);
             hipLaunchKernelGGL((k_fdm_plane<double, T>), dim3(nz), dim3(kFdmThreads), lds, S, nx, ny, 1, t0, z, Vx, Vy,
@@ -630,6 +675,7 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
     }
     klaunch(c, l == 0 ? "residual_l0" : "", 3 * unit, k_mg_residual<T, DIM, NZ>, dim3(G.nb), B, D, G, m->res, zl[l],
             rl[l], s, st);
+    if (l + 1 == Lc && coarse_rc<T, DIM>(m)) return;  // restricted by the coarsest solve's first pass
     const char* nr = l == 0 ? "restrict_l0" : "";
     const int nk = restrict_run_len(Dc);
     Grid3 Gc = grid3(Dc.w, Dc.h, (Dc.d + nk - 1) / nk);
@@ -728,7 +774,7 @@ static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const Solver
             default: level_down<T, DIM, 1>(c, m, l, zl, rl, st); break;
         }
     }
-    PANO_CHECK(coarse_solve<T, DIM>(c, m, m->z[Lc], m->r[Lc], st));
+    PANO_CHECK(coarse_solve<T, DIM>(c, m, m->z[Lc], coarse_rc<T, DIM>(m) ? (const T*)m->res : m->r[Lc], st));
     for (int l = Lc - 1; l >= 0; --l) {
         switch (run_len(m->dims[l])) {
             case 4: level_up<T, DIM, 4>(c, m, l, zl, rl, st, dot_part, dot_np); break;

@@ -75,9 +75,11 @@ __host__ __device__ inline int restrict_run_len(const Dims& Dc) {
 // the grid are read at a clamped, valid position and r1 discards them (its
 // hm/hp flags). Planes shared by neighbouring coarse cells are loaded and
 // reduced once.
-template <typename T, int DIM, int LC, int NK>
-__device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* rc, const T* res, int I, int J,
-                                             int K0) {
+// restrict_gather: the x-then-y restricted values ty[p] of the fine planes
+// p of that support; restrict_run combines them along z and stores.
+template <typename T, int DIM, int NK>
+__device__ __forceinline__ void restrict_gather(const Dims& Df, const Dims& Dc, const T* res, int I, int J, int K0,
+                                                T* ty) {
     constexpr int NP = DIM == 3 ? 2 * NK + 2 : 1;  // fine planes 2K0-1 .. 2K0+2NK (2D: the one plane)
     const bool hmI = I > 0, hpI = I < Dc.w - 1, hmJ = J > 0, hpJ = J < Dc.h - 1;
     const int wh = Df.w >> 1, hw = Df.h * wh, n2 = Df.d * hw;
@@ -96,7 +98,6 @@ __device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* 
             v[p][q][2] = ldx(res, io);                  // x = 2I+1
             v[p][q][3] = ldx(res, ie + (hpI ? 1 : 0));  // x = 2I+2
         }
-    T ty[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         T t[4];
@@ -104,6 +105,13 @@ __device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* 
         for (int q = 0; q < 4; ++q) t[q] = r1(v[p][q][0], v[p][q][1], v[p][q][2], v[p][q][3], hmI, hpI);
         ty[p] = r1(t[0], t[1], t[2], t[3], hmJ, hpJ);
     }
+}
+
+template <typename T, int DIM, int LC, int NK>
+__device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* rc, const T* res, int I, int J,
+                                             int K0) {
+    T ty[DIM == 3 ? 2 * NK + 2 : 1];
+    restrict_gather<T, DIM, NK>(Df, Dc, res, I, J, K0, ty);
     if constexpr (DIM == 3) {
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
@@ -115,6 +123,14 @@ __device__ __forceinline__ void restrict_run(const Dims& Df, const Dims& Dc, T* 
     } else {
         stx(rc, cidx<LC>(Dc, 0, J, I), ty[0]);
     }
+}
+
+// The value restrict_run<T, 3, *, 1> stores for coarse cell (K, J, I).
+template <typename T>
+__device__ __forceinline__ T restrict_value3(const Dims& Df, const Dims& Dc, const T* res, int I, int J, int K) {
+    T ty[4];
+    restrict_gather<T, 3, 1>(Df, Dc, res, I, J, K, ty);
+    return r1(ty[0], ty[1], ty[2], ty[3], K > 0, K < Dc.d - 1);
 }
 
 // Prolongation weights for fine index i over a coarse axis of n cells.
@@ -476,14 +492,15 @@ __device__ __forceinline__ void lds_axis(const double* a, double* b, int nx, int
 //  mode 0: forward x then y (input Tin, output double)
 //  mode 1: inverse y then x (input double, output Tout)
 //  mode 2: the whole 2D solve: forward x, y, divide by s*lambda, inverse y, x
-template <typename Tin, typename Tout>
-__device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int mode, const Tin* src, Tout* dst,
-                                               const double* Vx, const double* Vy, const double* lx,
-                                               const double* ly, double s, double eps) {
+// src(e): the input value of plane element e, as a double.
+template <typename Tout, typename Src>
+__device__ __forceinline__ void fdm_plane_body_src(double* lds, int nx, int ny, int mode, Src src, Tout* dst,
+                                                   const double* Vx, const double* Vy, const double* lx,
+                                                   const double* ly, double s, double eps) {
     double* a = lds;
     double* b = lds + nx * ny;
     int P = nx * ny;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = (double)src[e];
+    for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = src(e);
     double* vs = lds + 2 * kFdmPlane;
     const double* Vxg = Vx;
     Vx = stage_basis(vs, 0, Vx, nx);
@@ -519,6 +536,13 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
     }
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) dst[e] = (Tout)a[e];
+}
+template <typename Tin, typename Tout>
+__device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int mode, const Tin* src, Tout* dst,
+                                               const double* Vx, const double* Vy, const double* lx,
+                                               const double* ly, double s, double eps) {
+    fdm_plane_body_src<Tout>(
+        lds, nx, ny, mode, [src](int e) { return (double)src[e]; }, dst, Vx, Vy, lx, ly, s, eps);
 }
 
 // 2D coarsest level as three passes over lines, each output element summed in

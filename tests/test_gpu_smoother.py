@@ -1,8 +1,10 @@
-"""The one-pass pre/post-smoothing of f32 multigrid levels (3D and 2D), and the
-one-pass residual + restriction of 3D levels of 2^23 cells and more, against the
-half-sweep, residual and restriction kernels (PANO_MG_FUSED=0): both give the
-same V-cycle bit for bit (each cell is computed from the same six neighbour
-values in the same order, each coarse value from the same 64 residuals).
+"""The one-pass pre/post-smoothing of f32 multigrid levels (3D and 2D), the
+one-pass residual + restriction of 3D levels of 2^23 cells and more, and the
+restriction into the 3D coarsest level done by the coarsest solve's first pass,
+against the half-sweep, residual and restriction kernels (PANO_MG_FUSED=0):
+both give the same V-cycle bit for bit (each cell is computed from the same six
+neighbour values in the same order, each coarse value from the same 64
+residuals).
 The PCG's z.r partial sums are added in another order, so solves agree to
 rounding and take the same number of iterations.
 """
