@@ -141,58 +141,54 @@ __global__ __launch_bounds__(256) void k_neg_div_masked2(Dims D, Grid3 G, T* b, 
     b[(int64_t)y * w + x] = -d;
 }
 
-// pout (optional): the threads of faces x < w also copy cell (z, y, x) of p
-// into pout in natural layout, the pressure export without a pass of its own.
+// u += dt*g on every face (dec_fluid.rs:124-141), one thread per run of NZ
+// cells along z (planes z0 + k, row y, column x). A thread projects the faces
+// on the low x, y and z sides of its cells, and the far wall's face where its
+// cell is the last along an axis; each face is proj_*_face of the pressures
+// on its two sides, so every cell's pressure is loaded once for its three
+// faces (the neighbours' from cache). Every face is read and written by one
+// thread only. pout (optional): a natural-layout copy of p, the step's
+// pressure export without a pass of its own.
 template <typename T, int DIM, int L_>
-__global__ __launch_bounds__(256) void k_project_x(Dims D, Grid3 G, T* ux, const T* p, T dt, T* pout) {
+__global__ __launch_bounds__(256) void k_project_cells(Dims D, Grid3 G, Faces<T> u, const T* p, T dt, T* pout) {
     constexpr int NZ = kPlaneRun<DIM>;
-    PLANE_RUN_BEGIN(NZ);
-    if (x > D.w || y >= D.h) return;
-    T out[NZ], pc[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        int z = PLANE_Z(k, D.d);
-        pc[k] = ldx(p, lidx<L_>(D, z, y, imin(x, D.w - 1)));
-        out[k] = project_x_pc<T, L_>(D, ldx(ux, fx_idx(D, z, y, x)), p, dt, z, y, x, pc[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) stx(ux, fx_idx(D, z0 + k, y, x), out[k]);
-    if (pout && x < D.w) {
-#pragma unroll
-        for (int k = 0; k < NZ; ++k)
-            if (z0 + k < D.d) stx(pout, cell_index(D, z0 + k, y, x), pc[k]);
-    }
-}
-template <typename T, int DIM, int L_>
-__global__ __launch_bounds__(256) void k_project_y(Dims D, Grid3 G, T* uy, const T* p, T dt) {
-    constexpr int NZ = kPlaneRun<DIM>;
-    PLANE_RUN_BEGIN(NZ);
-    if (x >= D.w || y > D.h) return;
-    T out[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        int z = PLANE_Z(k, D.d);
-        out[k] = project_y<T, L_>(D, ldx(uy, fy_idx(D, z, y, x)), p, dt, z, y, x);
-    }
-#pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) stx(uy, fy_idx(D, z0 + k, y, x), out[k]);
-}
-template <typename T, int L_>
-__global__ __launch_bounds__(256) void k_project_z(Dims D, Grid3 G, T* uz, const T* p, T dt) {
-    constexpr int NZ = kPlaneRun<3>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    T out[NZ];
+    const bool xw = x == D.w - 1, yh = y == D.h - 1;
+    const int xm = imax(x - 1, 0), ym = imax(y - 1, 0);
+    T pc[NZ], px[NZ], py[NZ], ux[NZ], uy[NZ], uxw[NZ], uyh[NZ], uz[NZ];
+    T pz = (T)0, uzd = (T)0;
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        int z = PLANE_Z(k, D.d + 1);
-        out[k] = project_z<T, L_>(D, ldx(uz, fz_idx(D, z, y, x)), p, dt, z, y, x);
+    for (int k = 0; k < NZ; ++k) {  // every load first (planes clamped to the last)
+        const int z = PLANE_Z(k, D.d);
+        pc[k] = ldx(p, lidx<L_>(D, z, y, x));
+        px[k] = ldx(p, lidx<L_>(D, z, y, xm));
+        py[k] = ldx(p, lidx<L_>(D, z, ym, x));
+        ux[k] = ldx(u.x, fx_idx(D, z, y, x));
+        uy[k] = ldx(u.y, fy_idx(D, z, y, x));
+        uxw[k] = xw ? ldx(u.x, fx_idx(D, z, y, D.w)) : (T)0;
+        uyh[k] = yh ? ldx(u.y, fy_idx(D, z, D.h, x)) : (T)0;
+        uz[k] = DIM == 3 ? ldx(u.z, fz_idx(D, z, y, x)) : (T)0;
+    }
+    const bool zd = DIM == 3 && z0 + NZ >= D.d;  // the run holds plane d-1: it also does face d
+    if constexpr (DIM == 3) {
+        pz = ldx(p, lidx<L_>(D, imax(z0 - 1, 0), y, x));
+        if (zd) uzd = ldx(u.z, fz_idx(D, D.d, y, x));
     }
 #pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        if (z0 + k <= D.d) stx(uz, fz_idx(D, z0 + k, y, x), out[k]);
+    for (int k = 0; k < NZ; ++k) {
+        const int z = z0 + k;
+        if (z >= D.d) continue;
+        stx(u.x, fx_idx(D, z, y, x), proj_x_face(D, ux[k], dt, x, px[k], pc[k]));
+        if (xw) stx(u.x, fx_idx(D, z, y, D.w), proj_x_face(D, uxw[k], dt, D.w, pc[k], pc[k]));
+        stx(u.y, fy_idx(D, z, y, x), proj_y_face(D, uy[k], dt, y, py[k], pc[k]));
+        if (yh) stx(u.y, fy_idx(D, z, D.h, x), proj_y_face(D, uyh[k], dt, D.h, pc[k], pc[k]));
+        if constexpr (DIM == 3) {
+            stx(u.z, fz_idx(D, z, y, x), proj_z_face(D, uz[k], dt, z, k == 0 ? pz : pc[k > 0 ? k - 1 : 0], pc[k]));
+            if (z == D.d - 1) stx(u.z, fz_idx(D, D.d, y, x), proj_z_face(D, uzd, dt, D.d, pc[k], pc[k]));
+        }
+        if (pout) stx(pout, cell_index(D, z, y, x), pc[k]);
+    }
 }
 
 static Sphere make_sphere(const double* s) {
@@ -294,16 +290,12 @@ pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t
 template <typename T, int L_>
 static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
     Faces<T> u = split_faces(D, faces);
-    const int nz = D.d > 1 ? kPlaneRun<3> : 1;
-    Grid3 Gx = grid3_run(D.w + 1, D.h, D.d, nz), Gy = grid3_run(D.w, D.h + 1, D.d, nz);
     if (D.d > 1) {
-        Grid3 Gz = grid3_run(D.w, D.h, D.d + 1, nz);
-        LAUNCH3((k_project_x<T, 3, L_>), Gx, D, Gx, u.x, p, (T)dt, p_out);
-        LAUNCH3((k_project_y<T, 3, L_>), Gy, D, Gy, u.y, p, (T)dt);
-        LAUNCH3((k_project_z<T, L_>), Gz, D, Gz, u.z, p, (T)dt);
+        Grid3 Gc = grid3_run(D.w, D.h, D.d, kPlaneRun<3>);
+        LAUNCH3((k_project_cells<T, 3, L_>), Gc, D, Gc, u, p, (T)dt, p_out);
     } else {
-        LAUNCH3((k_project_x<T, 2, L_>), Gx, D, Gx, u.x, p, (T)dt, p_out);
-        LAUNCH3((k_project_y<T, 2, L_>), Gy, D, Gy, u.y, p, (T)dt);
+        Grid3 Gc = grid3_run(D.w, D.h, 1, 1);
+        LAUNCH3((k_project_cells<T, 2, L_>), Gc, D, Gc, u, p, (T)dt, p_out);
     }
     return PANO_OK;
 }

@@ -564,14 +564,14 @@ __device__ __forceinline__ int ncount(const Dims& D, int z, int y, int x) {
 // ---- projection (dec_fluid.rs:124-141) ----------------------------------------------------
 // u = u + dt*g, g = derivative_0_dual(p): x faces p[x-1]-p[x], y/z faces
 // -(p[+]-p[-]); closed-wall faces set to zero, open faces use p = 0 outside.
-// pc = p at (z, y, min(x, w-1)): the second operand of an interior face and
-// the only value an open boundary face reads (cell 0 at x = 0, cell w-1 at
-// x = w), passed in so a caller can reuse the load.
-template <typename T, int L>
-__device__ __forceinline__ T project_x_pc(const Dims& D, T u, const T* p, T dt, int z, int y, int x, T pc) {
+// The face formulas of the projection from the two pressures a face reads:
+// pc = p of the cell on its high side (the last cell for the far wall's face)
+// and pm = p of the cell below it (read by interior faces only).
+template <typename T>
+__host__ __device__ __forceinline__ T proj_x_face(const Dims& D, T u, T dt, int x, T pm, T pc) {
     const T zero = (T)0;
     T g;
-    if (x > 0 && x < D.w) g = ldx(p, lidx<L>(D, z, y, x - 1)) - pc;
+    if (x > 0 && x < D.w) g = pm - pc;
     else if (x == 0) {
         if (!(D.bc & 1)) return zero;
         g = zero - pc;
@@ -581,37 +581,53 @@ __device__ __forceinline__ T project_x_pc(const Dims& D, T u, const T* p, T dt, 
     }
     return u + dt * g;
 }
+template <typename T>
+__host__ __device__ __forceinline__ T proj_y_face(const Dims& D, T u, T dt, int y, T pm, T pc) {
+    const T zero = (T)0;
+    T g;
+    if (y > 0 && y < D.h) g = -(pc - pm);
+    else if (y == 0) {
+        if (!(D.bc & 4)) return zero;
+        g = -(pc - zero);
+    } else {
+        if (!(D.bc & 8)) return zero;
+        g = -(zero - pc);
+    }
+    return u + dt * g;
+}
+template <typename T>
+__host__ __device__ __forceinline__ T proj_z_face(const Dims& D, T u, T dt, int z, T pm, T pc) {
+    const T zero = (T)0;
+    T g;
+    if (z > 0 && z < D.d) g = -(pc - pm);
+    else if (z == 0) {
+        if (!(D.bc & 16)) return zero;
+        g = -(pc - zero);
+    } else {
+        if (!(D.bc & 32)) return zero;
+        g = -(zero - pc);
+    }
+    return u + dt * g;
+}
+
+// One face from the pressure field p (layout L).
 template <typename T, int L>
 __device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    return project_x_pc<T, L>(D, u, p, dt, z, y, x, ldx(p, lidx<L>(D, z, y, imin(x, D.w - 1))));
+    const T pc = ldx(p, lidx<L>(D, z, y, imin(x, D.w - 1)));
+    const T pm = (x > 0 && x < D.w) ? ldx(p, lidx<L>(D, z, y, x - 1)) : pc;
+    return proj_x_face(D, u, dt, x, pm, pc);
 }
 template <typename T, int L>
 __device__ __forceinline__ T project_y(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    const T zero = (T)0;
-    T g;
-    if (y > 0 && y < D.h) g = -(ldx(p, lidx<L>(D, z, y, x)) - ldx(p, lidx<L>(D, z, y - 1, x)));
-    else if (y == 0) {
-        if (!(D.bc & 4)) return zero;
-        g = -(ldx(p, lidx<L>(D, z, 0, x)) - zero);
-    } else {
-        if (!(D.bc & 8)) return zero;
-        g = -(zero - ldx(p, lidx<L>(D, z, D.h - 1, x)));
-    }
-    return u + dt * g;
+    const T pc = ldx(p, lidx<L>(D, z, imin(y, D.h - 1), x));
+    const T pm = (y > 0 && y < D.h) ? ldx(p, lidx<L>(D, z, y - 1, x)) : pc;
+    return proj_y_face(D, u, dt, y, pm, pc);
 }
 template <typename T, int L>
 __device__ __forceinline__ T project_z(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    const T zero = (T)0;
-    T g;
-    if (z > 0 && z < D.d) g = -(ldx(p, lidx<L>(D, z, y, x)) - ldx(p, lidx<L>(D, z - 1, y, x)));
-    else if (z == 0) {
-        if (!(D.bc & 16)) return zero;
-        g = -(ldx(p, lidx<L>(D, 0, y, x)) - zero);
-    } else {
-        if (!(D.bc & 32)) return zero;
-        g = -(zero - ldx(p, lidx<L>(D, D.d - 1, y, x)));
-    }
-    return u + dt * g;
+    const T pc = ldx(p, lidx<L>(D, imin(z, D.d - 1), y, x));
+    const T pm = (z > 0 && z < D.d) ? ldx(p, lidx<L>(D, z - 1, y, x)) : pc;
+    return proj_z_face(D, u, dt, z, pm, pc);
 }
 
 // ---- tiled 3D launch: 64 x 4 threads over (x, y) of one z plane --------------------------
