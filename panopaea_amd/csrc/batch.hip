@@ -104,26 +104,19 @@ __global__ __launch_bounds__(256) void kb_neg_div(BTab T, char* A) {
     fld<float>(A, E.o[0])[rb_index(D, z, y, x)] = -dv;
 }
 
-// u += dt d0_dual(p) with p red-black: o[0] = u, o[1] = p
-__global__ __launch_bounds__(256) void kb_project(BTab T, char* A, int axis) {
+// u += dt d0_dual(p) with p red-black, one thread per cell (its low faces and
+// the far walls' faces, project_cells_run): o[0] = u, o[1] = p
+__global__ __launch_bounds__(256) void kb_project(BTab T, char* A) {
     BATCH_ENTRY(T);
     int x, y, z;
     local_xyz(E.G, lb, x, y, z);
     const Dims& D = E.D;
-    if (x >= D.w + (axis == 0) || y >= D.h + (axis == 1)) return;
+    if (x >= D.w || y >= D.h) return;
     Faces<float> u = split_faces(D, fld<float>(A, E.o[0]));
     const float* p = fld<float>(A, E.o[1]);
-    float dt = (float)E.s;
-    if (axis == 0) {
-        int64_t i = fx_idx(D, z, y, x);
-        u.x[i] = project_x<float, REDBLACK>(D, u.x[i], p, dt, z, y, x);
-    } else if (axis == 1) {
-        int64_t i = fy_idx(D, z, y, x);
-        u.y[i] = project_y<float, REDBLACK>(D, u.y[i], p, dt, z, y, x);
-    } else {
-        int64_t i = fz_idx(D, z, y, x);
-        u.z[i] = project_z<float, REDBLACK>(D, u.z[i], p, dt, z, y, x);
-    }
+    const float dt = (float)E.s;
+    if (D.d > 1) project_cells_run<float, 3, REDBLACK, 1>(D, u, p, dt, (float*)nullptr, z, y, x);
+    else project_cells_run<float, 2, REDBLACK, 1>(D, u, p, dt, (float*)nullptr, z, y, x);
 }
 
 // ---- solver kernels --------------------------------------------------------------------------
@@ -590,9 +583,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     reset(flat_);
     for (int p = 0; p < 2; ++p) {
         reset(div_[p]);
-        reset(prx_[p]);
-        reset(pry_[p]);
-        reset(prz_[p]);
+        reset(prc_[p]);
     }
     for (int l = 0; l < 16; ++l) {
         reset(rb_[l]);
@@ -664,17 +655,10 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
             dv.o[0] = g.o_r;
             dv.o[1] = g.o_u[p];
             div_[p].h.push_back(dv);
-            BEntry px = fx_.h.back(), py = fy_.h.back();
-            px.o[0] = py.o[0] = g.o_u[p];
-            px.o[1] = py.o[1] = g.o_x;
-            prx_[p].h.push_back(px);
-            pry_[p].h.push_back(py);
-            if (D.d > 1) {
-                BEntry pz = fz_.h.back();
-                pz.o[0] = g.o_u[p];
-                pz.o[1] = g.o_x;
-                prz_[p].h.push_back(pz);
-            }
+            BEntry pc = e;  // the projection: natural cell tiles too
+            pc.o[0] = g.o_u[p];
+            pc.o[1] = g.o_x;
+            prc_[p].h.push_back(pc);
         }
         // flat vectors
         BEntry v = e;
@@ -836,7 +820,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
                    table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
     for (int p = 0; p < 2; ++p)
-        total += table_bytes(div_[p]) + table_bytes(prx_[p]) + table_bytes(pry_[p]) + table_bytes(prz_[p]);
+        total += table_bytes(div_[p]) + table_bytes(prc_[p]);
     for (int l = 0; l + 1 < levels_; ++l) {
         total += table_bytes(rb_[l]) + table_bytes(rbs_[l]) + table_bytes(co_[l]) + table_bytes(prl_[l]) +
                  table_bytes(rsd_[l]) + table_bytes(rr2_[l]);
@@ -865,9 +849,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     PANO_CHECK(place(flat_));
     for (int p = 0; p < 2; ++p) {
         PANO_CHECK(place(div_[p]));
-        PANO_CHECK(place(prx_[p]));
-        PANO_CHECK(place(pry_[p]));
-        PANO_CHECK(place(prz_[p]));
+        PANO_CHECK(place(prc_[p]));
     }
     for (int l = 0; l + 1 < levels_; ++l) {
         PANO_CHECK(place(rb_[l]));
@@ -919,9 +901,7 @@ pano_status BatchCore::enqueue_divergence(int ping) {
 
 pano_status BatchCore::enqueue_project(int ping) {
     dim3 blk(shapes.nat_x, shapes.nat_y);
-    BL(kb_project, prx_[ping], arena, 0);
-    BL(kb_project, pry_[ping], arena, 1);
-    BL(kb_project, prz_[ping], arena, 2);
+    BL(kb_project, prc_[ping], arena);
     PANO_HIP(hipGetLastError());
     return PANO_OK;
 }

@@ -144,7 +144,7 @@ private:
     Table nat_;            // level-0 cells, natural tiles (advection)
     Table fx_, fy_, fz_;   // face tiles per component (advection)
     Table div_[2];         // natural tiles: o[0] = r, o[1] = u[ping]
-    Table prx_[2], pry_[2], prz_[2];  // face tiles of the projection: o[0] = u[ping], o[1] = x
+    Table prc_[2];         // natural tiles of the projection: o[0] = u[ping], o[1] = x
     Table flat_;           // flat vector blocks (2048 cells per block)
     Table rb_[16];         // red-black pair tiles per level
     Table rbs_[16];        // the same, for the grids smoothed by half-sweeps at the level

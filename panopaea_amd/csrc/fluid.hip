@@ -141,54 +141,14 @@ __global__ __launch_bounds__(256) void k_neg_div_masked2(Dims D, Grid3 G, T* b, 
     b[(int64_t)y * w + x] = -d;
 }
 
-// u += dt*g on every face (dec_fluid.rs:124-141), one thread per run of NZ
-// cells along z (planes z0 + k, row y, column x). A thread projects the faces
-// on the low x, y and z sides of its cells, and the far wall's face where its
-// cell is the last along an axis; each face is proj_*_face of the pressures
-// on its two sides, so every cell's pressure is loaded once for its three
-// faces (the neighbours' from cache). Every face is read and written by one
-// thread only. pout (optional): a natural-layout copy of p, the step's
-// pressure export without a pass of its own.
+// The projection of every face (project_cells_run), one thread per run of
+// kPlaneRun cells along z; pout (optional): the natural-layout pressure export.
 template <typename T, int DIM, int L_>
 __global__ __launch_bounds__(256) void k_project_cells(Dims D, Grid3 G, Faces<T> u, const T* p, T dt, T* pout) {
     constexpr int NZ = kPlaneRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
-    const bool xw = x == D.w - 1, yh = y == D.h - 1;
-    const int xm = imax(x - 1, 0), ym = imax(y - 1, 0);
-    T pc[NZ], px[NZ], py[NZ], ux[NZ], uy[NZ], uxw[NZ], uyh[NZ], uz[NZ];
-    T pz = (T)0, uzd = (T)0;
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {  // every load first (planes clamped to the last)
-        const int z = PLANE_Z(k, D.d);
-        pc[k] = ldx(p, lidx<L_>(D, z, y, x));
-        px[k] = ldx(p, lidx<L_>(D, z, y, xm));
-        py[k] = ldx(p, lidx<L_>(D, z, ym, x));
-        ux[k] = ldx(u.x, fx_idx(D, z, y, x));
-        uy[k] = ldx(u.y, fy_idx(D, z, y, x));
-        uxw[k] = xw ? ldx(u.x, fx_idx(D, z, y, D.w)) : (T)0;
-        uyh[k] = yh ? ldx(u.y, fy_idx(D, z, D.h, x)) : (T)0;
-        uz[k] = DIM == 3 ? ldx(u.z, fz_idx(D, z, y, x)) : (T)0;
-    }
-    const bool zd = DIM == 3 && z0 + NZ >= D.d;  // the run holds plane d-1: it also does face d
-    if constexpr (DIM == 3) {
-        pz = ldx(p, lidx<L_>(D, imax(z0 - 1, 0), y, x));
-        if (zd) uzd = ldx(u.z, fz_idx(D, D.d, y, x));
-    }
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        const int z = z0 + k;
-        if (z >= D.d) continue;
-        stx(u.x, fx_idx(D, z, y, x), proj_x_face(D, ux[k], dt, x, px[k], pc[k]));
-        if (xw) stx(u.x, fx_idx(D, z, y, D.w), proj_x_face(D, uxw[k], dt, D.w, pc[k], pc[k]));
-        stx(u.y, fy_idx(D, z, y, x), proj_y_face(D, uy[k], dt, y, py[k], pc[k]));
-        if (yh) stx(u.y, fy_idx(D, z, D.h, x), proj_y_face(D, uyh[k], dt, D.h, pc[k], pc[k]));
-        if constexpr (DIM == 3) {
-            stx(u.z, fz_idx(D, z, y, x), proj_z_face(D, uz[k], dt, z, k == 0 ? pz : pc[k > 0 ? k - 1 : 0], pc[k]));
-            if (z == D.d - 1) stx(u.z, fz_idx(D, D.d, y, x), proj_z_face(D, uzd, dt, D.d, pc[k], pc[k]));
-        }
-        if (pout) stx(pout, cell_index(D, z, y, x), pc[k]);
-    }
+    project_cells_run<T, DIM, L_, NZ>(D, u, p, dt, pout, z0, y, x);
 }
 
 static Sphere make_sphere(const double* s) {

@@ -630,6 +630,52 @@ __device__ __forceinline__ T project_z(const Dims& D, T u, const T* p, T dt, int
     return proj_z_face(D, u, dt, z, pm, pc);
 }
 
+// u += dt*g on every face (dec_fluid.rs:124-141) for a run of NZ cells along
+// z (planes z0 + k, row y, column x; planes past the last are skipped). The
+// run's thread projects the faces on the low x, y and z sides of its cells,
+// and the far wall's face where a cell is the last along an axis; each face
+// is proj_*_face of the pressures on its two sides, so every cell's pressure
+// is loaded once for its three faces (the neighbours' from cache). Every face
+// belongs to one run only. pout (optional): a natural-layout copy of p.
+template <typename T, int DIM, int L, int NZ>
+__device__ __forceinline__ void project_cells_run(const Dims& D, const Faces<T>& u, const T* p, T dt, T* pout,
+                                                  int z0, int y, int x) {
+    const bool xw = x == D.w - 1, yh = y == D.h - 1;
+    const int xm = imax(x - 1, 0), ym = imax(y - 1, 0);
+    T pc[NZ], px[NZ], py[NZ], ux[NZ], uy[NZ], uxw[NZ], uyh[NZ], uz[NZ];
+    T pz = (T)0, uzd = (T)0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {  // every load first (planes clamped to the last)
+        const int z = imin(z0 + k, D.d - 1);
+        pc[k] = ldx(p, lidx<L>(D, z, y, x));
+        px[k] = ldx(p, lidx<L>(D, z, y, xm));
+        py[k] = ldx(p, lidx<L>(D, z, ym, x));
+        ux[k] = ldx(u.x, fx_idx(D, z, y, x));
+        uy[k] = ldx(u.y, fy_idx(D, z, y, x));
+        uxw[k] = xw ? ldx(u.x, fx_idx(D, z, y, D.w)) : (T)0;
+        uyh[k] = yh ? ldx(u.y, fy_idx(D, z, D.h, x)) : (T)0;
+        uz[k] = DIM == 3 ? ldx(u.z, fz_idx(D, z, y, x)) : (T)0;
+    }
+    if constexpr (DIM == 3) {
+        pz = ldx(p, lidx<L>(D, imax(z0 - 1, 0), y, x));
+        if (z0 + NZ >= D.d) uzd = ldx(u.z, fz_idx(D, D.d, y, x));  // the run holds plane d-1: face d
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        const int z = z0 + k;
+        if (z >= D.d) continue;
+        stx(u.x, fx_idx(D, z, y, x), proj_x_face(D, ux[k], dt, x, px[k], pc[k]));
+        if (xw) stx(u.x, fx_idx(D, z, y, D.w), proj_x_face(D, uxw[k], dt, D.w, pc[k], pc[k]));
+        stx(u.y, fy_idx(D, z, y, x), proj_y_face(D, uy[k], dt, y, py[k], pc[k]));
+        if (yh) stx(u.y, fy_idx(D, z, D.h, x), proj_y_face(D, uyh[k], dt, D.h, pc[k], pc[k]));
+        if constexpr (DIM == 3) {
+            stx(u.z, fz_idx(D, z, y, x), proj_z_face(D, uz[k], dt, z, k == 0 ? pz : pc[k > 0 ? k - 1 : 0], pc[k]));
+            if (z == D.d - 1) stx(u.z, fz_idx(D, D.d, y, x), proj_z_face(D, uzd, dt, D.d, pc[k], pc[k]));
+        }
+        if (pout) stx(pout, cell_index(D, z, y, x), pc[k]);
+    }
+}
+
 // ---- tiled 3D launch: 64 x 4 threads over (x, y) of one z plane --------------------------
 constexpr int kTX = 64, kTY = 4;
 struct Grid3 {
