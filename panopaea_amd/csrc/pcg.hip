@@ -171,6 +171,9 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     S->G = S->rb ? grid3_rb(D) : grid3(D.w, D.h, D.d);
     S->ntile = S->G.nb;
     int np = S->nflat > 2 * S->ntile ? S->nflat : 2 * S->ntile;
+    // the fused step's divergence may hand over one max|b| partial per tile
+    const Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
+    if (Gd.nb > np) np = Gd.nb;
     void *st, *p0, *p1;
     PANO_CHECK(c->workspace("pcg_state", sizeof(SolverState), &st));
     PANO_CHECK(c->workspace("pcg_part0", (size_t)np * sizeof(double), &p0));
@@ -216,9 +219,11 @@ static pano_status precond_dot(Solver<T>* S, T* z, const T* r, double* part, int
     return PANO_OK;
 }
 
-// Initialise from b already in S->r (in the solver's layout).
+// Initialise from b already in S->r (in the solver's layout). rhs_parts > 0:
+// the caller has already set x = 0 and left rhs_parts max|b| partials in
+// S->part1 (only without mean removal).
 template <typename T>
-static pano_status solver_init(Solver<T>* S, int max_iterations) {
+static pano_status solver_init(Solver<T>* S, int max_iterations, int rhs_parts = 0) {
     Ctx* c = S->c;
     hipStream_t st = c->stream;
     c->timer.solve = (int)c->timer.solve_iters.size();
@@ -228,13 +233,18 @@ static pano_status solver_init(Solver<T>* S, int max_iterations) {
     // FIN_THRESH writes the whole initial state (limit, rounded threshold, early out)
     S->cfg.max_iter = max_iterations;
     S->cfg.round_f32 = sizeof(T) == 4;
-    hipLaunchKernelGGL((k_zero_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x, (const T*)S->r,
-                       S->part0, S->part1);
-    if (S->cfg.remove_mean) {
-        fin(S, FIN_STATS, S->part0, S->nflat, 0);
-        hipLaunchKernelGGL((k_sub_mean<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->st, S->part1);
+    if (rhs_parts > 0 && !S->cfg.remove_mean) {
+        fin(S, FIN_THRESH, S->part1, rhs_parts, 1);
+    } else {
+        hipLaunchKernelGGL((k_zero_stats<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->x, (const T*)S->r,
+                           S->part0, S->part1);
+        if (S->cfg.remove_mean) {
+            fin(S, FIN_STATS, S->part0, S->nflat, 0);
+            hipLaunchKernelGGL((k_sub_mean<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->r, S->st,
+                               S->part1);
+        }
+        fin(S, FIN_THRESH, S->part1, S->nflat, 1);
     }
-    fin(S, FIN_THRESH, S->part1, S->nflat, 1);
     // z = M r ; p = z ; sigma = z.r   (pcg.rs:41-46)
     if (S->precond == PANO_PRECOND_IDENTITY) {
         hipLaunchKernelGGL((k_copy_dot<T>), dim3(S->nflat), dim3(kBlock), 0, st, S->n, S->p, S->r, S->r, S->part0,
@@ -427,19 +437,34 @@ template pano_status pcg_solve<double>(Ctx*, const Dims&, const pano_pcg_params*
 // ---- the fused fluid step ----------------------------------------------------------------
 
 // b = -div u written straight into the solver's layout (natural or red-black).
+// x0 (optional): also x = 0 at the same cells (pcg.rs:32); pmax (optional):
+// one max|b| partial per workgroup, the solve's threshold input when the mean
+// is not removed (a max does not depend on the order of its terms).
 template <typename T, int DIM, int L_>
-__global__ __launch_bounds__(256) void k_neg_div_to(Dims D, Grid3 G, T* b, Faces<const T> v) {
+__global__ __launch_bounds__(256) void k_neg_div_to(Dims D, Grid3 G, T* b, Faces<const T> v, T* x0,
+                                                    double* pmax) {
     constexpr int NZ = kPlaneRun<DIM>;
     int x, y, zb, L;
     tile_xyz(G, x, y, zb, L);
-    if (x >= D.w || y >= D.h) return;
     const int z0 = zb * NZ;
-    T out[NZ];
+    double m = 0.0;
+    if (x < D.w && y < D.h) {
+        T out[NZ];
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, imin(z0 + k, D.d - 1), y, x);
+        for (int k = 0; k < NZ; ++k) out[k] = -div_value<T, DIM>(D, v, imin(z0 + k, D.d - 1), y, x);
 #pragma unroll
-    for (int k = 0; k < NZ; ++k)
-        if (z0 + k < D.d) stx(b, lidx<L_>(D, z0 + k, y, x), out[k]);
+        for (int k = 0; k < NZ; ++k) {
+            if (z0 + k >= D.d) continue;
+            const int i = lidx<L_>(D, z0 + k, y, x);
+            stx(b, i, out[k]);
+            if (x0) stx(x0, i, (T)0);
+            m = absmax_acc(m, (double)out[k]);
+        }
+    }
+    if (pmax) {  // uniform: every thread of the workgroup takes part
+        const double t = block_max(m);
+        if (threadIdx.x == 0 && threadIdx.y == 0) pmax[L] = t;
+    }
 }
 
 template <typename T>
@@ -469,15 +494,20 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     // b = -div u
     Faces<const T> v = split_faces(D, (const T*)vel_out);
     Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
+    // without mean removal the divergence also sets x = 0 and hands over the
+    // max|b| partials of the stop threshold (no pass over b of the solver's own)
+    const bool rhs_fused = !S.cfg.remove_mean;
+    T* x0 = rhs_fused ? S.x : nullptr;
+    double* pm = rhs_fused ? S.part1 : nullptr;
     if (S.rb) {
-        if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gd, D, Gd, S.r, v);
-        else L3((k_neg_div_to<T, 2, REDBLACK>), Gd, D, Gd, S.r, v);
+        if (D.d > 1) L3((k_neg_div_to<T, 3, REDBLACK>), Gd, D, Gd, S.r, v, x0, pm);
+        else L3((k_neg_div_to<T, 2, REDBLACK>), Gd, D, Gd, S.r, v, x0, pm);
     } else {
-        if (D.d > 1) L3((k_neg_div_to<T, 3, NATURAL>), Gd, D, Gd, S.r, v);
-        else L3((k_neg_div_to<T, 2, NATURAL>), Gd, D, Gd, S.r, v);
+        if (D.d > 1) L3((k_neg_div_to<T, 3, NATURAL>), Gd, D, Gd, S.r, v, x0, pm);
+        else L3((k_neg_div_to<T, 2, NATURAL>), Gd, D, Gd, S.r, v, x0, pm);
     }
     PANO_HIP(hipGetLastError());
-    PANO_CHECK(solver_init<T>(&S, pp.max_iterations));
+    PANO_CHECK(solver_init<T>(&S, pp.max_iterations, rhs_fused ? (int)Gd.nb : 0));
     PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, hint_key(D, &pp, sizeof(T))));
     // u += dt d0_dual(p), closed walls zeroed; the x component's pass also
     // writes the pressure out in natural layout
