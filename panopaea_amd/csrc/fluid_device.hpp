@@ -610,26 +610,6 @@ __host__ __device__ __forceinline__ T proj_z_face(const Dims& D, T u, T dt, int 
     return u + dt * g;
 }
 
-// One face from the pressure field p (layout L).
-template <typename T, int L>
-__device__ __forceinline__ T project_x(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    const T pc = ldx(p, lidx<L>(D, z, y, imin(x, D.w - 1)));
-    const T pm = (x > 0 && x < D.w) ? ldx(p, lidx<L>(D, z, y, x - 1)) : pc;
-    return proj_x_face(D, u, dt, x, pm, pc);
-}
-template <typename T, int L>
-__device__ __forceinline__ T project_y(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    const T pc = ldx(p, lidx<L>(D, z, imin(y, D.h - 1), x));
-    const T pm = (y > 0 && y < D.h) ? ldx(p, lidx<L>(D, z, y - 1, x)) : pc;
-    return proj_y_face(D, u, dt, y, pm, pc);
-}
-template <typename T, int L>
-__device__ __forceinline__ T project_z(const Dims& D, T u, const T* p, T dt, int z, int y, int x) {
-    const T pc = ldx(p, lidx<L>(D, imin(z, D.d - 1), y, x));
-    const T pm = (z > 0 && z < D.d) ? ldx(p, lidx<L>(D, z - 1, y, x)) : pc;
-    return proj_z_face(D, u, dt, z, pm, pc);
-}
-
 // u += dt*g on every face (dec_fluid.rs:124-141) for a run of NZ cells along
 // z (planes z0 + k, row y, column x; planes past the last are skipped). The
 // run's thread projects the faces on the low x, y and z sides of its cells,

@@ -109,7 +109,6 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
     }
 }
 
-
 static __global__ void k_state_init(SolverState* st, int max_iter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         SolverState s = {};
@@ -118,7 +117,6 @@ static __global__ void k_state_init(SolverState* st, int max_iter) {
     }
 }
 
-// partial sum and max|.| of r (the right-hand side)
 // ---- flat vector kernels: block lb of a field of n elements covers
 // [lb*kFlatPer, (lb+1)*kFlatPer) with kItems strided elements per thread ------------------
 
@@ -360,11 +358,6 @@ __global__ __launch_bounds__(256) void k_sub_mean(int64_t n, T* r, const SolverS
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_zero(int64_t n, T* x) {
-    flat_zero(blockIdx.x, n, x);
-}
-
-template <typename T>
 __global__ __launch_bounds__(256) void k_copy_dot(int64_t n, T* p, const T* z, const T* r, double* part,
                                                   const SolverState* st) {
     if (st->done) return;
@@ -378,26 +371,6 @@ __global__ __launch_bounds__(256) void k_dot(int64_t n, const T* a, const T* b, 
     if (st->done) return;
     double t = flat_dot(blockIdx.x, n, a, b);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-// x += alpha p ; r -= alpha q ; partial ||r||_inf (and r.r for the identity
-// preconditioner, where z aliases r).
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(int64_t n, T* x, T* r, const T* p, const T* q, double* pmax,
-                                                double* prr, const SolverState* st) {
-    if (st->done) return;
-    double m, s;
-    flat_update(blockIdx.x, n, x, r, p, q, st->alpha, prr != nullptr, &m, &s);
-    if (threadIdx.x == 0) {
-        pmax[blockIdx.x] = m;
-        if (prr) prr[blockIdx.x] = s;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_search(int64_t n, T* p, const T* z, const SolverState* st) {
-    if (st->done) return;
-    flat_search(blockIdx.x, n, p, z, st->beta);
 }
 
 // The single-grid loop's update and search: x += alpha p moves from the
