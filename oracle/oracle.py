@@ -291,6 +291,13 @@ def solve(P, b):
     return x, info, b
 
 
+def source_sphere(P, rho):
+    """The density source in place: rho = 1 in cells whose centre lies in the
+    sphere of P (the analogue of dec_fluid.rs:48-57)."""
+    _fn("or_source_sphere", rho.dtype)(ctypes.byref(P), _p(rho))
+    return rho
+
+
 def step(P, rho, faces):
     """One fluid step in place on (rho, faces). Returns (p, info)."""
     vz, vy, vx = split_faces(P, faces)

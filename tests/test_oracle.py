@@ -466,6 +466,228 @@ def test_3d_mac_z_invariant_reproduces_2d():
     assert np.all(oz3 == 0)
 
 
+# ---------------------------------------------------------------- 3D z axis
+# The reference is 2D only; its trilinear interpolation (math/interp.rs:23-36)
+# has no caller there. The z parts of the 3D advection, the buoyancy term and
+# the sphere source are therefore pinned by what the mathematics fixes: the
+# symmetry between axes, exact integer shifts with hand-computed clamps,
+# exactness on linear fields and hand-built buoyancy fields.
+
+def _swap_zx(a, shape):
+    """Array of `shape` (z, y, x) flattened -> the same data with z and x exchanged."""
+    return np.ascontiguousarray(a.reshape(shape).transpose(2, 1, 0)).ravel()
+
+
+def _swap_faces_zx(P, Ps, faces):
+    """Faces of grid P in the layout of the z<->x swapped grid Ps: the x-face
+    component becomes the z-face component and vice versa."""
+    d, h, w = P.d, P.h, P.w
+    vz, vy, vx = o.split_faces(P, faces)
+    out = np.zeros(o.nfaces(Ps), faces.dtype)
+    sz, sy, sx = o.split_faces(Ps, out)
+    sz[:] = _swap_zx(vx, (d, h, w + 1))
+    sy[:] = _swap_zx(vy, (d, h + 1, w))
+    sx[:] = _swap_zx(vz, (d + 1, h, w))
+    return out
+
+
+@pytest.mark.parametrize("dims", [(5, 6, 7), (4, 3, 9)])
+def test_3d_advection_symmetric_under_z_x_swap(dims):
+    d, h, w = dims
+    P, Ps = o.params(dims), o.params((w, h, d))
+    rng = np.random.default_rng(21)
+    vel = rng.uniform(-4, 4, o.nfaces(P))  # departures up to several cells, clamped at every wall
+    q = rng.standard_normal(d * h * w)
+    src = rng.standard_normal(o.nfaces(P))
+    vel_s, src_s = _swap_faces_zx(P, Ps, vel), _swap_faces_zx(P, Ps, src)
+    # cells: the trilinear nesting order changes with the axes (x, then y, then z), so the
+    # two agree to rounding; a wrong z index, offset or clamp differs by O(1)
+    a = o.advect3(P, q, 0.8, vel)
+    b = _swap_zx(o.advect3(Ps, _swap_zx(q, (d, h, w)), 0.8, vel_s), (w, h, d))
+    assert np.abs(a - b).max() < 1e-13
+    # faces: z faces of the swapped grid are the x faces here, and vice versa
+    m = o.advect_mac3(P, src, 0.8, vel)
+    ms = o.advect_mac3(Ps, src_s, 0.8, vel_s)
+    assert np.abs(_swap_faces_zx(P, Ps, m) - ms).max() < 1e-13
+
+
+def _uniform_vel3(P, U, V, W):
+    return _uniform_faces(P, U, V, W)
+
+
+@pytest.mark.parametrize("k", [2, -2, 3])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_3d_integer_z_shift_is_exact_with_hand_clamps(k, dtype):
+    d, h, w = 7, 5, 6
+    P = o.params((d, h, w))
+    rng = np.random.default_rng(5)
+    vel = _uniform_vel3(P, 0.0, 0.0, float(k)).astype(dtype)
+    q = rng.standard_normal(d * h * w).astype(dtype)
+    out = o.advect3(P, q, 1.0, vel).reshape(d, h, w)
+    qq = q.reshape(d, h, w)
+    # cells (dec_fluid.rs:194-201 per axis): departure z - k, clamped to [0, d - 1.00001];
+    # x and y stay on their cell, except the far column/row, blended by the 1.00001 clamp
+    for z in range(d):
+        zs = z - k
+        if zs <= 0:
+            want = qq[0]
+        elif zs < d - 1:
+            want = qq[zs]
+        else:
+            f = dtype(dtype(d) - dtype(1.00001)) - dtype(d - 2)
+            want = qq[d - 2] * (dtype(1) - f) + qq[d - 1] * f
+        assert np.array_equal(out[z, :h - 1, :w - 1], want[:h - 1, :w - 1]), z
+    # faces (dec_fluid.rs:236-244 per axis): index clamps only, so every face is an exact copy;
+    # the z faces take face clip(z - k, 0, d), the x and y faces cell layer clip(z - k, 0, d - 1)
+    src = rng.standard_normal(o.nfaces(P)).astype(dtype)
+    m = o.advect_mac3(P, src, 1.0, vel)
+    sz, sy, sx = o.split_faces(P, src)
+    mz, my, mx = o.split_faces(P, m)
+    sz, mz = sz.reshape(d + 1, h, w), mz.reshape(d + 1, h, w)
+    sy, my = sy.reshape(d, h + 1, w), my.reshape(d, h + 1, w)
+    sx, mx = sx.reshape(d, h, w + 1), mx.reshape(d, h, w + 1)
+    for z in range(d + 1):
+        assert np.array_equal(mz[z], sz[min(max(z - k, 0), d)]), z
+    for z in range(d):
+        zc = min(max(z - k, 0), d - 1)
+        assert np.array_equal(my[z], sy[zc]) and np.array_equal(mx[z], sx[zc]), z
+
+
+def test_3d_linear_fields_reproduced_under_fractional_velocity():
+    d, h, w = 8, 7, 9
+    P = o.params((d, h, w))
+    U, V, W, dt = 0.7, -1.3, 1.9, 0.45
+    vel = _uniform_vel3(P, U, V, W)
+    a, b, c, e = 0.6, -1.1, 2.3, 0.4
+
+    def lin(x, y, z):
+        return a * x + b * y + c * z + e
+
+    # cells at (x+1/2, y+1/2, z+1/2); the gathered corners are cell centres
+    z, y, x = np.mgrid[0:d, 0:h, 0:w]
+    q = lin(x + 0.5, y + 0.5, z + 0.5).ravel()
+    out = o.advect3(P, q, dt, vel).reshape(d, h, w)
+    want = lin(x + 0.5 - dt * U, y + 0.5 - dt * V, z + 0.5 - dt * W)
+    # interior: departure points whose cells and upper neighbours lie inside
+    inner = (slice(2, d - 1), slice(1, h - 2), slice(1, w - 1))
+    assert np.abs(out[inner] - want[inner]).max() < 1e-12
+    # faces: each component linear at its own face centres
+    src = np.zeros(o.nfaces(P))
+    sz, sy, sx = o.split_faces(P, src)
+    zz, yz, xz = np.mgrid[0:d + 1, 0:h, 0:w]
+    sz[:] = lin(xz + 0.5, yz + 0.5, zz).ravel()
+    zy, yy, xy = np.mgrid[0:d, 0:h + 1, 0:w]
+    sy[:] = lin(xy + 0.5, yy, zy + 0.5).ravel()
+    zx, yx, xx = np.mgrid[0:d, 0:h, 0:w + 1]
+    sx[:] = lin(xx, yx + 0.5, zx + 0.5).ravel()
+    m = o.advect_mac3(P, src, dt, vel)
+    mz, my, mx = o.split_faces(P, m)
+    wz = lin(xz + 0.5 - dt * U, yz + 0.5 - dt * V, zz - dt * W)
+    wy = lin(xy + 0.5 - dt * U, yy - dt * V, zy + 0.5 - dt * W)
+    wx = lin(xx - dt * U, yx + 0.5 - dt * V, zx + 0.5 - dt * W)
+    iz = (slice(1, d), slice(1, h - 2), slice(1, w - 1))
+    iy = (slice(1, d - 1), slice(1, h), slice(1, w - 1))
+    ix = (slice(1, d - 1), slice(1, h - 2), slice(1, w))
+    assert np.abs(mz.reshape(d + 1, h, w)[iz] - wz[iz]).max() < 1e-12
+    assert np.abs(my.reshape(d, h + 1, w)[iy] - wy[iy]).max() < 1e-12
+    assert np.abs(mx.reshape(d, h, w + 1)[ix] - wx[ix]).max() < 1e-12
+
+
+def _sphere_cells(dims, center, radius):
+    """Brute force: cells whose centre lies in the closed ball, by integer-free
+    distance tests in exact rational terms (centres and radius are multiples of 1/4)."""
+    d, h, w = dims
+    q = 4  # work in quarter cells: every quantity below is an integer
+    cz, cy, cx = (int(round(c * q)) for c in center)
+    r = int(round(radius * q))
+    mask = np.zeros(dims, bool)
+    for z in range(d):
+        for y in range(h):
+            for x in range(w):
+                dz, dy, dx = (2 * z + 1) * q // 2 - cz, (2 * y + 1) * q // 2 - cy, (2 * x + 1) * q // 2 - cx
+                mask[z, y, x] = dz * dz + dy * dy + dx * dx <= r * r
+    return mask
+
+
+@pytest.mark.parametrize("center,radius", [((2.5, 8.0, 8.0), 3.25), ((0.75, 5.5, 6.0), 2.5), ((4.0, 4.0, 4.0), 3.0)])
+def test_source_sphere_matches_brute_force(center, radius):
+    dims = (8, 12, 13)
+    for dtype in (np.float32, np.float64):
+        P = o.params(dims, src_center=center, src_radius=radius)
+        rho = np.full(int(np.prod(dims)), 0.25, dtype)
+        o.source_sphere(P, rho)
+        mask = _sphere_cells(dims, center, radius)
+        assert mask.sum() > 3
+        want = np.where(mask, dtype(1), dtype(0.25)).ravel()
+        assert np.array_equal(rho, want)
+
+
+def _buoyancy_faces(P, rho, dt, beta):
+    """Hand-built z faces after one step's advection at zero velocity from zero
+    faces: dt*beta*(rho[z-1] + rho[z])/2 on interior faces, dt*beta*rho/2 on an
+    open top or bottom (the density outside is 0), 0 on closed ones."""
+    d, h, w = P.d, P.h, P.w
+    r = rho.reshape(d, h, w)
+    f = np.zeros(o.nfaces(P))
+    vz, _, _ = o.split_faces(P, f)
+    vz = vz.reshape(d + 1, h, w)
+    vz[1:d] = dt * beta * (r[:-1] + r[1:]) / 2
+    if P.bc[4]:
+        vz[0] = dt * beta * r[0] / 2
+    if P.bc[5]:
+        vz[d] = dt * beta * r[d - 1] / 2
+    return f
+
+
+@pytest.mark.parametrize("bc", [(0, 0, 0, 0, 0, 1), (0, 0, 0, 0, 1, 0), (0, 0, 0, 0, 1, 1)])
+def test_buoyancy_uniform_density_closed_forms(bc):
+    d, h, w = 6, 5, 4
+    dt, beta, rho0 = 0.25, 1.5, 0.5
+    P = o.params((d, h, w), bc=bc)
+    rho = np.full(d * h * w, rho0)
+    zero = np.zeros(o.nfaces(P))
+    # the step's coefficient: dt*beta/2 times the sum of the two adjacent densities
+    m = o.advect_mac3(P, zero, dt, zero, buoy_c=dt * beta * 0.5, rho=rho)
+    mz, my, mx = o.split_faces(P, m)
+    mz = mz.reshape(d + 1, h, w)
+    assert np.all(mz[1:d] == dt * beta * rho0)              # interior: dt*beta*rho0
+    assert np.all(mz[0] == (dt * beta * rho0 / 2 if bc[4] else 0.0))
+    assert np.all(mz[d] == (dt * beta * rho0 / 2 if bc[5] else 0.0))
+    assert np.all(my == 0) and np.all(mx == 0)
+    assert np.array_equal(m, _buoyancy_faces(P, rho, dt, beta))
+
+
+@pytest.mark.parametrize("center,radius", [((2.0, 4.0, 4.0), 2.25), ((3.0, 4.0, 7.5), 1.0)])
+def test_step_buoyancy_uses_the_source_density_before_advection(center, radius):
+    # zero velocity, zero density: the step's source makes the sphere, whose density
+    # (before advection) drives the hand-built buoyancy field; the step then projects it.
+    # The second sphere lies in the last x column only, where even zero-velocity
+    # advection changes the density (the 1.00001 clamp), so a step that took the
+    # advected density for the buoyancy would differ there.
+    dims, bc = (8, 8, 8), (0, 0, 0, 0, 0, 1)
+    dt, beta = 0.25, 1.0
+    P = o.params(dims, bc=bc, dt=dt, precond="jacobi", tol=1e-10, max_iterations=2000, buoyancy=beta,
+                 src_center=center, src_radius=radius)
+    n = int(np.prod(dims))
+    rho = np.zeros(n)
+    faces = np.zeros(o.nfaces(P))
+    p, info = o.step(P, rho, faces)
+    assert info.converged
+    src = _sphere_cells(dims, center, radius).astype(np.float64).ravel()
+    # the density at zero velocity: the source, except the far x/y column and row,
+    # which the 1.00001 position clamp blends with their neighbours
+    r3, s3 = rho.reshape(dims), src.reshape(dims)
+    assert np.array_equal(r3[:-1, :-1, :-1], s3[:-1, :-1, :-1])
+    pre = _buoyancy_faces(P, src, dt, beta)
+    b = o.neg_div(P, pre)
+    x, _, b_used = o.solve(P, b)
+    want = o.grad_sub(P, pre, x)
+    assert np.array_equal(faces, want)
+    assert np.array_equal(p, x)
+    # sanity: the sphere pushes upward and nothing moves sideways before projection
+    assert np.abs(b).max() > 0
+
+
 # ---------------------------------------------------------------- step
 
 @pytest.mark.parametrize("dims,bc,precond", [((32, 32), (0,) * 6, "jacobi"), ((8, 16, 16), (0, 0, 0, 0, 0, 1), "mg")])
