@@ -10,10 +10,16 @@ returns when the step is complete).
 Prints one JSON line: cell-updates/s over the timed steps, step latency
 p50/p99, the roofline of the dominant kernel (algorithmic bytes per launch /
 average launch duration measured with HIP events on the library's stream
-during the timed steps), and the CPU oracle timed on a bounded sample of the
-same workload on this host.
+during the timed steps), the accuracy of the projected velocity (FP64 on the
+host from the step's outputs), and the CPU oracle timed on a bounded sample
+of the same workload on this host (one thread and all cores), with its FP64
+error check. Unless --no-extra is given, the same line also carries the other
+workloads, each measured in this run: the 512^3 mixed-precision Poisson solve
+("mixed_poisson512"), the 64-grid batch ("batch64_2d") and the served
+projection mix ("serve", 60 s of arrivals).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel smooth_post_l0] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel smooth_post_l0] [--no-cpu] [--no-extra]
+    python bench.py --workload {poisson512_mixed,batch64_2d,serve,stream_copy}   (one workload alone)
 """
 import argparse
 import ctypes
@@ -43,6 +49,8 @@ def main():
     ap.add_argument("--duration", type=float, default=None, help="serve: seconds of arrivals (default 60)")
     ap.add_argument("--cpu-check", action="store_true",
                     help="poisson512_mixed: also solve in pure FP64 with the CPU oracle (minutes at 512^3)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default run: leave out the mixed, batch and serving workloads")
     args = ap.parse_args()
     args.steps_given = args.steps is not None
     if args.steps is None:
@@ -53,13 +61,13 @@ def main():
         sys.exit(2)
 
     if args.workload == "poisson512_mixed":
-        return bench_mixed(args)
+        return print(json.dumps(bench_mixed(args)), flush=True)
     if args.workload == "batch64_2d":
-        return bench_batch(args)
+        return print(json.dumps(bench_batch(args)), flush=True)
     if args.workload == "stream_copy":
         return bench_stream_copy(args)
     if args.workload == "serve":
-        return bench_serve(args)
+        return print(json.dumps(bench_serve(args)), flush=True)
 
     import panopaea_amd as pa
     from panopaea_amd import _lib
@@ -127,6 +135,20 @@ def main():
 
     # the whole step against the same unit rule: model bytes of the reference
     # method (phase-separated passes) over the measured step time
+    # accuracy of the last timed step, from its outputs (FP64 on the host)
+    from panopaea_amd import diagnostics as D
+    last = infos[-1]
+    rep = D.projection_report(g, vel[cur].download(), press.download(), cfg.dt, last["b_linf"])
+    accuracy = {
+        "recursive_rel_residual_linf": last["r_linf"] / last["b_linf"],
+        "true_rel_residual_linf": rep["div_rel_linf"],
+        "true_residual_l2": rep["div_l2"],
+        "rounding_bound_rel": rep["rounding_bound_rel"],
+        "note": "true = ||div u_new||_inf / ||b||_inf in FP64 from the step's f32 outputs: the solve's b - A p "
+                "up to the f32 rounding of the projected faces (rounding_bound_rel); last timed step",
+    }
+    print(f"bench: accuracy {accuracy}", file=sys.stderr, flush=True)
+
     k_mean = float(np.mean(iters))
     b_step = step_model_bytes(n, k_mean)
     step_gbs = b_step * args.steps / elapsed / 1e9
@@ -138,7 +160,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(wl, dims, cfg, init_vel)
+        cpu, acc_fp64 = cpu_baseline(wl, dims, cfg, init_vel)
+        accuracy["vs_fp64_oracle"] = acc_fp64
+    del rho, vel, press
 
     out = {
         "metric": "cell-updates/s 256^3 advect+project @1e-6 resid, plain FP32; p50/p99 step latency",
@@ -161,9 +185,30 @@ def main():
                    "cg_iterations_per_step": iters},
         "roofline": roof,
         "step_roofline": step_roof,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
     }
+    if not args.no_extra:
+        args.steps_given, args.warmup = False, 2
+        out["mixed_poisson512"] = extra(bench_mixed, args, reps=3)
+        out["batch64_2d"] = extra(bench_batch, args)
+        out["serve"] = extra(bench_serve, args)
     print(json.dumps(out), flush=True)
+
+
+_COMMON = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "warmup")
+
+
+def extra(fn, args, **kw):
+    """Another workload measured in the same run, as a key of the default
+    line (its own line's fields, without the ones the line already has); an
+    error is reported in place of the result."""
+    try:
+        r = fn(args, **kw)
+        return {k: v for k, v in r.items() if k not in _COMMON}
+    except Exception as e:  # reported, not hidden: the key carries the error
+        print(f"bench: {fn.__name__} failed: {e!r}", file=sys.stderr, flush=True)
+        return {"error": repr(e)}
 
 
 def step_model_bytes(n, k):
@@ -176,10 +221,16 @@ def step_model_bytes(n, k):
     return 4.0 * n * (8 + 4 + 7 + 18.54 + 29.54 * k)
 
 
-def bench_mixed(args):
+def bench_mixed(args, reps=None):
     """512^3 Poisson: FP64 iterative refinement around the FP32 MG-PCG against
     the pure-FP64 MG-PCG (same FP64-verified residual) and the plain FP32
-    MG-PCG (its attainable accuracy, measured in FP64)."""
+    MG-PCG (its attainable accuracy, measured in FP64). The plain FP32 form
+    stalls near 1e-6 and cannot reach 1e-10, so the mixed form's speed is
+    compared with the FP64 MG-PCG at the same verified residual. The CPU leg
+    (unless --no-cpu) solves a bounded sample (the same generator at 128^3)
+    in pure FP64 with the oracle and reports the mixed solve's error against
+    it."""
+    reps = reps or args.steps
     import panopaea_amd as pa
     from panopaea_amd import workloads as W
     wl = W.POISSON512_MIXED
@@ -212,14 +263,14 @@ def bench_mixed(args):
                                outer_tol=wl["outer_tol"])
     info_mx, t_mx = timed(lambda: pa.poisson_solve_mixed(x_mx, b64, dt=wl["dt"], preconditioner=mg,
                                                          inner_tol=wl["inner_tol"], outer_tol=wl["outer_tol"]),
-                          args.steps)
+                          reps)
     print(f"bench: mixed {info_mx.as_dict()} times {[round(t * 1e3, 1) for t in t_mx]} ms", file=sys.stderr,
           flush=True)
     # pure FP64 MG-PCG to the same relative residual
     x64, r, a, s = (g64.new_simplex_2() for _ in range(4))
     op64 = pa.PoissonOperator(g64, wl["dt"])
     info_64, t_64 = timed(lambda: pa.precond_conjugate_gradient(mg, x64, b64, 1000, wl["outer_tol"], r, a, s, op64,
-                                                                tol_kind="rel"), max(1, args.steps // 2))
+                                                                tol_kind="rel"), max(1, reps // 2))
     print(f"bench: fp64 pcg iterations={info_64.iterations} converged={info_64.converged} "
           f"times {[round(t * 1e3, 1) for t in t_64]} ms", file=sys.stderr, flush=True)
     # plain FP32 MG-PCG to 1e-6 (FP32 right-hand side)
@@ -228,7 +279,7 @@ def bench_mixed(args):
     x32, r32, a32, s32 = (g32.new_simplex_2() for _ in range(4))
     op32 = pa.PoissonOperator(g32, wl["dt"])
     info_32, t_32 = timed(lambda: pa.precond_conjugate_gradient(mg, x32, b32, 1000, 1e-6, r32, a32, s32, op32,
-                                                                tol_kind="rel"), max(1, args.steps // 2))
+                                                                tol_kind="rel"), max(1, reps // 2))
     # accuracy, measured in FP64 on the device: ||b - A x||_inf / ||b||_inf and the error vs the FP64 solve
     def rel_res(xd):
         q = g64.new_simplex_2()
@@ -252,6 +303,11 @@ def bench_mixed(args):
         ref = xo - xo.mean()
         acc["mixed_err_vs_fp64_cpu"] = err(xm)
         acc["fp64_gpu_err_vs_fp64_cpu"] = err(xf)
+    for f in (x_mx, x64, r, a, s, b32, x32, r32, a32, s32, x32_64, b64):
+        f.free()
+    del xm, xf, xs, bh, ref
+    if not args.no_cpu:
+        acc["sample_128_vs_fp64_cpu"] = mixed_cpu_check(pa, wl, mg)
     print(f"bench: accuracy {acc}", file=sys.stderr, flush=True)
     tm = float(np.median(t_mx))
     out = {
@@ -266,7 +322,42 @@ def bench_mixed(args):
         "fp32_pcg_1e-6": {"ms": float(np.median(t_32)) * 1e3, "iterations": info_32.iterations},
         "accuracy": acc,
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def mixed_cpu_check(pa, wl, mg, n=128):
+    """CPU leg of the mixed solve: the same generator and settings at n^3, the
+    GPU mixed solve against the oracle's pure-FP64 MG-PCG to 1e-12 (all host
+    cores). Errors are of the mean-free solutions, relative L-inf."""
+    from oracle import oracle as o
+    dims = (n, n, n)
+    g = pa.Grid3d(dims, np.float64, bc=wl["bc"])
+    faces = g.new_simplex_1()
+    pa.gen_uniform_faces_device(faces, wl["seed"])
+    b = g.new_simplex_2()
+    pa.neg_divergence(b, faces)
+    x = g.new_simplex_2()
+    info = pa.poisson_solve_mixed(x, b, dt=wl["dt"], preconditioner=mg, inner_tol=wl["inner_tol"],
+                                  outer_tol=wl["outer_tol"])
+    xg, bh = x.download(), b.download()
+    P = o.params(dims, bc=wl["bc"], dt=wl["dt"], precond="mg", levels=wl["mg_levels"], tol=1e-12, max_iterations=2000)
+    t = time.perf_counter()
+    xo, oinfo = o_solve_all_cores(o, P, bh)
+    el = time.perf_counter() - t
+    ref = xo - xo.mean()
+    err = float(np.abs((xg - xg.mean()) - ref).max() / np.abs(ref).max())
+    print(f"bench: mixed {dims} vs CPU FP64 oracle ({el:.1f}s, {oinfo.iterations} iterations): error {err:.2e}",
+          file=sys.stderr, flush=True)
+    return {"dims": list(dims), "err_vs_fp64_cpu": err, "gpu_rel_residual": info.r_linf / info.b_linf,
+            "cpu_fp64_iterations": oinfo.iterations, "cpu_fp64_s": el, "cpu_threads": o.omp_lib().or_threads()}
+
+
+def o_solve_all_cores(o, P, b):
+    b = np.array(b, copy=True)
+    x = np.zeros_like(b)
+    info = o.Info()
+    o._fn("or_solve", b.dtype, o.omp_lib())(ctypes.byref(P), o._p(x), o._p(b), ctypes.byref(info))
+    return x, info
 
 
 def batch_step_bytes(dims, iters, levels):
@@ -278,7 +369,7 @@ def batch_step_bytes(dims, iters, levels):
     return 4.0 * n * (6 + 3 + 5 + vc + (2 + 6 + 3 + vc) * iters)
 
 
-def bench_batch(args):
+def bench_batch(args, **_):
     """64 uneven square 2D grids advanced together: one set of launches per
     phase covers every grid, phases replayed as hipGraphs, tile shapes from
     the autotuner. Timed: K steps of all grids from the seeded state."""
@@ -328,7 +419,7 @@ def bench_batch(args):
         P = o.params(g["dims"], bc=g["bc"], dt=g["dt"], precond="mg", levels=wl["mg_levels"], tol=wl["tol"],
                      max_iterations=500)
         reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 10.0:
+        while time.perf_counter() - t < 5.0:
             o.step(P, rho, faces)
             reps += 1
         el = time.perf_counter() - t
@@ -355,10 +446,10 @@ def bench_batch(args):
                      "traffic": None},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_serve(args):
+def bench_serve(args, **_):
     """Open-loop serving: projection requests arrive by a seeded Poisson
     process; the loop submits each at its arrival time, ticks the server,
     and on completion records latency (completion - arrival) and releases
@@ -426,7 +517,7 @@ def bench_serve(args):
                    "precond": "multigrid 4 levels, 2+2 red-black GS", "tol": "1e-6 rel L-inf"},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_stream_copy(args):
@@ -468,21 +559,82 @@ def profiled_traffic(kernel, dims):
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, dims, cfg, init_vel):
-    """The CPU oracle (a C restatement of the reference method, single
-    thread) on one step of the same workload from the same initial state."""
+    """The CPU leg: the oracle (a C restatement of the reference method) on
+    one step of the same workload from the same initial state, on one thread
+    and on all host cores (the OpenMP build of the same sources, standing in
+    for the reference's rayon loops); and, as the checker, the GPU step's
+    error against the FP64 oracle on a bounded sample of the workload."""
     from oracle import oracle as o
     P = o.params(dims, bc=wl["bc"], dt=cfg.dt, precond="mg", levels=wl["mg_levels"], tol=wl["tol"],
                  max_iterations=500, buoyancy=wl["buoyancy"], src_center=cfg.src_center, src_radius=cfg.src_radius)
-    rho = np.zeros(int(np.prod(dims)), np.float32)
-    faces = init_vel.copy()
-    t = time.perf_counter()
-    _, info = o.step(P, rho, faces)
-    el = time.perf_counter() - t
-    print(f"bench: cpu oracle step {el:.2f}s iterations={info.iterations}", file=sys.stderr, flush=True)
-    return {"value": int(np.prod(dims)) / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"step 1 of the {'x'.join(map(str, dims))} smoke workload from the same seeded initial "
-                      f"state, CPU oracle (C, -O2, f32 fields, MG-PCG), 1 thread, {el:.1f}s"}
+    n = int(np.prod(dims))
+    legs = {}
+    for name, L in (("one_thread", None), ("all_cores", o.omp_lib())):
+        rho = np.zeros(n, np.float32)
+        faces = init_vel.copy()
+        t = time.perf_counter()
+        _, info = o.step(P, rho, faces, L)
+        el = time.perf_counter() - t
+        threads = (L or o.lib).or_threads()
+        print(f"bench: cpu oracle step ({name}, {threads} threads) {el:.2f}s iterations={info.iterations}",
+              file=sys.stderr, flush=True)
+        legs[name] = {"value": n / el, "unit": "cell-updates/s", "cores": threads, "seconds": el,
+                      "cg_iterations": info.iterations}
+    best = legs["all_cores"]
+    cpu = {"value": best["value"], "unit": "cell-updates/s", "cores": best["cores"], "kind": "port",
+           "sample": f"step 1 of the {'x'.join(map(str, dims))} smoke workload from the same seeded initial state, "
+                     f"CPU oracle (C, gcc -O3, f32 fields, MG-PCG) on all {best['cores']} threads, "
+                     f"{best['seconds']:.1f}s; one thread: {legs['one_thread']['value']:.3g} cell-updates/s "
+                     f"({legs['one_thread']['seconds']:.1f}s)",
+           "legs": legs, "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    return cpu, fp64_check(wl, o)
+
+
+def fp64_check(wl, o, n=64, steps=5):
+    """The f32 GPU step against the FP64 oracle on a bounded sample: the same
+    workload at n^3 (same generator, source, buoyancy, MG-PCG to 1e-6),
+    `steps` steps from the same seeded state; relative L-inf errors of the
+    velocity and the density per step, and the projected velocity's
+    divergence against ||b||_inf for both."""
+    import panopaea_amd as pa
+    from panopaea_amd import diagnostics as D
+    from panopaea_amd import workloads as W
+    w = dict(wl, dims=(n, n, n))
+    g = W.make_grid(w)
+    cfg = W.step_config(w)
+    rho, faces = W.initial_state(w, g)
+    P = o.params(w["dims"], bc=w["bc"], dt=cfg.dt, precond="mg", levels=w["mg_levels"], tol=w["tol"],
+                 max_iterations=500, buoyancy=w["buoyancy"], src_center=cfg.src_center, src_radius=cfg.src_radius)
+    r0, v0 = g.new_simplex_2().upload(rho), g.new_simplex_1().upload(faces)
+    r1, v1, pr = g.new_simplex_2(), g.new_simplex_1(), g.new_simplex_2()
+    r64, f64 = rho.astype(np.float64), faces.astype(np.float64)
+    ev, er, first = [], [], None
+    for k in range(steps):
+        info = pa.fluid_step(g, cfg, r0, v0, r1, v1, pr)
+        p64, i64 = o.step(P, r64, f64, o.omp_lib())
+        gv, gr = v1.download(), r1.download()
+        ev.append(float(np.abs(gv - f64).max() / np.abs(f64).max()))
+        er.append(float(np.abs(gr - r64).max() / np.abs(r64).max()))
+        if k == 0:
+            d32 = D.projection_report(g, gv, pr.download(), cfg.dt, info.b_linf)
+            d64 = D.projection_report(g, f64, p64, cfg.dt, i64.b_linf)
+            first = {"gpu_f32_div_rel_linf": d32["div_rel_linf"], "fp64_div_rel_linf": d64["div_rel_linf"]}
+        r0, r1, v0, v1 = r1, r0, v1, v0
+    out = {"sample": f"{n}^3 smoke workload, {steps} steps from the seeded state", "velocity_rel_linf": ev,
+           "density_rel_linf": er, "step0": first}
+    print(f"bench: fp64 check {out}", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -201,7 +201,9 @@ pano_status pano_poisson_solve_mixed(pano_ctx, const pano_grid*, const pano_mixe
 /* ---- one fluid step (dec_fluid.rs:46-141) ---------------------------------- */
 typedef struct {
     double dt;
-    pano_pcg_params pcg;     /* pcg.dt is ignored (the step's dt is used) */
+    pano_pcg_params pcg;     /* pcg.dt is ignored (the step's dt is used); pcg.face_mask must be
+                                NULL (PANO_ERR_INVALID): the step's divergence and projection are
+                                unmasked, so the obstacle scene is composed from the operators */
     double buoyancy;         /* smoke buoyancy on the z (3D) faces; 0 = off */
     double src_center[3];    /* (z, y, x) centre of the density source sphere */
     double src_radius;       /* <= 0: no source */
@@ -209,8 +211,9 @@ typedef struct {
 
 /* source -> advect density and MAC velocity -> b = -div u -> PCG ->
  * projection. density_in/velocity_in are read, density_out/velocity_out
- * written (they must be distinct buffers); pressure (optional, may be NULL,
- * distinct from both outputs) receives the solved pressure. Synchronous. */
+ * written; pressure (optional, may be NULL) receives the solved pressure.
+ * No output may overlap an input or another output (byte ranges of the
+ * grid's cell and face sizes; PANO_ERR_INVALID otherwise). Synchronous. */
 pano_status pano_fluid_step(pano_ctx, const pano_grid*, const pano_step_params*,
                             const void* density_in, const void* velocity_in, void* density_out,
                             void* velocity_out, void* pressure, pano_pcg_info* info);

@@ -14,6 +14,29 @@
 
 static inline int64_t mini64(int64_t a, int64_t b) { return a < b ? a : b; }
 
+/* Loop annotations of the all-cores build (liboracle_omp.so, -fopenmp): the
+ * grid loops split their (z, y) rows over threads and the vector loops their
+ * elements. Every element is still computed by the same expression, so only
+ * the reductions (dot products, norms) may round differently; the serial
+ * build (liboracle.so) ignores them. This build times the CPU path on all
+ * host cores, standing in for the reference's rayon par_azip! loops
+ * (panopaea/src/dec/grid.rs:277-333). */
+#ifdef _OPENMP
+#define OR_PAR_ROWS _Pragma("omp parallel for collapse(2) schedule(static)")
+#define OR_PAR_VEC _Pragma("omp parallel for schedule(static)")
+#define OR_PAR_SUM_S _Pragma("omp parallel for reduction(+:s) schedule(static)")
+#define OR_PAR_SUM_RS2 _Pragma("omp parallel for reduction(+:rs2) schedule(static)")
+#define OR_PAR_SUM_PQ _Pragma("omp parallel for reduction(+:pq) schedule(static)")
+#define OR_PAR_SUM_M _Pragma("omp parallel for reduction(+:m) schedule(static)")
+#else
+#define OR_PAR_ROWS
+#define OR_PAR_VEC
+#define OR_PAR_SUM_S
+#define OR_PAR_SUM_RS2
+#define OR_PAR_SUM_PQ
+#define OR_PAR_SUM_M
+#endif
+
 /* imgproc::transfer, panopaea_utils/src/imgproc.rs:2-5: x.min(upper).max(lower)
  * (f64::min/max return the other operand for a NaN, as fmin/fmax do), then
  * ((v - lower) / (upper - lower) * 255.0) as u8 (truncation). */
@@ -48,6 +71,15 @@ typedef struct {
     double r_linf;          /* recursive residual at exit */
     double threshold;       /* absolute threshold used */
 } or_info;
+
+/* Threads the loops above run on: omp_get_max_threads() in the all-cores
+ * build, 1 in the serial one. */
+#ifdef _OPENMP
+#include <omp.h>
+int or_threads(void) { return omp_get_max_threads(); }
+#else
+int or_threads(void) { return 1; }
+#endif
 
 #define T float
 #define FN(n) n##_f32

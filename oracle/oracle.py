@@ -12,19 +12,30 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+_LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def _load():
-    if not os.path.exists(_LIB):
+def _load(path=_LIB):
+    if not os.path.exists(path):
         build()
-    return ctypes.CDLL(_LIB)
+    return ctypes.CDLL(path)
 
 
 lib = _load()
+_omp = None
+
+
+def omp_lib():
+    """The all-cores build (OpenMP over the grid rows and vector loops),
+    for the CPU baseline's all-cores leg; its threads: omp_lib().or_threads()."""
+    global _omp
+    if _omp is None:
+        _omp = _load(_LIB_OMP)
+    return _omp
 
 c_i64 = ctypes.c_int64
 c_ptr = ctypes.c_void_p
@@ -91,8 +102,8 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(c_ptr)
 
 
-def _fn(name, dtype):
-    return getattr(lib, name + _sfx(dtype))
+def _fn(name, dtype, L=None):
+    return getattr(L or lib, name + _sfx(dtype))
 
 
 def _real(dtype, v):
@@ -298,13 +309,14 @@ def source_sphere(P, rho):
     return rho
 
 
-def step(P, rho, faces):
-    """One fluid step in place on (rho, faces). Returns (p, info)."""
+def step(P, rho, faces, L=None):
+    """One fluid step in place on (rho, faces). Returns (p, info). L: the
+    library to run it with (default the serial build; omp_lib() for all cores)."""
     vz, vy, vx = split_faces(P, faces)
     p = np.zeros_like(rho)
     info = Info()
-    _fn("or_step", rho.dtype)(ctypes.byref(P), _p(rho), _p(vz), _p(vy), _p(vx), _p(p),
-                              ctypes.byref(info))
+    _fn("or_step", rho.dtype, L)(ctypes.byref(P), _p(rho), _p(vz), _p(vy), _p(vx), _p(p),
+                                 ctypes.byref(info))
     return p, info
 
 

@@ -408,8 +408,12 @@ class Batch:
         return d, f
 
     def step(self):
+        """One step of every grid; returns each grid's CG iteration count.
+        self.converged is False when some grid's solve stopped at
+        max_iterations without meeting its tolerance."""
         it = (ctypes.c_int32 * len(self.grids))()
-        _lib.check(_lib.lib().pano_batch_step(self.handle, it), allow=(PANO_OK, PANO_NOT_CONVERGED))
+        st = _lib.check(_lib.lib().pano_batch_step(self.handle, it), allow=(PANO_OK, PANO_NOT_CONVERGED))
+        self.converged = st == PANO_OK
         return list(it)
 
     def tiles(self):

@@ -133,9 +133,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// max that keeps a NaN from either side (fmax would drop it): the norms built
+// from it are NaN as soon as one term is, so the solver's NaN tests see it.
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     return v;
 }
 // Block of blockDim.x*blockDim.y*blockDim.z threads (a multiple of 64, <= 1024).
@@ -163,7 +166,7 @@ __device__ __forceinline__ double block_max(double v) {
     __syncthreads();
     double s = 0.0;
     if (tid == 0)
-        for (int i = 0; i < (nthr >> 6); ++i) s = fmax(s, redm_s[i]);
+        for (int i = 0; i < (nthr >> 6); ++i) s = nan_max(s, redm_s[i]);
     return s;
 }
 

@@ -731,10 +731,30 @@ pano_status pano_fluid_step(pano_ctx ctx, const pano_grid* g, const pano_step_pa
     PANO_CHECK(check_grid(g));
     if (!P || !density_in || !velocity_in || !density_out || !velocity_out)
         return fail(PANO_ERR_INVALID, "null argument");
-    if (density_in == density_out || velocity_in == velocity_out || density_out == velocity_out)
-        return fail(PANO_ERR_INVALID, "step outputs must not alias its inputs");
-    if (pressure && (pressure == density_out || pressure == velocity_out))
-        return fail(PANO_ERR_INVALID, "pressure must not alias density_out or velocity_out");
+    if (P->pcg.face_mask)
+        return fail(PANO_ERR_INVALID,
+                    "the step takes no face mask (its divergence and projection are unmasked); compose the masked "
+                    "scene from pano_neg_divergence_masked, pano_precond_conjugate_gradient and pano_project");
+    {
+        // every output (pressure included) must not overlap any other buffer of the step: the
+        // advection kernels still read both inputs while they write the outputs, and the
+        // buoyancy reads the input density beside the face advection
+        const size_t el = g->dtype == PANO_F64 ? 8 : 4;
+        const size_t cb = (size_t)pano_num_elem_2(g) * el, fb = (size_t)pano_num_elem_1(g) * el;
+        struct Range { const char* p; size_t n; };
+        const Range in_r[2] = {{(const char*)density_in, cb}, {(const char*)velocity_in, fb}};
+        const Range out_r[3] = {{(const char*)density_out, cb}, {(const char*)velocity_out, fb},
+                                {(const char*)pressure, pressure ? cb : 0}};
+        auto overlap = [](const Range& a, const Range& b) {
+            return a.n && b.n && a.p < b.p + b.n && b.p < a.p + a.n;
+        };
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 2; ++j)
+                if (overlap(out_r[i], in_r[j])) return fail(PANO_ERR_INVALID, "step outputs must not overlap its inputs");
+            for (int j = i + 1; j < 3; ++j)
+                if (overlap(out_r[i], out_r[j])) return fail(PANO_ERR_INVALID, "step outputs must not overlap each other");
+        }
+    }
     PANO_HIP(hipSetDevice(c->device));
     Dims D = dims_of(g);
     if (g->dtype == PANO_F64)

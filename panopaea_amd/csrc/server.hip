@@ -155,11 +155,29 @@ static pano_status request_layout(Server* S, Request* r, int64_t base) {
     return PANO_OK;
 }
 
+static pano_status admit_at(Server* S, Request* r, int64_t off, std::vector<Request*>& fresh);
+
+// Admits the queue's front request if its fields fit now (PANO_NOT_CONVERGED
+// when they do not). On an error after the arena range is taken (layout,
+// pinned staging, upload or generator launch) the range and the solver slot
+// are given back and the request stays queued as it was.
 static pano_status admit(Server* S, Request* r, std::vector<Request*>& fresh) {
-    Ctx* c = S->c;
-    int64_t need = r->bytes + (r->faces.empty() ? al256(r->tab_bytes) : 0);
-    int64_t off = S->alloc.alloc(need);
+    const int64_t need = r->bytes + (r->faces.empty() ? al256(r->tab_bytes) : 0);
+    const int64_t off = S->alloc.alloc(need);
     if (off < 0) return PANO_NOT_CONVERGED;  // does not fit now
+    const pano_status st = admit_at(S, r, off, fresh);
+    if (st != PANO_OK) {
+        S->alloc.release(off, need);
+        if (r->slot >= 0) S->free_slots.push_back(r->slot);
+        r->slot = -1;
+        r->off = -1;
+        r->tab_off = -1;
+    }
+    return st;
+}
+
+static pano_status admit_at(Server* S, Request* r, int64_t off, std::vector<Request*>& fresh) {
+    Ctx* c = S->c;
     r->off = off;
     PANO_CHECK(request_layout(S, r, off));
     r->tab_off = r->faces.empty() ? off + al256(r->bytes) : -1;

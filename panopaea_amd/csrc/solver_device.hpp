@@ -63,7 +63,9 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
             if (!(tot == tot) || tot > 1.7e308) {
                 s.nonfinite = 1;
                 s.done = 1;
-            } else if (s.b_linf < s.threshold) {  // pcg.rs:35-38
+            } else if (s.b_linf < s.threshold || tot == 0.0) {
+                // pcg.rs:35-38; b = 0 is solved by x = 0 also when the relative
+                // threshold tol * ||b|| is 0 (the recursion would divide 0 by 0)
                 s.converged = 1;
                 s.done = 1;
             }

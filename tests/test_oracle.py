@@ -796,3 +796,31 @@ def test_poisson_matrix_equals_matrix_free(dims, bc):
     assert np.array_equal(A.mul_vec(p), o.poisson(P, p))
     if not any(bc):
         assert np.abs(Ad.sum(axis=1)).max() == 0.0
+
+
+# ---- the all-cores build (CPU baseline) computes the same step ---------------------------------
+
+def test_omp_build_step_matches_serial():
+    # same expressions per element; only the reductions (dots, norms, means) are split over
+    # threads, so the step agrees with the serial build to rounding
+    P = o.params((16, 16, 16), bc=(0, 0, 0, 0, 0, 1), dt=0.5 / 16, precond="mg", levels=3, tol=1e-8,
+                 max_iterations=500, buoyancy=1.0, src_center=(1.6, 8.0, 8.0), src_radius=2.5)
+    rng = np.random.default_rng(9)
+    faces = o.grad_sub(P, rng.uniform(-1, 1, o.nfaces(P)), np.zeros(16 ** 3))
+    rho = np.zeros(16 ** 3)
+    r1, f1, r2, f2 = rho.copy(), faces.copy(), rho.copy(), faces.copy()
+    p1, i1 = o.step(P, r1, f1)
+    p2, i2 = o.step(P, r2, f2, o.omp_lib())
+    assert i1.converged and i2.converged and abs(i1.iterations - i2.iterations) <= 1
+    assert np.array_equal(r1, r2)  # advection has no reductions: bit for bit
+    assert np.abs(f1 - f2).max() < 1e-9 * np.abs(f1).max()
+    assert o.omp_lib().or_threads() >= 1 and o.lib.or_threads() == 1
+
+
+def test_zero_rhs_is_solved_by_zero_under_relative_tolerance():
+    # the relative threshold tol * ||b|| is 0 for b = 0: the early out still fires
+    # (the recursion would form 0/0), for every preconditioner
+    for pre in ("identity", "jacobi", "mg"):
+        P = o.params((16, 16), dt=0.1, precond=pre, levels=3, tol=1e-6)
+        x, info, _ = o.solve(P, np.zeros(256))
+        assert info.converged and info.iterations == 0 and np.all(x == 0)
