@@ -173,6 +173,18 @@ pano_status pano_precond_conjugate_gradient(pano_ctx, const pano_grid*, const pa
                                             void* x, const void* b, void* residual,
                                             void* auxiliary, void* search, pano_pcg_info* info);
 
+/* No reference counterpart: pano_precond_conjugate_gradient for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, arithmetic and results
+ * (bit for bit); the solver's own state and partial sums live in a fixed
+ * 9 MiB device array per GPU (up to 2^26 cells when nx is a multiple of 64;
+ * larger grids: PANO_ERR_BAD_SHAPE) instead of allocated workspace, and no
+ * host containers grow. Identity and Jacobi preconditioners only (multigrid:
+ * PANO_ERR_INVALID). Calls that overlap on one GPU share that array and race. */
+pano_status pano_precond_conjugate_gradient_noalloc(pano_ctx, const pano_grid*, const pano_pcg_params*,
+                                                    void* x, const void* b, void* residual,
+                                                    void* auxiliary, void* search, pano_pcg_info* info);
+
 /* ---- mixed precision: FP64 iterative refinement around an FP32 solve ------- */
 /* No reference counterpart (the reference solves in f64 only, pcg.rs with
  * f64 fields at dec_fluid.rs:18-24). Outer loop in FP64: r = b - A x

@@ -125,6 +125,8 @@ _SIGS = {
     "pano_precond_apply": ([c_void_p, _P(GridDesc), _P(PcgParams), c_void_p, c_void_p], c_i32),
     "pano_precond_conjugate_gradient": ([c_void_p, _P(GridDesc), _P(PcgParams), c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, _P(PcgInfo)], c_i32),
+    "pano_precond_conjugate_gradient_noalloc": ([c_void_p, _P(GridDesc), _P(PcgParams), c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, _P(PcgInfo)], c_i32),
     "pano_poisson_solve_mixed": ([c_void_p, _P(GridDesc), _P(MixedParams), c_void_p, c_void_p, _P(MixedInfo)],
                                  c_i32),
     "pano_fluid_step": ([c_void_p, _P(GridDesc), _P(StepParams), c_void_p, c_void_p, c_void_p, c_void_p,

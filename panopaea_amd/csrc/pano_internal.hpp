@@ -83,10 +83,19 @@ struct Ctx {
     SolverState* host_state = nullptr;   // pinned mirror of the device state
     hipEvent_t ev[2] = {nullptr, nullptr};
     int num_cu = 256;
+    // While `arena` is set (by the no-allocation entry point, for one call),
+    // workspace() hands out consecutive 256-byte aligned ranges of it instead
+    // of pooled allocations, and fails once `arena_cap` would be exceeded.
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     // Returns a device buffer of at least `bytes` for `name`, (re)allocated
     // on growth; contents are unspecified.
     pano_status workspace(const char* name, size_t bytes, void** out);
 };
+
+// Per-device set-up of the no-allocation scratch (noalloc.hip): called by
+// pano_ctx_create with `device` current.
+pano_status scratch_setup(int device);
 
 Ctx* ctx_of(pano_ctx c);
 bool timer_on(Ctx* c, const char* name);
@@ -248,10 +257,14 @@ inline FaceLayout face_layout(const Dims& D) {
     return f;
 }
 
-// PCG driver (pcg.hip).
+// PCG driver (pcg.hip). no_host_alloc: keep no iteration hint and no timer
+// record for this solve (their containers would allocate host memory).
 template <typename T>
 pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, const T* b,
                       T* residual, T* auxiliary, T* search, pano_pcg_info* info,
-                      bool b_in_residual_rb);
+                      bool no_host_alloc);
+// Device workspace bytes that pcg_solve takes for an identity or Jacobi
+// solve on D: the solver state and its two partial-sum arrays.
+size_t pcg_workspace_bytes(const Dims& D);
 
 }  // namespace pano

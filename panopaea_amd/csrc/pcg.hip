@@ -147,6 +147,22 @@ struct Solver {
     int it;  // iterations enqueued in this solve
 };
 
+// Entries of each partial-sum array: one per flat block, two per tile (the
+// red-black V-cycle's two sweeps), and one per tile of the fused step's
+// divergence (its max|b| partials).
+static int solver_partials(const Dims& D, bool rb) {
+    const int nflat = (int)blocks_for((int64_t)D.d * D.h * D.w, kFlatPer);
+    const int ntile = (rb ? grid3_rb(D) : grid3(D.w, D.h, D.d)).nb;
+    int np = nflat > 2 * ntile ? nflat : 2 * ntile;
+    const Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
+    return Gd.nb > np ? Gd.nb : np;
+}
+
+size_t pcg_workspace_bytes(const Dims& D) {
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    return al(sizeof(SolverState)) + 2 * al((size_t)solver_partials(D, false) * sizeof(double));
+}
+
 template <typename T>
 static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P, Solver<T>* S) {
     S->c = c;
@@ -170,10 +186,7 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     S->nflat = (int)blocks_for(S->n, kFlatPer);
     S->G = S->rb ? grid3_rb(D) : grid3(D.w, D.h, D.d);
     S->ntile = S->G.nb;
-    int np = S->nflat > 2 * S->ntile ? S->nflat : 2 * S->ntile;
-    // the fused step's divergence may hand over one max|b| partial per tile
-    const Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
-    if (Gd.nb > np) np = Gd.nb;
+    const int np = solver_partials(D, S->rb);
     void *st, *p0, *p1;
     PANO_CHECK(c->workspace("pcg_state", sizeof(SolverState), &st));
     PANO_CHECK(c->workspace("pcg_part0", (size_t)np * sizeof(double), &p0));
@@ -330,7 +343,7 @@ static pano_status solver_iteration(Solver<T>* S) {
 // check_every > 0 it enqueues chunks of that size, two in flight. Leaves the
 // final state in c->host_state[0]. The result does not depend on the hint.
 template <typename T>
-static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string& key) {
+static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string* key) {
     Ctx* c = S->c;
     SolverState* hs = c->host_state;
     int launched = 0;
@@ -355,7 +368,7 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
             ++chunk;
         }
     } else {
-        auto it = c->iter_hint.find(key);
+        auto it = key ? c->iter_hint.find(*key) : c->iter_hint.end();
         int first = it != c->iter_hint.end() ? it->second : 4;
         PANO_CHECK(enqueue(first < 1 ? 1 : first));
         for (;;) {
@@ -371,8 +384,10 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
     PANO_HIP(hipGetLastError());
     PANO_HIP(hipStreamSynchronize(c->stream));
     PANO_HIP(hipMemcpy(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost));
-    c->iter_hint[key] = hs[0].iter;
-    c->timer.solve_iters.push_back(hs[0].iter);
+    if (key) {
+        c->iter_hint[*key] = hs[0].iter;
+        c->timer.solve_iters.push_back(hs[0].iter);
+    }
     c->timer.solve = -1;
     return PANO_OK;
 }
@@ -403,7 +418,7 @@ static pano_status finish_info(Ctx* c, pano_pcg_info* info) {
 // internally for multigrid, with x accumulated in a workspace copy).
 template <typename T>
 pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, const T* b, T* residual,
-                      T* auxiliary, T* search, pano_pcg_info* info, bool) {
+                      T* auxiliary, T* search, pano_pcg_info* info, bool no_host_alloc) {
     Solver<T> S;
     PANO_CHECK(solver_setup<T>(c, D, P, &S));
     S.r = residual;
@@ -421,7 +436,12 @@ pano_status pcg_solve(Ctx* c, const Dims& D, const pano_pcg_params* P, T* x, con
         PANO_HIP(hipMemcpyAsync(S.r, b, (size_t)S.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     }
     PANO_CHECK(solver_init<T>(&S, P->max_iterations));
-    PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every, hint_key(D, P, sizeof(T))));
+    if (no_host_alloc) {
+        PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every, nullptr));
+    } else {
+        const std::string key = hint_key(D, P, sizeof(T));
+        PANO_CHECK(solver_loop<T>(&S, P->max_iterations, P->check_every, &key));
+    }
     if (S.rb) {
         L3((k_from_rb<T>), Gn, D, Gn, x, S.x);
         PANO_HIP(hipStreamSynchronize(c->stream));
@@ -508,7 +528,8 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     }
     PANO_HIP(hipGetLastError());
     PANO_CHECK(solver_init<T>(&S, pp.max_iterations, rhs_fused ? (int)Gd.nb : 0));
-    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, hint_key(D, &pp, sizeof(T))));
+    const std::string key = hint_key(D, &pp, sizeof(T));
+    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, &key));
     // u += dt d0_dual(p), closed walls zeroed; the x component's pass also
     // writes the pressure out in natural layout
     if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x, pressure));
@@ -654,7 +675,7 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
         if (hs[1].done) break;
         // inner FP32 solve A e = r to the inner relative tolerance
         PANO_CHECK(solver_init<float>(&S, ip.max_iterations));
-        PANO_CHECK(solver_loop<float>(&S, ip.max_iterations, ip.check_every, key));
+        PANO_CHECK(solver_loop<float>(&S, ip.max_iterations, ip.check_every, &key));
         if (hs[0].nonfinite) return fail(PANO_ERR_NONFINITE, "non-finite value in the inner solve");
         inner_total += hs[0].iter;
         ++outer;

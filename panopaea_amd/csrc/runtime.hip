@@ -66,6 +66,15 @@ pano_status check_grid(const pano_grid* g) {
 }
 
 pano_status Ctx::workspace(const char* name, size_t bytes, void** out) {
+    if (arena) {
+        const size_t want = (bytes + 255) & ~size_t(255);
+        if (want > arena_cap - arena_used)
+            return fail(PANO_ERR_BAD_SHAPE, "workspace '%s' (%zu bytes) exceeds the no-allocation scratch of %zu bytes",
+                        name, bytes, arena_cap);
+        *out = arena + arena_used;
+        arena_used += want;
+        return PANO_OK;
+    }
     Buffer& b = pool[name];
     if (b.bytes < bytes) {
         if (b.ptr) {
@@ -150,6 +159,7 @@ pano_status pano_ctx_create(int32_t device, pano_ctx* out) {
     }
     if (device < 0 || device >= n) return fail(PANO_ERR_INVALID, "device %d out of range [0,%d)", device, n);
     PANO_HIP(hipSetDevice(device));
+    PANO_CHECK(scratch_setup(device));
     Ctx* c = new Ctx();
     c->device = device;
     // Default priority stream (no priority streams anywhere in this library).
