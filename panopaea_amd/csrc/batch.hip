@@ -343,15 +343,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // the sweeps start from 0), o[1] = z_l (output), o[2] = r_l. Tile (bx, by, bz)
 // of E.G as in the single-grid kernel. With DOT, the tile's z.r sums of the
 // last two half-sweeps go to p0[base + b] and p2[base + b].
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int SK>
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb(
     BTab T, char* A, const SolverState* st, double* p0, double* p2, int base) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
-    __shared__ float lds[TbShape<4, TXP, TY, NT, SK>::LDS_FLOATS];
+    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
     const int bx = lb % E.G.gx, q = lb / E.G.gx, by = q % E.G.gy, bz = q / E.G.gy;
     double acc[2];
-    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, SK>(lds, E.D, bx, by, bz, E.zc, fld<float>(A, E.o[0]),
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, E.D, bx, by, bz, E.zc, fld<float>(A, E.o[0]),
                                                fld<float>(A, E.o[1]), fld<float>(A, E.o[2]), (float)E.s, E.rinv,
                                                acc);
     if constexpr (DOT) {
@@ -918,13 +918,13 @@ void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
         if (t.nbt == 0) return;
         const dim3 g(t.nbt), b(C::NT);
         if (!post)
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 0, true, false, C::SK>), g, b, 0, c->stream,
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 0, true, false>), g, b, 0, c->stream,
                                t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
         else if (dot)
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, true, C::SK>), g, b, 0, c->stream,
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, true>), g, b, 0, c->stream,
                                t.view(), arena, st, part0_, part2_, tb_base_[k]);
         else
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, false, C::SK>), g, b, 0, c->stream,
+            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, false>), g, b, 0, c->stream,
                                t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
     };
     go(TbLarge{}, 0);

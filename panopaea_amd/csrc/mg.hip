@@ -177,16 +177,16 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
 // NS half-sweeps from xin into xout (TbThread); with DOT the workgroup's sums
 // of z.r over its tile for the last two half-sweeps go to part[L] and
 // part[nb + L].
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int SK>
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb(
     Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
     const SolverState* st) {
     if (st->done) return;
-    __shared__ float lds[TbShape<4, TXP, TY, NT, SK>::LDS_FLOATS];
+    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
     const int L = xcd_remap(blockIdx.x, g.nb);
     const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
     double acc[2];
-    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, SK>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc);
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc);
     if constexpr (DOT) {
         const double a0 = block_sum(acc[0]);
         const double a1 = block_sum(acc[1]);
@@ -234,8 +234,8 @@ static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, con
     with_tb(D, [&](auto cfg) {
         using C = decltype(cfg);
         const TbGrid g = tb_grid_of<C>(D);
-        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT, C::SK>, dim3(g.nb),
-                dim3(C::NT), D, g, xin, xout, r, s, ri, part, st);
+        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT), D, g, xin, xout, r,
+                s, ri, part, st);
         if (np) *np = 2 * g.nb;
         return 0;
     });

@@ -28,15 +28,6 @@
 // in an earlier phase; the kernel puts a barrier after each half-sweep:
 // prologue | begin + half-sweep 0 | 1 | ... | NS-1 | end + begin of the next
 // step + half-sweep 0 | ...
-//
-// Skewed form (SK = 2): at step t half-sweep h updates plane t - 2h instead.
-// Half-sweep h then reads planes t-2h-1 .. t-2h+1 of the colour half-sweep
-// h-1 wrote, and h-1 finished them in steps t-3 .. t-1, while h+1 reaches
-// them only in steps t+1 .. t+3. No half-sweep of a step reads what another
-// half-sweep of the same step writes, so a step needs one barrier (after its
-// half-sweeps) instead of NS. The ring then holds SK*(NS-1)+4 planes and b
-// stays in registers for SK*(NS-1)+1 planes. Every cell still gets the same
-// six neighbour values in the same order: the bits do not depend on SK.
 #pragma once
 
 #include <utility>
@@ -45,17 +36,14 @@
 
 namespace pano {
 
-template <int NS, int TXP, int TY, int NTHREADS, int SK = 1>
+template <int NS, int TXP, int TY, int NTHREADS>
 struct TbShape {
     static_assert(NS % 2 == 0 && TY % 2 == 0, "even half-sweep count and tile height");
-    static_assert(SK == 1 || SK == 2, "stage skew 1 (a barrier per half-sweep) or 2 (one per step)");
     static constexpr int HP = NS / 2;                // halo pairs in x (2*HP = NS cells)
     static constexpr int PW = TXP + 2 * HP;          // pairs per LDS row
     static constexpr int PH = TY + 2 * NS;           // LDS rows
     static constexpr int PLANE = PW * PH;            // cells of one colour of one plane
-    static constexpr int DEPTH = SK * (NS - 1);      // plane lag of the last half-sweep
-    static constexpr int SLOTS = DEPTH + 4;          // resident planes t-DEPTH-1 .. t+2 at step t
-    static constexpr int NB = DEPTH + 1;             // planes of b in registers
+    static constexpr int SLOTS = NS + 3;             // resident planes t-NS .. t+2 at step t
     static constexpr int NT = NTHREADS;              // threads per workgroup
     static constexpr int CPOS = (PH / 2) * PW;       // positions of one row parity
     static constexpr int KC = (CPOS + NT - 1) / NT;  // position slots per thread and parity
@@ -113,11 +101,11 @@ __host__ inline float tb_opaque(float v) { return v; }
 // while this one stores). r: right-hand side. With DOT, acc[0] and acc[1]
 // collect sum(v*r) over the tile's cells of the last two half-sweeps.
 // Needs D.d >= 2 (every cell then has a z neighbour, so nc >= 1).
-template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT, int SK = 1>
+template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT>
 struct TbThread {
-    using S = TbShape<NS, TXP, TY, NTHREADS, SK>;
+    using S = TbShape<NS, TXP, TY, NTHREADS>;
     static constexpr int PW = S::PW, PH = S::PH, PLANE = S::PLANE, SLOTS = S::SLOTS, NT = S::NT, KC = S::KC,
-                         KP = S::KP, HP = S::HP, CPOS = S::CPOS, NB = S::NB, DEPTH = S::DEPTH;
+                         KP = S::KP, HP = S::HP, CPOS = S::CPOS;
 
     float* lds;
     Dims D;
@@ -131,7 +119,7 @@ struct TbThread {
     int act[KP];             // bit h + 4p: half-sweep h updates the cell when its x parity is p
     bool ind[KP], tile[KP];  // inside the grid; inside the tile
     float rin[KP][2], rbd[KP][2];  // 1/(s*nc) for x parity p: interior plane / one z side missing
-    float xpf[2][KP], bpf[2][KP], bq[NB][2][KP];  // bq[j]: b of plane t - j at step t
+    float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
     bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
     double acc[2];
 
@@ -228,7 +216,7 @@ struct TbThread {
         }
     }
     __host__ __device__ __forceinline__ int first_step() const { return t0; }
-    __host__ __device__ __forceinline__ int last_step() const { return ze - 1 + DEPTH; }
+    __host__ __device__ __forceinline__ int last_step() const { return ze + NS - 2; }
 
     // planes t0-1 .. t0+1 into slots 0..2; b of plane t0; prefetch x of t0+2, b of t0+1
     __host__ __device__ __forceinline__ void prologue() {
@@ -239,7 +227,7 @@ struct TbThread {
         load_x(t0 + 1);
         fill<2>();
 #pragma unroll
-        for (int j = 1; j < NB; ++j)
+        for (int j = 1; j < NS; ++j)
 #pragma unroll
             for (int k = 0; k < KP; ++k) bq[j][0][k] = bq[j][1][k] = 0.0f;
         const bool ok0 = load_b(t0, bq[0]);
@@ -259,15 +247,15 @@ struct TbThread {
         fill<mod(J + 3)>();
         load_x(t + 3);
     }
-    // half-sweep H on plane t-SK*H (t - t0 = J mod SLOTS)
+    // half-sweep H on plane t-H (t - t0 = J mod SLOTS)
     template <int H, int J>
     __host__ __device__ __forceinline__ void stage(int t) {
-        const int z = t - SK * H;
+        const int z = t - H;
         const int lo = zs - (NS - 1 - H) > 0 ? zs - (NS - 1 - H) : 0;
         const int hi = ze + (NS - 1 - H) < D.d ? ze + (NS - 1 - H) : D.d;
         if (z < lo || z >= hi) return;
         constexpr int c = (CF + H) & 1;
-        constexpr int sz = mod(J - SK * H + 1), sm = mod(J - SK * H), sp = mod(J - SK * H + 2);
+        constexpr int sz = mod(J - H + 1), sm = mod(J - H), sp = mod(J - H + 2);
         const float* xo = lds + (sz * 2 + (1 - c)) * PLANE;
         const float* xm = lds + (sm * 2 + (1 - c)) * PLANE;
         const float* xp = lds + (sp * 2 + (1 - c)) * PLANE;
@@ -280,7 +268,7 @@ struct TbThread {
             const int p = (c + k / KC + z) & 1;  // x parity of the colour-c cell (row parity k / KC)
             if ((act[k] >> (H + 4 * p)) & 1) {
                 const int e = e_[k];
-                const float rv = bq[SK * H][c][k];
+                const float rv = bq[H][c][k];
                 const float a0 = tb_opaque(zin ? rin[k][0] : rbd[k][0]);
                 const float a1 = tb_opaque(zin ? rin[k][1] : rbd[k][1]);
                 float v;
@@ -303,13 +291,13 @@ struct TbThread {
             }
         }
     }
-    // store plane t-DEPTH (final after half-sweep NS-1 of step t); advance the
+    // store plane t-NS+1 (final after half-sweep NS-1 of step t); advance the
     // b ring; prefetch b of plane t+2
     template <int J>
     __host__ __device__ __forceinline__ void end_step(int t) {
-        const int zo = t - DEPTH;
+        const int zo = t - NS + 1;
         if (zo >= zs && zo < ze) {
-            const float* base = lds + mod(J - DEPTH + 1) * 2 * PLANE;
+            const float* base = lds + mod(J - NS + 2) * 2 * PLANE;
             float* o0 = xout + zo * hw;
             float* o1 = o0 + n2;
 #pragma unroll
@@ -321,7 +309,7 @@ struct TbThread {
             }
         }
 #pragma unroll
-        for (int j = NB - 1; j > 0; --j)
+        for (int j = NS - 1; j > 0; --j)
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 bq[j][0][k] = bq[j - 1][0][k];
@@ -466,13 +454,13 @@ struct Tb2Thread {
 // small levels, so every CU still gets workgroups. WANT/ZMIN: workgroups
 // aimed for and the shortest chunk (tb_grid).
 struct TbLarge {  // >= 2^23 cells
-    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 512, ZMIN = 8, SK = 1;
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 512, ZMIN = 8;
 };
 struct TbMedium {  // >= 2^20 cells
-    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 1024, ZMIN = 4, SK = 1;
+    static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 6, WANT = 1024, ZMIN = 4;
 };
 struct TbSmall {
-    static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2, SK = 1;
+    static constexpr int TXP = 32, TY = 8, NT = 320, WPE = 4, WANT = 2048, ZMIN = 2;
 };
 // 2D levels: 32 x-pairs by 32 rows, one plane per workgroup
 struct Tb2D {
@@ -520,16 +508,11 @@ inline bool tb_on(bool enabled, int nu, const Dims& D) {
 }
 
 // One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
-// after each (SK = 1) or one barrier after all of them (SK = 2), end.
+// after each, end.
 template <int J, typename TH, int... H>
 __device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
     th.template begin_step<J>(t);
-    if constexpr (TH::S::DEPTH == sizeof...(H) - 1) {
-        ((th.template stage<H, J>(t), __syncthreads()), ...);
-    } else {
-        (th.template stage<H, J>(t), ...);
-        __syncthreads();
-    }
+    ((th.template stage<H, J>(t), __syncthreads()), ...);
     th.template end_step<J>(t);
 }
 // Steps t .. t+SLOTS-1 (up to t1), then t advanced past them.
@@ -540,11 +523,11 @@ __device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_se
 // One workgroup of the one-pass smoother (four half-sweeps) on tile
 // (bx, by, bz) of a level; acc: its tile's z.r sums of the last two
 // half-sweeps (with DOT).
-template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT, int SK = 1>
+template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT>
 __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, int by, int bz, int zc,
                                              const float* xin, float* xout, const float* r, float s,
                                              const float* rinv, double* acc) {
-    using TH = TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT, SK>;
+    using TH = TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT>;
     TH th;
     th.init(lds, D, bx, by, bz, zc, xin, xout, r, s, rinv, threadIdx.x);
     th.prologue();

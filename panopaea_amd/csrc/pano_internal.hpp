@@ -216,6 +216,12 @@ pano_status gen_faces_enqueue(Ctx* c, const pano_grid* g, int n_modes, double am
 // Fluid-step operator launches (fluid.hip), templated on T in {float,double}.
 template <typename T>
 pano_status launch_neg_div(Ctx* c, const Dims& D, T* b, const T* faces);
+// The step's advection of a 3D f32 grid in one pass (density and all face
+// components; bit-identical to launch_advect + launch_advect_mac). Used
+// unless the environment sets PANO_ADV_FUSED=0.
+bool advect_fused_on(const Dims& D);
+pano_status launch_advect_fused(Ctx* c, const Dims& D, float* rho_out, const float* rho_in, float* vel_out,
+                                const float* vel_in, double dt, double buoy_c, const double* src_sphere);
 template <typename T>
 pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t* mask, T* q,
                                  const T* p);

@@ -508,9 +508,14 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
     const double* sp = P->src_radius > 0 ? sphere : nullptr;
     // source -> advection of density and velocity from the old velocity
-    PANO_CHECK(launch_advect<T>(c, D, rho_out, rho_in, P->dt, vel_in, sp));
     double buoy_c = (double)(T)(P->dt * P->buoyancy * 0.5);
-    PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
+    if (sizeof(T) == 4 && advect_fused_on(D)) {
+        PANO_CHECK(launch_advect_fused(c, D, (float*)rho_out, (const float*)rho_in, (float*)vel_out,
+                                       (const float*)vel_in, P->dt, buoy_c, sp));
+    } else {
+        PANO_CHECK(launch_advect<T>(c, D, rho_out, rho_in, P->dt, vel_in, sp));
+        PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
+    }
     // b = -div u
     Faces<const T> v = split_faces(D, (const T*)vel_out);
     Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);

@@ -19,6 +19,7 @@
 #include <random>
 #include <vector>
 
+#include "adv_fused.hpp"
 #include "fluid_device.hpp"
 using namespace pano;
 
@@ -173,6 +174,82 @@ static int case2(int h, int w, T dt, unsigned seed) {
     return bad;
 }
 
+// ---- the fused 3D advection (adv_fused.hpp), one workgroup at a time -----------------------
+// The oracle's step composition: the source sphere applied to rho, then
+// or_advect3 of rho and or_advect_mac3 of the velocity with the buoyancy of
+// the source-applied rho. The workgroup's threads run each phase between two
+// barriers in the order given (forward, reverse, or thread by thread).
+static Sphere sphere_of(double cz, double cy, double cx, double r) {
+    Sphere S{cz, cy, cx, r * r, 1, {0, 0, 0}, {0, 0, 0}};
+    const double c[3] = {cz, cy, cx};
+    for (int a = 0; a < 3; ++a) {
+        S.lo[a] = (int)std::floor(c[a] - r - 0.5) - 1;
+        S.hi[a] = (int)std::ceil(c[a] + r - 0.5) + 1;
+    }
+    return S;
+}
+
+template <int TX, int TY>
+static int fused_case(int d, int h, int w, float dt, float vamp, int zopen, int zc, int order, unsigned seed) {
+    using TH = AdvThread<TX, TY>;
+    Dims D{d, h, w, zopen ? 32 : 0};
+    const size_t nx = (size_t)d * h * (w + 1), ny = (size_t)d * (h + 1) * w, nz = (size_t)(d + 1) * h * w,
+                 nc = (size_t)d * h * w;
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    Arr<float> vx(nx), vy(ny), vz(nz), rho(nc);
+    for (size_t i = 0; i < nx; ++i) vx.p[i] = (float)(vamp * U(rng));
+    for (size_t i = 0; i < ny; ++i) vy.p[i] = (float)(vamp * U(rng));
+    for (size_t i = 0; i < nz; ++i) vz.p[i] = (float)(vamp * U(rng));
+    for (size_t i = 0; i < nc; ++i) rho.p[i] = (float)(0.5 * (U(rng) + 1.0));
+    const Sphere S = sphere_of(0.3 * d, 0.5 * h, 0.6 * w, 0.25 * (h < w ? h : w) + 0.7);
+    const float buoy = 0.37f;
+    // the GPU form
+    Arr<float> ox(nx), oy(ny), oz(nz), orho(nc);
+    for (size_t i = 0; i < nx; ++i) ox.p[i] = NAN;
+    for (size_t i = 0; i < ny; ++i) oy.p[i] = NAN;
+    for (size_t i = 0; i < nz; ++i) oz.p[i] = NAN;
+    for (size_t i = 0; i < nc; ++i) orho.p[i] = NAN;
+    const AdvGrid g = adv_grid<TX, TY>(D, zc);
+    for (int L = 0; L < g.nb; ++L) {
+        const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
+        std::vector<float> lds(TH::S::LDS_FLOATS, NAN);
+        std::vector<TH> th(TH::S::NT);
+        for (int i = 0; i < TH::S::NT; ++i)
+            th[i].init(lds.data(), D, bx, by, bz, g.zc, Faces<const float>{vz.p, vy.p, vx.p}, rho.p,
+                       Faces<float>{oz.p, oy.p, ox.p}, orho.p, S, dt, buoy, i);
+        for (auto& t : th) t.prologue();
+        for (int z = th[0].first_plane(); z < th[0].last_plane(); ++z) {
+            const int n = (int)th.size();
+            auto outputs = [&](TH& t) { t.outputs(z); };
+            if (order == 2) {  // thread by thread: all phases of one thread, then the next
+                for (int i = 0; i < n; ++i) { th[i].prefetch(z); outputs(th[i]); th[i].fill_next(z); }
+            } else {
+                for (int k = 0; k < n; ++k) { int i = order ? n - 1 - k : k; th[i].prefetch(z); }
+                for (int k = 0; k < n; ++k) { int i = order ? n - 1 - k : k; outputs(th[i]); }
+                for (int k = 0; k < n; ++k) { int i = order ? n - 1 - k : k; th[i].fill_next(z); }
+            }
+        }
+    }
+    // the oracle's composition
+    Arr<float> rs(nc), ex(nx), ey(ny), ez(nz), erho(nc);
+    memcpy(rs.p, rho.p, nc * sizeof(float));
+    for (int z = 0; z < d; ++z)
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                double dz = (double)z + 0.5 - S.cz, dy = (double)y + 0.5 - S.cy, dx = (double)x + 0.5 - S.cx;
+                if (dz * dz + dy * dy + dx * dx <= S.r2) rs.p[(z * h + y) * w + x] = 1.0f;
+            }
+    or_advect3_f32(d, h, w, erho.p, rs.p, dt, vz.p, vy.p, vx.p);
+    const int bc[6] = {0, 0, 0, 0, 0, zopen};
+    or_advect_mac3_f32(d, h, w, ez.p, ey.p, ex.p, vz.p, vy.p, vx.p, dt, vz.p, vy.p, vx.p, bc, buoy, rs.p);
+    int bad = compare("fused rho", orho.p, erho.p, nc);
+    bad += compare("fused fx", ox.p, ex.p, nx);
+    bad += compare("fused fy", oy.p, ey.p, ny);
+    bad += compare("fused fz", oz.p, ez.p, nz);
+    return bad;
+}
+
 int main() {
     const int dims3[][3] = {{2, 2, 2}, {3, 5, 7}, {5, 4, 1}, {4, 3, 2}, {3, 1, 5}, {7, 9, 13}, {4, 4, 64}, {3, 2, 65},
                             {5, 6, 3}};
@@ -200,6 +277,24 @@ int main() {
             fails += b != 0;
             if (b) printf("BAD 2D (%d,%d) dt=%g: %d mismatches\n", s[0], s[1], dt, b);
         }
+    // the fused 3D advection: slow flows (every corner in LDS), fast flows (corners from
+    // global memory), tiles that cover the grid partly, chunks of 1..16 planes
+    const int dimsf[][3] = {{2, 2, 2}, {5, 9, 70}, {17, 8, 64}, {3, 17, 130}, {20, 4, 5}, {6, 11, 3}, {9, 1, 66}};
+    const double flows[][2] = {{0.3, 1.0}, {0.05, 15.0}, {1.0, 2.5}, {4.0, 1.0}};  // (dt, |v| max)
+    for (auto& s : dimsf)
+        for (auto& fl : flows)
+            for (int zc : {16, 3, 1})
+                for (int order : {0, 1, 2}) {
+                    if (order && zc != 3) continue;
+                    int b = fused_case<64, 8>(s[0], s[1], s[2], (float)fl[0], (float)fl[1], (zc + order) & 1, zc,
+                                              order, seed++);
+                    b += fused_case<8, 4>(s[0], s[1], s[2], (float)fl[0], (float)fl[1], zc & 1, zc, order, seed++);
+                    cases += 2;
+                    fails += b != 0;
+                    if (b)
+                        printf("BAD fused (%d,%d,%d) dt=%g v=%g zc=%d order=%d: %d mismatches\n", s[0], s[1], s[2],
+                               fl[0], fl[1], zc, order, b);
+                }
     printf("%d cases, %d failed\n", cases, fails);
     return fails != 0;
 }

@@ -46,52 +46,28 @@ static void ref_sweeps(const Dims& D, int NS, int CF, bool zero_x, std::vector<f
     }
 }
 
-// Order of the work between two barriers. STAGE_MAJOR: every thread runs
-// half-sweep 0, then every thread half-sweep 1, ... (a barrier after each, the
-// unskewed kernel). The skewed kernel has no barrier between the half-sweeps
-// of a step, so any interleaving is legal there; THREAD_FWD / THREAD_REV run
-// all half-sweeps of one thread before the next thread (in both directions),
-// which would expose a half-sweep reading what another one of the same step
-// writes.
-enum Order { STAGE_MAJOR = 0, THREAD_FWD = 1, THREAD_REV = 2 };
-
-template <int NS, int J, int H = 0, typename TT>
-static void thread_stages(TT& tt, int t) {
-    if constexpr (H < NS) {
-        tt.template stage<H, J>(t);
-        thread_stages<NS, J, H + 1>(tt, t);
-    }
-}
 template <int NS, int J, int H = 0, typename V>
-static void stages(V& th, int t, int order) {
-    if (order == THREAD_FWD) {
-        for (auto& tt : th) thread_stages<NS, J>(tt, t);
-        return;
-    }
-    if (order == THREAD_REV) {
-        for (int i = (int)th.size() - 1; i >= 0; --i) thread_stages<NS, J>(th[i], t);
-        return;
-    }
+static void stages(V& th, int t) {
     if constexpr (H < NS) {
         for (auto& tt : th) tt.template stage<H, J>(t);
-        stages<NS, J, H + 1>(th, t, order);
+        stages<NS, J, H + 1>(th, t);
     }
 }
 template <int NS, int SL, int J = 0, typename V>
-static void round_(V& th, int& t, int t1, int order) {
+static void round_(V& th, int& t, int t1) {
     if constexpr (J < SL) {
         if (t > t1) return;
         for (auto& tt : th) tt.template begin_step<J>(t);
-        stages<NS, J>(th, t, order);
+        stages<NS, J>(th, t);
         for (auto& tt : th) tt.template end_step<J>(t);
         ++t;
-        round_<NS, SL, J + 1>(th, t, t1, order);
+        round_<NS, SL, J + 1>(th, t, t1);
     }
 }
 
-template <int NS, int TXP, int TY, int CF, bool ZX, bool DOT, int NT = 512, int SK = 1>
-static int run(const Dims& D, int want, int zmin, unsigned seed, int order = STAGE_MAJOR) {
-    using TH = TbThread<NS, TXP, TY, NT, CF, ZX, DOT, SK>;
+template <int NS, int TXP, int TY, int CF, bool ZX, bool DOT, int NT = 512>
+static int run(const Dims& D, int want, int zmin, unsigned seed) {
+    using TH = TbThread<NS, TXP, TY, NT, CF, ZX, DOT>;
     const int n = D.d * D.h * D.w;
     std::mt19937 rng(seed);
     std::uniform_real_distribution<float> U(-1, 1);
@@ -114,7 +90,7 @@ static int run(const Dims& D, int want, int zmin, unsigned seed, int order = STA
             th[i].init(lds.data(), D, bx, by, bz, g.zc, xin.data(), xout.data(), r.data(), s, ri.v, i);
         for (auto& t : th) t.prologue();
         int t = th[0].first_step(), t1 = th[0].last_step();
-        while (t <= t1) round_<NS, TH::SLOTS>(th, t, t1, order);
+        while (t <= t1) round_<NS, TH::SLOTS>(th, t, t1);
         for (auto& tt : th) { dsum[0] += tt.acc[0]; dsum[1] += tt.acc[1]; }
     }
     int bad = 0;
@@ -125,9 +101,8 @@ static int run(const Dims& D, int want, int zmin, unsigned seed, int order = STA
         }
     double e0 = DOT ? fabs(dsum[0] - dref[0]) / (fabs(dref[0]) + 1e-30) : 0;
     double e1 = DOT ? fabs(dsum[1] - dref[1]) / (fabs(dref[1]) + 1e-30) : 0;
-    printf("NS=%d SK=%d order=%d CF=%d ZX=%d DOT=%d dims %dx%dx%d bc=%d grid gx=%d gy=%d zc=%d nb=%d: %d mismatches, "
-           "dot rel err %.2e %.2e\n",
-           NS, SK, order, CF, ZX, DOT, D.d, D.h, D.w, D.bc, g.gx, g.gy, g.zc, g.nb, bad, e0, e1);
+    printf("NS=%d CF=%d ZX=%d DOT=%d dims %dx%dx%d bc=%d grid gx=%d gy=%d zc=%d nb=%d: %d mismatches, dot rel err %.2e %.2e\n",
+           NS, CF, ZX, DOT, D.d, D.h, D.w, D.bc, g.gx, g.gy, g.zc, g.nb, bad, e0, e1);
     return bad || e0 > 1e-12 || e1 > 1e-12;
 }
 
@@ -214,13 +189,6 @@ int main() {
             fails += run<4, 32, 8, 1, false, false, 320>(D, 2048, zmin, seed++);
             // a small generic tile (several workgroups on every shape)
             fails += run<4, 8, 4, 1, false, true, 64>(D, 64, zmin, seed++);
-            // the skewed form (one barrier per step), every half-sweep of a step in any order
-            for (int order : {STAGE_MAJOR, THREAD_FWD, THREAD_REV}) {
-                fails += run<4, 32, 32, 0, true, false, 768, 2>(D, 512, zmin, seed++, order);
-                fails += run<4, 32, 32, 1, false, true, 768, 2>(D, 512, zmin, seed++, order);
-                fails += run<4, 32, 16, 1, false, false, 768, 2>(D, 1024, zmin, seed++, order);
-                fails += run<4, 8, 4, 1, false, true, 64, 2>(D, 64, zmin, seed++, order);
-            }
         }
     }
     const Dims c2[] = {{1, 64, 64, 0}, {1, 40, 70, 2}, {1, 100, 200, 15}, {1, 8, 6, 4}, {1, 33, 130, 9}};
