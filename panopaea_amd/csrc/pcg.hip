@@ -278,8 +278,12 @@ static pano_status solver_init(Solver<T>* S, int max_iterations, int rhs_parts =
     return PANO_OK;
 }
 
+// One CG iteration in two parts: the head (q = A p, alpha, the x/r update and
+// the stop test) and the tail (z = M r, beta, the search). The loop can end
+// an iteration after its head: when the stop test fires there, the tail's
+// launches would all be no-ops.
 template <typename T>
-static pano_status solver_iteration(Solver<T>* S) {
+static pano_status solver_iter_head(Solver<T>* S) {
     Ctx* c = S->c;
     const Dims& D = S->D;
     const double unit = (double)S->n * sizeof(T);  // one field pass
@@ -316,6 +320,16 @@ static pano_status solver_iteration(Solver<T>* S) {
     klaunch(c, "update", 3 * unit, k_update_r<T>, dim3(S->nflat), dim3(kBlock), S->n, S->r, (const T*)S->q,
             S->part1, ident ? S->part0 : (double*)nullptr, cst);
     fin(S, FIN_RNORM, S->part1, S->nflat, 1);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
+template <typename T>
+static pano_status solver_iter_tail(Solver<T>* S) {
+    Ctx* c = S->c;
+    const double unit = (double)S->n * sizeof(T);  // one field pass
+    const SolverState* cst = S->st;
+    const bool ident = S->precond == PANO_PRECOND_IDENTITY;
     // z = M r ; sigma' = z.r ; beta   (pcg.rs:65-68)
     T* z;
     if (ident) {
@@ -335,13 +349,20 @@ static pano_status solver_iteration(Solver<T>* S) {
     return PANO_OK;
 }
 
+template <typename T>
+static pano_status solver_iteration(Solver<T>* S) {
+    PANO_CHECK(solver_iter_head<T>(S));
+    return solver_iter_tail<T>(S);
+}
+
 // Runs the CG loop. Every loop kernel returns at once after convergence, but
 // a launch past convergence still dispatches all its workgroups, so the loop
 // avoids them: with check_every == 0 it enqueues as many iterations as the
 // previous solve of the same problem needed (a hint kept in the context; 4
-// without one), then polls the state after each further iteration. With
-// check_every > 0 it enqueues chunks of that size, two in flight. Leaves the
-// final state in c->host_state[0]. The result does not depend on the hint.
+// without one), the last of them up to its stop test only, then polls the
+// state and continues iteration by iteration. With check_every > 0 it
+// enqueues chunks of that size, two in flight. Leaves the final state in
+// c->host_state[0]. The result does not depend on the hint.
 template <typename T>
 static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string* key) {
     Ctx* c = S->c;
@@ -370,10 +391,24 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
     } else {
         auto it = key ? c->iter_hint.find(*key) : c->iter_hint.end();
         int first = it != c->iter_hint.end() ? it->second : 4;
-        PANO_CHECK(enqueue(first < 1 ? 1 : first));
-        for (;;) {
+        if (first < 1) first = 1;
+        if (first > max_iterations) first = max_iterations;
+        hs[0].done = 1;  // nothing to run when max_iterations is 0
+        if (first >= 1) {
+            // the hinted count, the last iteration's head only: a solve that converges as the
+            // previous one did stops there without enqueueing a tail of no-op launches
+            PANO_CHECK(enqueue(first - 1));
+            PANO_CHECK(solver_iter_head<T>(S));
             PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
             PANO_HIP(hipStreamSynchronize(c->stream));
+            if (!hs[0].done) PANO_CHECK(solver_iter_tail<T>(S));
+            launched += 1;
+        }
+        for (;;) {
+            if (!hs[0].done && launched < max_iterations) {
+                PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+                PANO_HIP(hipStreamSynchronize(c->stream));
+            }
             if (hs[0].done || launched >= max_iterations) break;
             PANO_CHECK(enqueue(it != c->iter_hint.end() ? 1 : 2));
         }
