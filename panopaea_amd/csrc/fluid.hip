@@ -170,8 +170,11 @@ __global__ __launch_bounds__(256) void k_neg_div_masked2(Dims D, Grid3 G, T* b, 
 
 // The projection of every face (project_cells_run), one thread per run of
 // kPlaneRun cells along z; pout (optional): the natural-layout pressure export.
+// gate (optional): a no-op unless that solve has finished.
 template <typename T, int DIM, int L_>
-__global__ __launch_bounds__(256) void k_project_cells(Dims D, Grid3 G, Faces<T> u, const T* p, T dt, T* pout) {
+__global__ __launch_bounds__(256) void k_project_cells(Dims D, Grid3 G, Faces<T> u, const T* p, T dt, T* pout,
+                                                       const SolverState* gate) {
+    if (gate && !solve_finished(gate)) return;
     constexpr int NZ = kPlaneRun<DIM>;
     PLANE_RUN_BEGIN(NZ);
     if (x >= D.w || y >= D.h) return;
@@ -292,25 +295,28 @@ pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t
 }
 
 template <typename T, int L_>
-static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
+static pano_status launch_project_l(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out,
+                                    const SolverState* gate) {
     Faces<T> u = split_faces(D, faces);
     if (D.d > 1) {
         Grid3 Gc = grid3_run(D.w, D.h, D.d, kPlaneRun<3>);
-        LAUNCH3((k_project_cells<T, 3, L_>), Gc, D, Gc, u, p, (T)dt, p_out);
+        LAUNCH3((k_project_cells<T, 3, L_>), Gc, D, Gc, u, p, (T)dt, p_out, gate);
     } else {
         Grid3 Gc = grid3_run(D.w, D.h, 1, 1);
-        LAUNCH3((k_project_cells<T, 2, L_>), Gc, D, Gc, u, p, (T)dt, p_out);
+        LAUNCH3((k_project_cells<T, 2, L_>), Gc, D, Gc, u, p, (T)dt, p_out, gate);
     }
     return PANO_OK;
 }
 
 template <typename T>
-pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
-    return launch_project_l<T, NATURAL>(c, D, dt, faces, p, p_out);
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out,
+                           const SolverState* gate) {
+    return launch_project_l<T, NATURAL>(c, D, dt, faces, p, p_out, gate);
 }
 template <typename T>
-pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out) {
-    return launch_project_l<T, REDBLACK>(c, D, dt, faces, p, p_out);
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out,
+                              const SolverState* gate) {
+    return launch_project_l<T, REDBLACK>(c, D, dt, faces, p, p_out, gate);
 }
 
 #define INST(T)                                                                                     \
@@ -321,8 +327,10 @@ pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const 
     template pano_status launch_neg_div<T>(Ctx*, const Dims&, T*, const T*);                         \
     template pano_status launch_poisson_apply<T>(Ctx*, const Dims&, double, const uint8_t*, T*,       \
                                                  const T*);                                         \
-    template pano_status launch_project<T>(Ctx*, const Dims&, double, T*, const T*, T*);             \
-    template pano_status launch_project_rb<T>(Ctx*, const Dims&, double, T*, const T*, T*);
+    template pano_status launch_project<T>(Ctx*, const Dims&, double, T*, const T*, T*,               \
+                                           const SolverState*);                                     \
+    template pano_status launch_project_rb<T>(Ctx*, const Dims&, double, T*, const T*, T*,            \
+                                              const SolverState*);
 INST(float)
 INST(double)
 #undef INST

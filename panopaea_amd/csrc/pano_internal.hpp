@@ -58,6 +58,13 @@ struct SolverState {
     int32_t pad[2];
 };
 
+// The solve has ended (stop test, iteration limit, non-finite value, or no
+// iteration allowed at all): launches that act on the solution are gated on
+// this, so they can be enqueued before the host knows whether the solve ended.
+__device__ __forceinline__ bool solve_finished(const SolverState* st) {
+    return st->done || st->iter >= st->max_iter;
+}
+
 // Times every launch of one named kernel with HIP events on the context's
 // stream. Launches inside a solve are tagged with the number of convergence
 // checks enqueued before them so that launches the converged flag turned
@@ -180,7 +187,7 @@ __device__ __forceinline__ double block_max(double v) {
 }
 
 // NaN-propagating |x| max for norms: a NaN anywhere makes the norm NaN.
-__device__ __forceinline__ double absmax_acc(double m, double x) {
+__host__ __device__ __forceinline__ double absmax_acc(double m, double x) {
     double a = fabs(x);
     return (a > m || a != a) ? a : m;
 }
@@ -227,9 +234,11 @@ pano_status launch_poisson_apply(Ctx* c, const Dims& D, double dt, const uint8_t
                                  const T* p);
 // p_out (optional, may be null): a natural-layout copy of p written on the way
 template <typename T>
-pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out);
+pano_status launch_project(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out,
+                           const SolverState* gate = nullptr);
 template <typename T>
-pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out);
+pano_status launch_project_rb(Ctx* c, const Dims& D, double dt, T* faces, const T* p, T* p_out,
+                              const SolverState* gate = nullptr);
 template <typename T>
 pano_status launch_advect(Ctx* c, const Dims& D, T* dst, const T* src, double dt, const T* vel,
                           const double* src_sphere /* cz,cy,cx,r or null */);

@@ -163,6 +163,23 @@ size_t pcg_workspace_bytes(const Dims& D) {
     return al(sizeof(SolverState)) + 2 * al((size_t)solver_partials(D, false) * sizeof(double));
 }
 
+// The parameter checks of solver_setup (no device work, no allocation).
+static pano_status solver_check(const Dims& D, const pano_pcg_params* P) {
+    if (P->precond < 0 || P->precond > 2) return fail(PANO_ERR_INVALID, "unknown preconditioner %d", P->precond);
+    if (P->tol_kind != PANO_TOL_ABS_LINF && P->tol_kind != PANO_TOL_REL_LINF)
+        return fail(PANO_ERR_INVALID, "unknown tol_kind %d", P->tol_kind);
+    if (P->max_iterations < 0) return fail(PANO_ERR_INVALID, "max_iterations < 0");
+    const bool rb = P->precond == PANO_PRECOND_MULTIGRID;
+    if (P->face_mask && (D.d != 1 || D.bc != 0 || rb))
+        return fail(PANO_ERR_BAD_SHAPE, "face_mask needs a 2D closed grid and the identity or Jacobi preconditioner");
+    if (rb) {
+        if (P->mg_nu1 != P->mg_nu2 || P->mg_nu1 < 1)
+            return fail(PANO_ERR_INVALID, "multigrid needs mg_nu1 == mg_nu2 >= 1 (a symmetric V-cycle)");
+        PANO_CHECK(mg_check(D, P->mg_levels));
+    }
+    return PANO_OK;
+}
+
 template <typename T>
 static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P, Solver<T>* S) {
     S->c = c;
@@ -172,17 +189,8 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     S->rb = P->precond == PANO_PRECOND_MULTIGRID;
     S->mask = P->face_mask;
     S->dt = (T)P->dt;
-    if (P->precond < 0 || P->precond > 2) return fail(PANO_ERR_INVALID, "unknown preconditioner %d", P->precond);
-    if (P->tol_kind != PANO_TOL_ABS_LINF && P->tol_kind != PANO_TOL_REL_LINF)
-        return fail(PANO_ERR_INVALID, "unknown tol_kind %d", P->tol_kind);
-    if (P->max_iterations < 0) return fail(PANO_ERR_INVALID, "max_iterations < 0");
-    if (S->mask && (D.d != 1 || D.bc != 0 || S->rb))
-        return fail(PANO_ERR_BAD_SHAPE, "face_mask needs a 2D closed grid and the identity or Jacobi preconditioner");
-    if (S->rb) {
-        if (P->mg_nu1 != P->mg_nu2 || P->mg_nu1 < 1)
-            return fail(PANO_ERR_INVALID, "multigrid needs mg_nu1 == mg_nu2 >= 1 (a symmetric V-cycle)");
-        PANO_CHECK(mg_setup<T>(c, D, P->mg_levels, P->mg_nu1, P->dt, &S->mg));
-    }
+    PANO_CHECK(solver_check(D, P));
+    if (S->rb) PANO_CHECK(mg_setup<T>(c, D, P->mg_levels, P->mg_nu1, P->dt, &S->mg));
     S->nflat = (int)blocks_for(S->n, kFlatPer);
     S->G = S->rb ? grid3_rb(D) : grid3(D.w, D.h, D.d);
     S->ntile = S->G.nb;
@@ -355,16 +363,21 @@ static pano_status solver_iteration(Solver<T>* S) {
     return solver_iter_tail<T>(S);
 }
 
-// Runs the CG loop. Every loop kernel returns at once after convergence, but
-// a launch past convergence still dispatches all its workgroups, so the loop
-// avoids them: with check_every == 0 it enqueues as many iterations as the
-// previous solve of the same problem needed (a hint kept in the context; 4
-// without one), the last of them up to its stop test only, then polls the
-// state and continues iteration by iteration. With check_every > 0 it
-// enqueues chunks of that size, two in flight. Leaves the final state in
-// c->host_state[0]. The result does not depend on the hint.
-template <typename T>
-static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string* key) {
+// Runs the CG loop, then the epilogue: the last update's x step when no
+// search followed it (k_x_final) and the caller's launches on the solution
+// (`epi`, kernels gated on solve_finished). Every loop kernel returns at once
+// after convergence, but a launch past convergence still dispatches all its
+// workgroups, so the loop avoids them: with check_every == 0 it enqueues as
+// many iterations as the previous solve of the same problem needed (a hint
+// kept in the context; 4 without one), the last of them up to its stop test
+// only, then the epilogue, and only then waits for the state. When the stop
+// test fired there (the usual case once the hint is right) the solve is done
+// with one host round trip; otherwise the epilogue's launches were no-ops and
+// the loop continues iteration by iteration and enqueues it again. With
+// check_every > 0 it enqueues chunks of that size, two in flight. Leaves the
+// final state in c->host_state[0]. The result does not depend on the hint.
+template <typename T, typename Epi>
+static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string* key, Epi&& epi) {
     Ctx* c = S->c;
     SolverState* hs = c->host_state;
     int launched = 0;
@@ -372,6 +385,17 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
         if (launched + k > max_iterations) k = max_iterations - launched;
         for (int i = 0; i < k; ++i) PANO_CHECK(solver_iteration<T>(S));
         launched += k;
+        return PANO_OK;
+    };
+    auto epilogue = [&]() -> pano_status {
+        hipLaunchKernelGGL((k_x_final<T>), dim3(S->nflat), dim3(kBlock), 0, c->stream, S->n, S->x,
+                           (const T*)S->p, (const SolverState*)S->st);
+        PANO_HIP(hipGetLastError());
+        return epi();
+    };
+    auto read_state = [&]() -> pano_status {
+        PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+        PANO_HIP(hipStreamSynchronize(c->stream));
         return PANO_OK;
     };
     if (every > 0) {
@@ -388,43 +412,45 @@ static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, cons
             }
             ++chunk;
         }
+        PANO_CHECK(epilogue());
+        PANO_CHECK(read_state());
     } else {
         auto it = key ? c->iter_hint.find(*key) : c->iter_hint.end();
         int first = it != c->iter_hint.end() ? it->second : 4;
         if (first < 1) first = 1;
         if (first > max_iterations) first = max_iterations;
-        hs[0].done = 1;  // nothing to run when max_iterations is 0
+        bool finished = false;
         if (first >= 1) {
-            // the hinted count, the last iteration's head only: a solve that converges as the
-            // previous one did stops there without enqueueing a tail of no-op launches
+            // the hinted count, the last iteration up to its stop test, then the epilogue
             PANO_CHECK(enqueue(first - 1));
             PANO_CHECK(solver_iter_head<T>(S));
-            PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-            PANO_HIP(hipStreamSynchronize(c->stream));
-            if (!hs[0].done) PANO_CHECK(solver_iter_tail<T>(S));
             launched += 1;
         }
-        for (;;) {
-            if (!hs[0].done && launched < max_iterations) {
-                PANO_HIP(hipMemcpyAsync(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
-                PANO_HIP(hipStreamSynchronize(c->stream));
+        PANO_CHECK(epilogue());
+        PANO_CHECK(read_state());
+        finished = hs[0].done || hs[0].iter >= hs[0].max_iter;
+        if (!finished) {
+            PANO_CHECK(solver_iter_tail<T>(S));
+            for (;;) {
+                if (launched >= max_iterations) break;
+                PANO_CHECK(enqueue(it != c->iter_hint.end() ? 1 : 2));
+                PANO_CHECK(read_state());
+                if (hs[0].done) break;
             }
-            if (hs[0].done || launched >= max_iterations) break;
-            PANO_CHECK(enqueue(it != c->iter_hint.end() ? 1 : 2));
+            PANO_CHECK(epilogue());
+            PANO_CHECK(read_state());
         }
     }
-    // the last update's x step when no search followed it
-    hipLaunchKernelGGL((k_x_final<T>), dim3(S->nflat), dim3(kBlock), 0, c->stream, S->n, S->x, (const T*)S->p,
-                       (const SolverState*)S->st);
-    PANO_HIP(hipGetLastError());
-    PANO_HIP(hipStreamSynchronize(c->stream));
-    PANO_HIP(hipMemcpy(&hs[0], S->st, sizeof(SolverState), hipMemcpyDeviceToHost));
     if (key) {
         c->iter_hint[*key] = hs[0].iter;
         c->timer.solve_iters.push_back(hs[0].iter);
     }
     c->timer.solve = -1;
     return PANO_OK;
+}
+template <typename T>
+static pano_status solver_loop(Solver<T>* S, int max_iterations, int every, const std::string* key) {
+    return solver_loop<T>(S, max_iterations, every, key, []() { return PANO_OK; });
 }
 
 // Key of the iteration-count hint: the problem shape and solver settings.
@@ -527,6 +553,20 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
                               const T* vel_in, T* rho_out, T* vel_out, T* pressure, pano_pcg_info* info) {
     pano_pcg_params pp = P->pcg;
     pp.dt = P->dt;
+    // the parameters are checked first; the solver's set-up (workspace and
+    // multigrid tables) then runs on the host while the advection runs
+    PANO_CHECK(solver_check(D, &pp));
+    double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
+    const double* sp = P->src_radius > 0 ? sphere : nullptr;
+    // source -> advection of density and velocity from the old velocity
+    double buoy_c = (double)(T)(P->dt * P->buoyancy * 0.5);
+    if (sizeof(T) == 4 && advect_fused_on(D)) {
+        PANO_CHECK(launch_advect_fused(c, D, (float*)rho_out, (const float*)rho_in, (float*)vel_out,
+                                       (const float*)vel_in, P->dt, buoy_c, sp));
+    } else {
+        PANO_CHECK(launch_advect<T>(c, D, rho_out, rho_in, P->dt, vel_in, sp));
+        PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
+    }
     Solver<T> S;
     PANO_CHECK(solver_setup<T>(c, D, &pp, &S));
     void *r, *a, *s, *xv;
@@ -540,17 +580,6 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     S.p = (T*)s;
     S.x = (T*)xv;
     S.q = nullptr;
-    double sphere[4] = {P->src_center[0], P->src_center[1], P->src_center[2], P->src_radius};
-    const double* sp = P->src_radius > 0 ? sphere : nullptr;
-    // source -> advection of density and velocity from the old velocity
-    double buoy_c = (double)(T)(P->dt * P->buoyancy * 0.5);
-    if (sizeof(T) == 4 && advect_fused_on(D)) {
-        PANO_CHECK(launch_advect_fused(c, D, (float*)rho_out, (const float*)rho_in, (float*)vel_out,
-                                       (const float*)vel_in, P->dt, buoy_c, sp));
-    } else {
-        PANO_CHECK(launch_advect<T>(c, D, rho_out, rho_in, P->dt, vel_in, sp));
-        PANO_CHECK(launch_advect_mac<T>(c, D, vel_out, vel_in, P->dt, vel_in, D.d > 1 ? buoy_c : 0.0, rho_in, sp));
-    }
     // b = -div u
     Faces<const T> v = split_faces(D, (const T*)vel_out);
     Grid3 Gd = grid3_run(D.w, D.h, D.d, D.d > 1 ? kPlaneRun<3> : 1);
@@ -569,12 +598,14 @@ static pano_status fluid_step(Ctx* c, const Dims& D, const pano_step_params* P, 
     PANO_HIP(hipGetLastError());
     PANO_CHECK(solver_init<T>(&S, pp.max_iterations, rhs_fused ? (int)Gd.nb : 0));
     const std::string key = hint_key(D, &pp, sizeof(T));
-    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, &key));
-    // u += dt d0_dual(p), closed walls zeroed; the x component's pass also
-    // writes the pressure out in natural layout
-    if (S.rb) PANO_CHECK(launch_project_rb<T>(c, D, P->dt, vel_out, S.x, pressure));
-    else PANO_CHECK(launch_project<T>(c, D, P->dt, vel_out, S.x, pressure));
-    PANO_HIP(hipStreamSynchronize(c->stream));
+    // the loop's epilogue: u += dt d0_dual(p), closed walls zeroed; the x
+    // component's pass also writes the pressure out in natural layout (gated
+    // on the finished solve: the loop may enqueue it before it knows)
+    const SolverState* gate = S.st;
+    PANO_CHECK(solver_loop<T>(&S, pp.max_iterations, pp.check_every, &key, [&]() -> pano_status {
+        if (S.rb) return launch_project_rb<T>(c, D, P->dt, vel_out, S.x, pressure, gate);
+        return launch_project<T>(c, D, P->dt, vel_out, S.x, pressure, gate);
+    }));
     return finish_info(c, info);
 }
 

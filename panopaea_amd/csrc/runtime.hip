@@ -31,7 +31,10 @@ bool timer_pair(Ctx* c, double bytes, hipEvent_t* e0, hipEvent_t* e1) {
     KernelTimer& t = c->timer;
     while (t.used + 2 > t.ev.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return false;
+        // timing only: read after a stream synchronize, so the event needs no
+        // system-scope fence (whose cache write-back would stall the stream
+        // around every timed launch)
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return false;
         t.ev.push_back(e);
     }
     t.tag_solve.push_back(t.solve);

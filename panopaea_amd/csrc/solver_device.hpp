@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void k_search_x(int64_t n, T* x, T* p, const T
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_x_final(int64_t n, T* x, const T* p, const SolverState* st) {
-    if (st->xit >= st->iter) return;
+    if (!solve_finished(st) || st->xit >= st->iter) return;
     flat_axpy(blockIdx.x, n, x, p, st->alpha);
 }
 

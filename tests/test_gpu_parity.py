@@ -561,3 +561,34 @@ def test_gpu_fused_advection_step_equals_per_component_kernels(ctx, monkeypatch,
     assert a[0] == b[0] and a[1] == b[1]
     for x, y in zip(a[2:], b[2:]):
         assert np.array_equal(x, y)
+
+
+def test_gpu_pcg_after_an_overpredicting_hint(ctx):
+    """A solve whose iteration hint (the previous solve of the same problem)
+    is larger than it needs: the loop enqueues iterations past the stop test
+    as no-ops, and the result must equal the same solve in a fresh context
+    (no hint) bit for bit."""
+    dims, bc = (64, 64, 64), (0, 0, 0, 0, 0, 1)
+    fresh = pa.Context()
+    g0 = pa.Grid3d(dims, np.float32, bc=bc)
+    b1 = o.neg_div(o.params(dims, bc=bc), pa.gen_uniform_faces(g0, 11)).astype(np.float32)
+    b2 = b1 * np.float32(2.0 ** -12)  # the same field, scaled exactly: fewer iterations to an absolute threshold
+    thr = 1e-6 * float(np.abs(b1).max())
+
+    def solve(context, b_host):
+        g = pa.Grid3d(dims, np.float32, bc=bc, ctx=context)
+        b = g.new_simplex_2().upload(b_host)
+        x = g.new_simplex_2()
+        scratch = [g.new_simplex_2() for _ in range(3)]
+        info = pa.precond_conjugate_gradient(pa.Multigrid(4, 2), x, b, 200, thr, *scratch,
+                                             pa.PoissonOperator(g, 0.01), tol_kind="abs")
+        return info.iterations, x.download()
+
+    it1, _ = solve(ctx, b1)            # sets the hint of this problem (same key: dims, solver, threshold)
+    it2, x_hinted = solve(ctx, b2)     # converges sooner
+    it_fresh, x_fresh = solve(fresh, b2)
+    fresh.close()
+    assert 2 <= it2 < it1
+    assert it2 == it_fresh
+    assert np.array_equal(x_hinted, x_fresh)
+
