@@ -127,11 +127,16 @@ def main():
     if launches.value > 0:
         avg_s = tot_ms.value / launches.value / 1e3
         achieved = bpl.value / avg_s / 1e9
+        traffic = profiled_traffic(args.kernel, dims)
         roof = {"bound": "hbm", "kernel": args.kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(args.kernel, dims),
+                "traffic": traffic,
                 "launches": launches.value, "avg_us": round(avg_s * 1e6, 2),
                 "bytes_per_launch": bpl.value}
+        if traffic:
+            # the bytes the launch really moves (counter-calibrated) over the same time
+            roof["traffic_gbs"] = round(traffic / avg_s / 1e9, 1)
+            roof["traffic_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4)
 
     # the whole step against the same unit rule: model bytes of the reference
     # method (phase-separated passes) over the measured step time

@@ -411,6 +411,8 @@ pano_status mg_check(const Dims& D, int levels) {
     return PANO_OK;
 }
 
+static bool fdm_split_on(const Dims& C);
+
 template <typename T>
 pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>* m) {
     PANO_CHECK(mg_check(D, levels));
@@ -454,6 +456,7 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
     m->coarse_n[0] = C.w;
     m->coarse_n[1] = C.h;
     m->coarse_n[2] = C.d;
+    m->fdm_split = fdm_split_on(C);
     PANO_CHECK(axis_basis(c, C.w, C.bc & 1, (C.bc >> 1) & 1, &m->fdm_v[0], &m->fdm_lam[0]));
     PANO_CHECK(axis_basis(c, C.h, (C.bc >> 2) & 1, (C.bc >> 3) & 1, &m->fdm_v[1], &m->fdm_lam[1]));
     if (C.d > 1) PANO_CHECK(axis_basis(c, C.d, (C.bc >> 4) & 1, (C.bc >> 5) & 1, &m->fdm_v[2], &m->fdm_lam[2]));
@@ -519,6 +522,187 @@ __global__ __launch_bounds__(kFdmThreads) void k_fdm_zcol(int nx, int ny, int nz
     fdm_zcol_body(fdm_lds, nx, ny, nz, blockIdx.x, data, Vz, lx, ly, lz, s, eps);
 }
 
+// ---- 3D coarsest level over many workgroups -------------------------------------------
+// The three passes of k_fdm_plane (forward), k_fdm_zcol and k_fdm_plane
+// (inverse), each split over the index that its output sums leave free, so a
+// 32^3 level runs on 256 workgroups of 128 threads instead of 32 of 1024:
+//  forward: plane K, a block of columns i: x for those columns (every row),
+//           then y for them;
+//  z:       row j, a block of columns i: z forward, divide, z inverse;
+//  inverse: plane K, a block of rows j: y for those rows (every column),
+//           then x for them.
+// Every output is the same sum, in the same order (q = 0 .. n-1 from 0), of
+// the same products as in lds_axis, so the results are those of the
+// one-workgroup-per-plane kernels bit for bit. Bases are staged in LDS when
+// they fit in kFdmSplitV doubles (else read from global memory).
+constexpr int kFdmSplitThreads = 128;
+constexpr int kFdmSplitV = 4096;
+// outputs per line block: lines of n outputs per workgroup
+__host__ __device__ inline int fdm_split_w(int n) {
+    const int w = kFdmSplitThreads / n;
+    return w < 1 ? 1 : w;
+}
+// doubles staged for two bases of sizes n0^2 and n1^2 (the first, then the
+// second if both fit), as fdm_split_stage stages them
+__host__ __device__ inline int fdm_split_cap(int n0, int n1) {
+    const int a = n0 * n0 <= kFdmSplitV ? n0 * n0 : 0;
+    const int b = a + n1 * n1 <= kFdmSplitV ? n1 * n1 : 0;
+    return a + b;
+}
+// V (or its transpose with T) into dst + used when it fits within cap; returns
+// the staged copy or V itself (nullptr for a transpose that does not fit)
+template <bool TRANS>
+__device__ __forceinline__ const double* fdm_split_stage(double* dst, int& used, int cap, const double* V, int n) {
+    if (used + n * n > cap) return TRANS ? nullptr : V;
+    double* d = dst + used;
+    if (TRANS)
+        lds_fill(d, n * n, [&](int e) {
+            const int q = e / n, i = e - q * n;
+            return V[i * n + q];
+        });
+    else
+        lds_fill(d, n * n, [&](int e) { return V[e]; });
+    used += n * n;
+    return d;
+}
+inline size_t fdm_split_lds_fwd(int nx, int ny) {
+    return (size_t)(nx * ny + ny * fdm_split_w(ny) + fdm_split_cap(nx, ny)) * sizeof(double);
+}
+inline size_t fdm_split_lds_z(int nz) {
+    return (size_t)(2 * nz * fdm_split_w(nz) + fdm_split_cap(nz, 0)) * sizeof(double);
+}
+inline size_t fdm_split_lds_inv(int nx, int ny) {
+    return (size_t)(nx * ny + nx * fdm_split_w(nx) + fdm_split_cap(ny, nx)) * sizeof(double);
+}
+
+// forward x then y of plane K, columns [i0, i0 + nc): in -> out (double)
+template <typename Tin>
+__global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_fwd_cols(int nx, int ny, const Tin* in, double* out,
+                                                                   const double* Vx, const double* Vy,
+                                                                   const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    const int cw = fdm_split_w(ny), ncb = (nx + cw - 1) / cw;
+    const int K = blockIdx.x / ncb, i0 = (blockIdx.x - K * ncb) * cw;
+    const int nc = imin(cw, nx - i0), P = nx * ny;
+    double* a = fdm_lds;   // the plane
+    double* t = a + P;     // its x transform, columns i0.. : t[q * cw + il]
+    double* vs = t + ny * cw;
+    const Tin* src = in + (int64_t)K * P;
+    lds_fill(a, P, [&](int e) { return (double)src[e]; });
+    int used = 0;
+    const int cap = fdm_split_cap(nx, ny);
+    const double* vx = fdm_split_stage<false>(vs, used, cap, Vx, nx);
+    const double* vy = fdm_split_stage<false>(vs, used, cap, Vy, ny);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ny * nc; e += blockDim.x) {
+        const int q = e / nc, il = e - q * nc, i = i0 + il;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int p = 0; p < nx; ++p) acc += vx[p * nx + i] * a[q * nx + p];
+        t[q * cw + il] = acc;
+    }
+    __syncthreads();
+    double* dst = out + (int64_t)K * P;
+    for (int e = threadIdx.x; e < ny * nc; e += blockDim.x) {
+        const int j = e / nc, il = e - j * nc, i = i0 + il;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < ny; ++q) acc += vy[q * ny + j] * t[q * cw + il];
+        dst[j * nx + i] = acc;
+    }
+}
+
+// row j, columns [i0, i0 + nc): z forward, divide by s*(lx+ly+lz) (0 on the
+// null space), z inverse, in place on data (double)
+__global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_z_cols(int nx, int ny, int nz, double* data,
+                                                                 const double* Vz, const double* lx,
+                                                                 const double* ly, const double* lz, double s,
+                                                                 double eps, const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    const int cw = fdm_split_w(nz), ncb = (nx + cw - 1) / cw;
+    const int j = blockIdx.x / ncb, i0 = (blockIdx.x - j * ncb) * cw;
+    const int nc = imin(cw, nx - i0);
+    const int64_t zs = (int64_t)nx * ny;
+    double* sl = fdm_lds;    // sl[k * cw + il]
+    double* u = sl + nz * cw;
+    double* vs = u + nz * cw;
+    lds_fill(sl, nz * cw, [&](int e) {
+        const int k = e / cw, il = e - k * cw;
+        return data[k * zs + (int64_t)j * nx + i0 + imin(il, nc - 1)];
+    });
+    int used = 0;
+    const double* vz = fdm_split_stage<false>(vs, used, fdm_split_cap(nz, 0), Vz, nz);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nz * nc; e += blockDim.x) {
+        const int k = e / nc, il = e - k * nc, i = i0 + il;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < nz; ++q) acc += vz[q * nz + k] * sl[q * cw + il];
+        const double lam = lx[i] + ly[j] + lz[k];
+        u[k * cw + il] = lam > eps ? acc / (s * lam) : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nz * nc; e += blockDim.x) {
+        const int k = e / nc, il = e - k * nc;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < nz; ++q) acc += vz[k * nz + q] * u[q * cw + il];
+        data[k * zs + (int64_t)j * nx + i0 + il] = acc;
+    }
+}
+
+// inverse y then x of plane K, rows [j0, j0 + nr): in (double) -> out
+template <typename Tout>
+__global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_inv_rows(int nx, int ny, const double* in, Tout* out,
+                                                                   const double* Vx, const double* Vy,
+                                                                   const SolverState* st) {
+    if (st->done) return;
+    extern __shared__ double fdm_lds[];
+    const int rw = fdm_split_w(nx), nrb = (ny + rw - 1) / rw;
+    const int K = blockIdx.x / nrb, j0 = (blockIdx.x - K * nrb) * rw;
+    const int nr = imin(rw, ny - j0), P = nx * ny;
+    double* a = fdm_lds;   // the plane
+    double* u = a + P;     // its inverse y transform, rows j0.. : u[jl * nx + i]
+    double* vs = u + rw * nx;
+    const double* src = in + (int64_t)K * P;
+    lds_fill(a, P, [&](int e) { return src[e]; });
+    int used = 0;
+    const int cap = fdm_split_cap(ny, nx);
+    const double* vy = fdm_split_stage<false>(vs, used, cap, Vy, ny);
+    const double* wx = fdm_split_stage<true>(vs, used, cap, Vx, nx);  // W[q*nx+i] = Vx[i*nx+q]
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * nx; e += blockDim.x) {
+        const int jl = e / nx, i = e - jl * nx, j = j0 + jl;
+        double acc = 0.0;
+#pragma unroll 8
+        for (int q = 0; q < ny; ++q) acc += vy[j * ny + q] * a[q * nx + i];
+        u[jl * nx + i] = acc;
+    }
+    __syncthreads();
+    Tout* dst = out + (int64_t)K * P;
+    for (int e = threadIdx.x; e < nr * nx; e += blockDim.x) {
+        const int jl = e / nx, i = e - jl * nx;
+        double acc = 0.0;
+        if (wx) {
+#pragma unroll 8
+            for (int q = 0; q < nx; ++q) acc += wx[q * nx + i] * u[jl * nx + q];
+        } else {
+#pragma unroll 8
+            for (int q = 0; q < nx; ++q) acc += Vx[i * nx + q] * u[jl * nx + q];
+        }
+        dst[(j0 + jl) * nx + i] = (Tout)acc;
+    }
+}
+
+// Whether the 3D coarsest level takes the split passes (PANO_FDM_SPLIT=0: the
+// one-workgroup-per-plane passes).
+static bool fdm_split_on(const Dims& C) {
+    const char* e = getenv("PANO_FDM_SPLIT");
+    return !(e && e[0] == '0') && C.d > 1 && C.w * C.h <= kFdmPlane && C.w * C.d <= kFdmPlane;
+}
+
 // 3D f32 with the fused passes on: the restriction into the coarsest level is
 // done by the coarsest solve's first pass (k_fdm_plane_rc) when that level
 // takes the in-LDS plane path and the level above restricts a stored residual.
@@ -527,6 +711,7 @@ static bool coarse_rc(const MG<T>* m) {
     if constexpr (!std::is_same<T, float>::value || DIM != 3) {
         return false;
     } else {
+        if (m->fdm_split) return false;
         const int Lc = m->levels - 1;
         const Dims& C = m->dims[Lc];
         return m->tb && Lc >= 1 && !rr_on(m->tb, m->dims[Lc - 1]) && C.w * C.h <= kFdmPlane &&
@@ -557,6 +742,19 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
         hipLaunchKernelGGL(k_fdm2_cols, dim3(ncol), dim3(256), lds, S, nx, ny, t0, Vy, lx, ly, s, eps, st);
         hipLaunchKernelGGL((k_fdm2_rows<double, T>), dim3(nr), dim3(256), lds, S, nx, ny, 1, (const double*)t0, z,
                            Vx, st);
+        PANO_HIP(hipGetLastError());
+        return PANO_OK;
+    }
+    if (DIM == 3 && m->fdm_split) {
+        const unsigned nf = (unsigned)(nz * ((nx + fdm_split_w(ny) - 1) / fdm_split_w(ny)));
+        const unsigned nzb = (unsigned)(ny * ((nx + fdm_split_w(nz) - 1) / fdm_split_w(nz)));
+        const unsigned ni = (unsigned)(nz * ((ny + fdm_split_w(nx) - 1) / fdm_split_w(nx)));
+        hipLaunchKernelGGL((k_fdm_fwd_cols<T>), dim3(nf), dim3(kFdmSplitThreads), fdm_split_lds_fwd(nx, ny), S,
+                           nx, ny, r, t0, Vx, Vy, st);
+        hipLaunchKernelGGL(k_fdm_z_cols, dim3(nzb), dim3(kFdmSplitThreads), fdm_split_lds_z(nz), S, nx, ny, nz, t0,
+                           Vz, lx, ly, lz, s, eps, st);
+        hipLaunchKernelGGL((k_fdm_inv_rows<T>), dim3(ni), dim3(kFdmSplitThreads), fdm_split_lds_inv(nx, ny), S,
+                           nx, ny, (const double*)t0, z, Vx, Vy, st);
         PANO_HIP(hipGetLastError());
         return PANO_OK;
     }

@@ -21,6 +21,7 @@ struct MG {
     double* fdm_lam[3] = {nullptr, nullptr, nullptr};   // eigenvalues
     int coarse_n[3] = {1, 1, 1};                         // coarsest extents x, y, z
     bool tb = true;  // 3D f32 levels: pre/post-smoothing as one pass each (PANO_MG_FUSED=0 turns it off)
+    bool fdm_split = false;  // 3D coarsest level: passes split over many workgroups (PANO_FDM_SPLIT=0: off)
 };
 
 // Device copy of the eigenbasis of the 1D operator along an axis of n cells

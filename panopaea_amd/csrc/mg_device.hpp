@@ -208,33 +208,59 @@ __device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T*
     return acc;
 }
 
+// q = A p for the x-pair run, with sum(p*q) over its cells in the order
+// (plane, x0 then x1). Each distinct value is loaded once: the pair's x0 and
+// x1 on planes z0-1 .. z0+NZ (the run's cells and their z neighbours), and per
+// run plane the x1 of pair m-1, the x0 of pair m+1 and both cells of rows y-1
+// and y+1 (the y neighbours of x0 / x1 are the x0 / x1 of those rows, as the
+// colours alternate). Missing neighbours load a clamped, valid cell and
+// poisson_from ignores them (existence bits).
 template <typename T, int DIM, int NZ>
 __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T dt, int m, int y, int z0) {
-    T nb[NZ][2][6], pc[NZ][2];
-    int idx[NZ][2], hs[NZ][2];
+    const int wh = D.w >> 1, hw = D.h * wh, n2 = D.d * hw;
+    const int ml = m > 0 ? m - 1 : m, mr = m + 1 < wh ? m + 1 : m;
+    const int yl = y > 0 ? y - 1 : y, yr = y + 1 < D.h ? y + 1 : y;
+    // index of x_c (c = 0: cell 2mm, 1: cell 2mm+1) of pair mm in row yy of plane zz
+    auto at = [&](int zz, int yy, int mm, int c) { return (((yy + zz) & 1) ^ c) * n2 + zz * hw + yy * wh + mm; };
+    T cv[NZ + 2][2], xl[NZ], xr[NZ], ylv[NZ][2], yrv[NZ][2];
+#pragma unroll
+    for (int kk = 0; kk < NZ + 2; ++kk) {
+        // plane z0 + kk - 1, clamped into the grid (a missing z neighbour's value is not used)
+        int zz = z0 + kk - 1;
+        zz = zz < 0 ? 0 : (zz >= D.d ? D.d - 1 : zz);
+        if (DIM == 3 || (kk >= 1 && kk <= NZ)) {
+            cv[kk][0] = ldx(p, at(zz, y, m, 0));
+            cv[kk][1] = ldx(p, at(zz, y, m, 1));
+        } else {
+            cv[kk][0] = cv[kk][1] = (T)0;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-        int z = zclamp(D, z0 + k);
-        RBRow R = rb_row<DIM>(D, z, y, m);
+        const int zz = zclamp(D, z0 + k);
+        xl[k] = ldx(p, at(zz, y, ml, 1));
+        xr[k] = ldx(p, at(zz, y, mr, 0));
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            idx[k][c] = c == 0 ? R.i0 : R.i1;
-            hs[k][c] = c == 0 ? R.has0 : R.has1;
-            rb_nbrs<T, DIM>(p, R, c, nb[k][c]);
-            pc[k][c] = ldx(p, idx[k][c]);
+            ylv[k][c] = ldx(p, at(zz, yl, m, c));
+            yrv[k][c] = ldx(p, at(zz, yr, m, c));
         }
     }
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-        if (z0 + k < D.d) {
+        const int z = z0 + k;
+        if (z >= D.d) continue;
+        const RBRow R = rb_row<DIM>(D, z, y, m);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const T* n = nb[k][c];
-                T v = poisson_from<T, DIM>(pc[k][c], n[0], n[1], n[2], n[3], n[4], n[5], hs[k][c], D.bc, dt);
-                stx(q, idx[k][c], v);
-                s += (double)pc[k][c] * (double)v;
-            }
+        for (int c = 0; c < 2; ++c) {
+            const T pc = cv[k + 1][c];
+            const T a = c == 0 ? xl[k] : cv[k + 1][0];  // x-1
+            const T b = c == 0 ? cv[k + 1][1] : xr[k];  // x+1
+            const T v = poisson_from<T, DIM>(pc, a, b, ylv[k][c], yrv[k][c], cv[k][c], cv[k + 2][c],
+                                             c == 0 ? R.has0 : R.has1, D.bc, dt);
+            stx(q, c == 0 ? R.i0 : R.i1, v);
+            s += (double)pc * (double)v;
         }
     }
     return s;
@@ -413,9 +439,29 @@ constexpr size_t kFdmLdsBytes = (2 * (size_t)kFdmPlane + kFdmVStage) * sizeof(do
 // Copies the n x n eigenbasis V to `dst` (LDS) when it fits beside `used`
 // doubles already staged there; returns the copy or V itself. The caller
 // synchronises before the first use.
+// dst[e] = src(e) for e < n: each thread issues up to 8 loads before it stores
+// any of them (a load-then-store loop would wait for one load at a time)
+template <typename Src>
+__device__ __forceinline__ void lds_fill(double* dst, int n, Src src) {
+    constexpr int U = 8;
+    for (int base = threadIdx.x; base < n; base += U * blockDim.x) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * blockDim.x;
+            v[u] = src(e < n ? e : n - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * blockDim.x;
+            if (e < n) dst[e] = v[u];
+        }
+    }
+}
+
 __device__ __forceinline__ const double* stage_basis(double* dst, int used, const double* V, int n) {
     if (used + n * n > kFdmVStage) return V;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) dst[used + e] = V[e];
+    lds_fill(dst + used, n * n, [&](int e) { return V[e]; });
     return dst + used;
 }
 // The transpose of V (W[q*n+i] = V[i*n+q]) into `dst` beside `used` doubles,
@@ -425,10 +471,10 @@ __device__ __forceinline__ const double* stage_basis(double* dst, int used, cons
 // entries, and the same products in the same order.
 __device__ __forceinline__ const double* stage_basis_t(double* dst, int used, const double* V, int n) {
     if (used + n * n > kFdmVStage) return nullptr;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    lds_fill(dst + used, n * n, [&](int e) {
         const int q = e / n, i = e - q * n;
-        dst[used + e] = V[i * n + q];
-    }
+        return V[i * n + q];
+    });
     return dst + used;
 }
 
@@ -493,14 +539,19 @@ __device__ __forceinline__ void lds_axis(const double* a, double* b, int nx, int
 //  mode 1: inverse y then x (input double, output Tout)
 //  mode 2: the whole 2D solve: forward x, y, divide by s*lambda, inverse y, x
 // src(e): the input value of plane element e, as a double.
-template <typename Tout, typename Src>
+// LOADS: src(e) is a plain load, so up to 8 per thread are issued before
+// any is stored (src computing a value from many loads goes one at a time).
+template <typename Tout, bool LOADS = false, typename Src>
 __device__ __forceinline__ void fdm_plane_body_src(double* lds, int nx, int ny, int mode, Src src, Tout* dst,
                                                    const double* Vx, const double* Vy, const double* lx,
                                                    const double* ly, double s, double eps) {
     double* a = lds;
     double* b = lds + nx * ny;
     int P = nx * ny;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = src(e);
+    if (LOADS)
+        lds_fill(a, P, src);
+    else
+        for (int e = threadIdx.x; e < P; e += blockDim.x) a[e] = src(e);
     double* vs = lds + 2 * kFdmPlane;
     const double* Vxg = Vx;
     Vx = stage_basis(vs, 0, Vx, nx);
@@ -541,7 +592,7 @@ template <typename Tin, typename Tout>
 __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int mode, const Tin* src, Tout* dst,
                                                const double* Vx, const double* Vy, const double* lx,
                                                const double* ly, double s, double eps) {
-    fdm_plane_body_src<Tout>(
+    fdm_plane_body_src<Tout, true>(
         lds, nx, ny, mode, [src](int e) { return (double)src[e]; }, dst, Vx, Vy, lx, ly, s, eps);
 }
 
@@ -563,7 +614,7 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     const int j0 = blk * R;
     const int nr = min(R, ny - j0);
     const int P = nr * nx;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) lds[e] = (double)src[(int64_t)j0 * nx + e];
+    lds_fill(lds, P, [&](int e) { return (double)src[(int64_t)j0 * nx + e]; });
     // inverse: the staged transpose W (W[k*nx+i] = Vx[i*nx+k]) when it fits
     const double* W = inverse ? stage_basis_t(lds + 2 * kFdmPlane, 0, Vx, nx) : nullptr;
     if (!W) Vx = stage_basis(lds + 2 * kFdmPlane, 0, Vx, nx);
@@ -591,9 +642,18 @@ __device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int 
     const int P = ny * nc;
     double* a = lds;
     double* b = lds + ny * C;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-        int j = e / nc, ci = e - j * nc;
-        a[j * C + ci] = t[(int64_t)j * nx + i0 + ci];
+    for (int base = threadIdx.x; base < P; base += 8 * blockDim.x) {  // up to 8 loads in flight per thread
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = imin(base + u * blockDim.x, P - 1), j = e / nc, ci = e - j * nc;
+            v[u] = t[(int64_t)j * nx + i0 + ci];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = base + u * blockDim.x, j = e / nc, ci = e - j * nc;
+            if (e < P) a[j * C + ci] = v[u];
+        }
     }
     Vy = stage_basis(lds + 2 * kFdmPlane, 0, Vy, ny);
     __syncthreads();
@@ -622,10 +682,10 @@ __device__ __forceinline__ void fdm_zcol_body(double* lds, int nx, int ny, int n
     double* b = lds + nx * nz;
     int P = nx * nz;
     int64_t zs = (int64_t)nx * ny;
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-        int k = e / nx, i = e - k * nx;
-        a[e] = data[k * zs + (int64_t)j * nx + i];
-    }
+    lds_fill(a, P, [&](int e) {
+        const int k = e / nx, i = e - k * nx;
+        return data[k * zs + (int64_t)j * nx + i];
+    });
     Vz = stage_basis(lds + 2 * kFdmPlane, 0, Vz, nz);
     __syncthreads();
     lds_axis(a, b, nx, nz, 1, 0, Vz);  // slice rows are z: the "y" axis of lds_axis

@@ -592,3 +592,46 @@ def test_gpu_pcg_after_an_overpredicting_hint(ctx):
     assert it2 == it_fresh
     assert np.array_equal(x_hinted, x_fresh)
 
+
+@pytest.mark.parametrize("dims,bc,dtype,levels", [
+    ((256, 256, 256), (0, 0, 0, 0, 0, 1), np.float32, 4),  # the full-size step's 32^3 coarsest level
+    ((48, 40, 64), (0,) * 6, np.float32, 4),               # 6 x 5 x 8 coarsest, closed (null space)
+    ((96, 64, 32), (1, 0, 0, 1, 0, 1), np.float64, 3),     # f64, mixed open sides, 24 x 16 x 8
+])
+def test_gpu_split_coarsest_solve_equals_plane_kernels(ctx, monkeypatch, dims, bc, dtype, levels):
+    """The 3D coarsest-level solve split over many workgroups (PANO_FDM_SPLIT,
+    default on) against the one-workgroup-per-plane passes (PANO_FDM_SPLIT=0,
+    which also restrict into the coarsest level inside their first pass): an
+    MG-PCG solve and, in f32, two smoke steps, bit for bit."""
+    from panopaea_amd import workloads as W
+    g = pa.Grid3d(dims, dtype, bc=bc)
+    b_host = o.neg_div(o.params(dims, bc=bc), pa.gen_uniform_faces(g, 3)).astype(dtype)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PANO_FDM_SPLIT", flag)
+        b = g.new_simplex_2().upload(b_host)
+        x = g.new_simplex_2()
+        scratch = [g.new_simplex_2() for _ in range(3)]
+        info = pa.precond_conjugate_gradient(pa.Multigrid(levels, 2), x, b, 300, 1e-6, *scratch,
+                                             pa.PoissonOperator(g, 0.02), tol_kind="rel")
+        res = [info.iterations, x.download()]
+        if dtype == np.float32:
+            wl = dict(W.SMOKE3D_256, dims=dims, bc=bc, precond="mg", mg_levels=levels)
+            gs = W.make_grid(wl)
+            cfg = W.step_config(wl)
+            r0, v0 = gs.new_simplex_2(), gs.new_simplex_1()
+            pa.gen_bandlimited_faces_device(v0, 5, wl["n_modes"], wl["k_max"])
+            r1, v1, p = gs.new_simplex_2(), gs.new_simplex_1(), gs.new_simplex_2()
+            rr, vv = gs.new_simplex_2(), gs.new_simplex_1()
+            i1 = pa.fluid_step(gs, cfg, r0, v0, r1, v1, p)
+            i2 = pa.fluid_step(gs, cfg, r1, v1, rr, vv, p)
+            res += [i1.iterations, i2.iterations, r1.download(), v1.download(), rr.download(), vv.download(),
+                    p.download()]
+        outs.append(res)
+    a, b2 = outs
+    assert a[0] > 0
+    for k, (u, w) in enumerate(zip(a, b2)):
+        if isinstance(u, np.ndarray):
+            assert np.array_equal(u, w), k
+        else:
+            assert u == w, k
