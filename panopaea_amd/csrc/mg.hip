@@ -123,12 +123,25 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
         int K = imin(imax(Kb + iz, 0), Dc.d - 1);
         vc[i] = ldx(zc, cidx<LC>(Dc, K, J, I));
     }
+    // the thread's fine values too, before the barrier (clamped to a valid cell
+    // for threads past the grid's edge, which store nothing)
+    const int m = m0 + threadIdx.x, y = y0 + threadIdx.y;
+    const bool mine = m < (Df.w >> 1) && y < Df.h;
+    float f0[NZ], f1[NZ];
+    int i0[NZ], i1[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        RBRow R = rb_row<3>(Df, zclamp(Df, z0 + k), mine ? y : 0, mine ? m : 0);
+        i0[k] = R.i0;
+        i1[k] = R.i1;
+        f0[k] = ldx(zf, R.i0);
+        f1[k] = ldx(zf, R.i1);
+    }
 #pragma unroll
     for (int i = 0; i < NP; ++i)
         if (tid + i * kTX * kTY < CZ * CY * CX) ct[tid + i * kTX * kTY] = vc[i];
     __syncthreads();
-    const int m = m0 + threadIdx.x, y = y0 + threadIdx.y;
-    if (m >= (Df.w >> 1) || y >= Df.h) return;
+    if (!mine) return;
     const float w34 = 0.75f, w14 = 0.25f, one = 1.0f, zero = 0.0f;
     int J0, J1;
     bool ey;
@@ -138,16 +151,6 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
     const int Il = el ? m : m - 1, Ir = er ? m : m + 1;
     const float wl0 = el ? one : w34, wl1 = el ? zero : w14;
     const float wr0 = er ? one : w34, wr1 = er ? zero : w14;
-    float f0[NZ], f1[NZ];
-    int i0[NZ], i1[NZ];
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        RBRow R = rb_row<3>(Df, zclamp(Df, z0 + k), y, m);
-        i0[k] = R.i0;
-        i1[k] = R.i1;
-        f0[k] = ldx(zf, R.i0);
-        f1[k] = ldx(zf, R.i1);
-    }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
         int z = z0 + k;

@@ -95,6 +95,21 @@ __device__ __forceinline__ float tb_opaque(float v) {
 }
 __host__ inline float tb_opaque(float v) { return v; }
 
+// Up to four floats as named scalars, read and written at indices that are
+// compile-time constants after unrolling: unlike an array, a select between
+// entries of two of them stays a select of values (an array entry selected
+// this way can become a load from a selected address, in scratch memory).
+struct Reg4 {
+    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
+    __host__ __device__ __forceinline__ float get(int i) const { return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3; }
+    __host__ __device__ __forceinline__ void set(int i, float x) {
+        if (i == 0) v0 = x;
+        else if (i == 1) v1 = x;
+        else if (i == 2) v2 = x;
+        else v3 = x;
+    }
+};
+
 // Per-thread state and phases of one workgroup. xin: the field before the
 // half-sweeps (unused with ZERO_X: the sweeps start from 0); xout: the result
 // (may equal xin only with ZERO_X: other workgroups still read xin's halo
@@ -119,6 +134,12 @@ struct TbThread {
     int act[KP];             // bit h + 4p: half-sweep h updates the cell when its x parity is p
     bool ind[KP], tile[KP];  // inside the grid; inside the tile
     float rin[KP][2], rbd[KP][2];  // 1/(s*nc) for x parity p: interior plane / one z side missing
+    // A cell updated at step t has x parity p = (CF + row parity + t) & 1 in
+    // every half-sweep of the step (half-sweep h updates colour CF+h of plane
+    // t-h), so ra / rb hold 1/(s*nc) of that cell (interior plane / one z side
+    // missing) for the whole step (set by begin_step).
+    static_assert(KP <= 4, "per-step reciprocals are held in a Reg4");
+    Reg4 ra, rb;
     float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
     bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
     double acc[2];
@@ -246,8 +267,21 @@ struct TbThread {
     __host__ __device__ __forceinline__ void begin_step(int t) {
         fill<mod(J + 3)>();
         load_x(t + 3);
+        step_recips(t);
     }
     // half-sweep H on plane t-H (t - t0 = J mod SLOTS)
+    // The step's reciprocals (ra, rb) from rin / rbd.
+    __host__ __device__ __forceinline__ void step_recips(int t) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int p = (CF + k / KC + t) & 1;
+            // selects of opaque values (not a load from a selected table entry)
+            const float i0 = tb_opaque(rin[k][0]), i1 = tb_opaque(rin[k][1]);
+            const float d0 = tb_opaque(rbd[k][0]), d1 = tb_opaque(rbd[k][1]);
+            ra.set(k, p ? i1 : i0);
+            rb.set(k, p ? d1 : d0);
+        }
+    }
     template <int H, int J>
     __host__ __device__ __forceinline__ void stage(int t) {
         const int z = t - H;
@@ -266,25 +300,25 @@ struct TbThread {
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             const int p = (c + k / KC + z) & 1;  // x parity of the colour-c cell (row parity k / KC)
-            if ((act[k] >> (H + 4 * p)) & 1) {
+            if (act[k] & (1 << (H + 4 * p))) {
                 const int e = e_[k];
                 const float rv = bq[H][c][k];
-                const float a0 = tb_opaque(zin ? rin[k][0] : rbd[k][0]);
-                const float a1 = tb_opaque(zin ? rin[k][1] : rbd[k][1]);
+                const float a = zin ? ra.get(k) : rb.get(k);  // 1/(s*nc) of this cell (step_recips)
                 float v;
                 if constexpr (ZERO_X && H == 0) {
                     // first half-sweep from z = 0: every neighbour still holds +0, whose
                     // sum is +0 (red_init_run's s*0)
-                    v = (rv + s * 0.0f) * (p ? a1 : a0);
+                    v = (rv + s * 0.0f) * a;
                 } else {
+                    const float* xs = xo + (p - 1);  // x-1 of the cell of parity p
                     float n[6];
-                    n[0] = xo[e - 1 + p];  // x-1
-                    n[1] = xo[e + p];      // x+1
-                    n[2] = xo[e - PW];     // y-1
-                    n[3] = xo[e + PW];     // y+1
-                    n[4] = xm[e];          // z-1
-                    n[5] = xp[e];          // z+1
-                    v = (rv + s * nsum<float, 3>(n)) * (p ? a1 : a0);  // gs_value, nc >= 1
+                    n[0] = xs[e];       // x-1
+                    n[1] = xs[e + 1];   // x+1
+                    n[2] = xo[e - PW];  // y-1
+                    n[3] = xo[e + PW];  // y+1
+                    n[4] = xm[e];       // z-1
+                    n[5] = xp[e];       // z+1
+                    v = (rv + s * nsum<float, 3>(n)) * a;  // gs_value, nc >= 1
                 }
                 xc[e] = v;
                 if (own && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
