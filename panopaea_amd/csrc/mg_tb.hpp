@@ -12,12 +12,18 @@
 // with the same arithmetic (gs_value of the same six neighbour values in the
 // same order), so every cell gets the bits of the unfused half-sweeps.
 //
-// LDS holds SLOTS planes of the tile + halo in the red-black compacted layout
-// of the field (plane slot, colour, row, pair); cells outside the grid hold 0,
-// which is the value a missing neighbour contributes to the sum. Plane z sits
-// in slot (z - t0 + 1) mod SLOTS; steps run in rounds of SLOTS, so every LDS
-// offset is a compile-time constant. b stays in registers: a ring of the NS
-// planes in flight, both colours.
+// LDS holds SLOTS = NS + 2 planes of the tile + halo in the red-black
+// compacted layout of the field (plane slot, colour, row, pair); cells outside
+// the grid hold 0, which is the value a missing neighbour contributes to the
+// sum. Plane z sits in slot (z - t0 + 1) mod SLOTS; steps run in rounds of
+// SLOTS, so every LDS offset is a compile-time constant. At step t the ring
+// holds planes t-NS .. t+1: plane t+1 is filled at the start of step t, into
+// the slot of plane t-NS-1. Half-sweep 0 of step t reads plane t+1 only as
+// the z+1 neighbour of its own positions, which the same thread has just
+// filled (a thread's z neighbours are always at its own positions), so no
+// barrier separates the fill from that read; other threads read the plane
+// in-plane from step t+1 on. b stays in registers: a ring of the NS planes in
+// flight, both colours.
 //
 // Thread positions are split by row parity (rows 2i+q, q = 0, 1), so the x
 // parity of the cell a half-sweep updates is the same across a position slot:
@@ -43,7 +49,7 @@ struct TbShape {
     static constexpr int PW = TXP + 2 * HP;          // pairs per LDS row
     static constexpr int PH = TY + 2 * NS;           // LDS rows
     static constexpr int PLANE = PW * PH;            // cells of one colour of one plane
-    static constexpr int SLOTS = NS + 3;             // resident planes t-NS .. t+2 at step t
+    static constexpr int SLOTS = NS + 2;             // resident planes t-NS .. t+1 at step t
     static constexpr int NT = NTHREADS;              // threads per workgroup
     static constexpr int CPOS = (PH / 2) * PW;       // positions of one row parity
     static constexpr int KC = (CPOS + NT - 1) / NT;  // position slots per thread and parity
@@ -239,14 +245,12 @@ struct TbThread {
     __host__ __device__ __forceinline__ int first_step() const { return t0; }
     __host__ __device__ __forceinline__ int last_step() const { return ze + NS - 2; }
 
-    // planes t0-1 .. t0+1 into slots 0..2; b of plane t0; prefetch x of t0+2, b of t0+1
+    // planes t0-1, t0 into slots 0, 1; b of plane t0; prefetch x of t0+1, b of t0+1
     __host__ __device__ __forceinline__ void prologue() {
         load_x(t0 - 1);
         fill<0>();
         load_x(t0);
         fill<1>();
-        load_x(t0 + 1);
-        fill<2>();
 #pragma unroll
         for (int j = 1; j < NS; ++j)
 #pragma unroll
@@ -258,15 +262,16 @@ struct TbThread {
             bq[0][0][k] = ok ? bq[0][0][k] : 0.0f;
             bq[0][1][k] = ok ? bq[0][1][k] : 0.0f;
         }
-        load_x(t0 + 2);
+        load_x(t0 + 1);
         bok = load_b(t0 + 1, bpf);
     }
-    // Step t with t - t0 = J (mod SLOTS): plane t+2 (prefetched) into its
-    // slot, which no phase of step t reads; prefetch plane t+3.
+    // Step t with t - t0 = J (mod SLOTS): plane t+1 (prefetched) into its
+    // slot, which only this thread's own reads of step t touch; prefetch plane
+    // t+2.
     template <int J>
     __host__ __device__ __forceinline__ void begin_step(int t) {
-        fill<mod(J + 3)>();
-        load_x(t + 3);
+        fill<mod(J + 2)>();
+        load_x(t + 2);
         step_recips(t);
     }
     // half-sweep H on plane t-H (t - t0 = J mod SLOTS)
