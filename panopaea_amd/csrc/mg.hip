@@ -179,17 +179,18 @@ __global__ __launch_bounds__(256) void k_prolong3_lds(Dims Df, Dims Dc, Grid3 G,
 // ---- 3D f32 levels: the pre- and post-smoothing as one pass each ----------------------
 // NS half-sweeps from xin into xout (TbThread); with DOT the workgroup's sums
 // of z.r over its tile for the last two half-sweeps go to part[L] and
-// part[nb + L].
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+// part[nb + L]. PL >= 0: from xin + P zco (zco: the coarse level, layout PL,
+// dims Dc) instead of xin.
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int PL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb(
     Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
-    const SolverState* st) {
+    const float* zco, Dims Dc, const SolverState* st) {
     if (st->done) return;
-    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
+    __shared__ float lds[TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int L = xcd_remap(blockIdx.x, g.nb);
     const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
     double acc[2];
-    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc);
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc, zco, Dc);
     if constexpr (DOT) {
         const double a0 = block_sum(acc[0]);
         const double a1 = block_sum(acc[1]);
@@ -222,23 +223,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 }
 
 // The four half-sweeps (colours CF, CF^1, ...) of a 3D f32 level in one pass;
-// with part, the z.r partials (2 per workgroup, *np set).
-template <int CF, bool ZERO_X, bool DOT>
+// with part, the z.r partials (2 per workgroup, *np set). PL >= 0 (3D only):
+// the sweeps start from xin + P zco, zco the coarse level (layout PL, dims Dc).
+template <int CF, bool ZERO_X, bool DOT, int PL = -1>
 static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, const float* xin, float* xout,
-                      const float* r, float s, const RInv<float>& ri, double* part, int* np, const SolverState* st) {
-    if (D.d <= 1) {
-        using C = Tb2D;
-        const TbGrid g = tb2_grid(D);
-        klaunch(c, name, bytes, k_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT),
-                D, g, xin, xout, r, s, ri, part, st);
-        if (np) *np = 2 * g.nb;
-        return;
+                      const float* r, float s, const RInv<float>& ri, double* part, int* np, const SolverState* st,
+                      const float* zco = nullptr, const Dims& Dc = Dims{}) {
+    if constexpr (PL < 0) {
+        if (D.d <= 1) {
+            using C = Tb2D;
+            const TbGrid g = tb2_grid(D);
+            klaunch(c, name, bytes, k_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb),
+                    dim3(C::NT), D, g, xin, xout, r, s, ri, part, st);
+            if (np) *np = 2 * g.nb;
+            return;
+        }
     }
     with_tb(D, [&](auto cfg) {
         using C = decltype(cfg);
         const TbGrid g = tb_grid_of<C>(D);
-        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb), dim3(C::NT), D, g, xin, xout, r,
-                s, ri, part, st);
+        klaunch(c, name, bytes, k_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT, PL>, dim3(g.nb),
+                dim3(C::NT), D, g, xin, xout, r, s, ri, part, zco, Dc, st);
         if (np) *np = 2 * g.nb;
         return 0;
     });
@@ -416,6 +421,13 @@ pano_status mg_check(const Dims& D, int levels) {
 
 static bool fdm_split_on(const Dims& C);
 
+// Level l (above the coarsest) of a 3D f32 hierarchy runs its prolongation
+// inside the one-pass post-smoothing.
+template <typename T>
+static bool pro_level(const MG<T>* m, int l) {
+    return m->pro && l + 1 < m->levels && tb_on(m->tb, m->nu, m->dims[l]);
+}
+
 template <typename T>
 pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>* m) {
     PANO_CHECK(mg_check(D, levels));
@@ -444,6 +456,18 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
             m->z[l] = (T*)pz;
             m->r[l] = (T*)pr;
         }
+    }
+    const char* pro = getenv("PANO_MG_PRO");
+    m->pro = std::is_same<T, float>::value && D.d > 1 && !(pro && pro[0] == '0');
+    for (int l = 0; l + 1 < levels; ++l) {
+        m->w[l] = nullptr;
+        if (!pro_level(m, l)) continue;
+        const Dims& Dl = m->dims[l];
+        char kw[32];
+        snprintf(kw, sizeof(kw), "mg_w%d", l);
+        void* pw;
+        PANO_CHECK(c->workspace(kw, (size_t)Dl.d * Dl.h * Dl.w * sizeof(T), &pw));
+        m->w[l] = (T*)pw;
     }
     void* pres;
     PANO_CHECK(c->workspace("mg_res", (size_t)D.d * D.h * D.w * sizeof(T), &pres));
@@ -897,10 +921,12 @@ static void level_down(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl
 }
 
 // One level of the up leg: prolongation from level l+1, post-smoothing; at
-// level 0 the last (black, red) pair also yields the partials of z.r.
+// level 0 the last (black, red) pair also yields the partials of z.r. zd: the
+// level's z as the down leg left it (zl[l], or m->w[l] when the prolongation
+// runs inside the post-smoothing).
 template <typename T, int DIM, int NZ>
-static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, const SolverState* st,
-                     double* dot_part, int* dot_np) {
+static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* zd, const T* const* rl,
+                     const SolverState* st, double* dot_part, int* dot_np) {
     const int Lc = m->levels - 1;
     const Dims& D = m->dims[l];
     const Dims& Dc = m->dims[l + 1];
@@ -912,6 +938,32 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* const* rl, 
     const double unit = (double)D.d * D.h * D.w * sizeof(T);
     const char* nh = l == 0 ? "smooth_l0" : "";
     const char* np = l == 0 ? "prolong_l0" : "";
+    // 3D f32 with the prolongation in the post-smoothing: the four half-sweeps
+    // from zd + P z_{l+1} into z in one pass (zd: the pre-smoothed z, a
+    // separate buffer, as other workgroups still read their halo from it while
+    // this one stores). Timed bytes: the prolongation pass (z and z_c in, z
+    // out) and two red-black sweeps of 3 field passes each; the pass moves
+    // about 3.
+    if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+        if (zd != zl[l]) {
+            const char* npost = l == 0 ? "smooth_post_l0" : "";
+            const double bytes = (8 + cfrac) * unit;
+            const bool dot = l == 0 && dot_part;
+            auto post = [&](auto lc) {
+                constexpr int LC = decltype(lc)::value;
+                if (dot)
+                    smooth_tb<1, false, true, LC>(c, npost, bytes, D, (const float*)zd, zl[l], rl[l], s, ri,
+                                                  dot_part, dot_np, st, (const float*)zl[l + 1], Dc);
+                else
+                    smooth_tb<1, false, false, LC>(c, npost, bytes, D, (const float*)zd, zl[l], rl[l], s, ri,
+                                                   (double*)nullptr, (int*)nullptr, st, (const float*)zl[l + 1],
+                                                   Dc);
+            };
+            if (l + 1 == Lc) post(std::integral_constant<int, NATURAL>{});
+            else post(std::integral_constant<int, REDBLACK>{});
+            return;
+        }
+    }
     // f32, nu = 2: prolongation into the residual scratch (free on the up
     // leg), then the four post-smoothing half-sweeps from there back into z in
     // one pass (out of place: workgroups read their halo from the input while
@@ -968,19 +1020,23 @@ static pano_status vcycle_dim(Ctx* c, MG<T>* m, T* z0, const T* r0, const Solver
         zl[l] = m->z[l];
         rl[l] = m->r[l];
     }
+    // the down leg's z: the pre-smoothing's output, read by the residual and
+    // (with the prolongation in the post-smoothing) by the post-smoothing
+    T* zd[16];
+    for (int l = 0; l <= Lc; ++l) zd[l] = l < Lc && m->w[l] ? m->w[l] : zl[l];
     for (int l = 0; l < Lc; ++l) {
         switch (run_len(m->dims[l])) {
-            case 4: level_down<T, DIM, 4>(c, m, l, zl, rl, st); break;
-            case 2: level_down<T, DIM, 2>(c, m, l, zl, rl, st); break;
-            default: level_down<T, DIM, 1>(c, m, l, zl, rl, st); break;
+            case 4: level_down<T, DIM, 4>(c, m, l, zd, rl, st); break;
+            case 2: level_down<T, DIM, 2>(c, m, l, zd, rl, st); break;
+            default: level_down<T, DIM, 1>(c, m, l, zd, rl, st); break;
         }
     }
     PANO_CHECK(coarse_solve<T, DIM>(c, m, m->z[Lc], coarse_rc<T, DIM>(m) ? (const T*)m->res : m->r[Lc], st));
     for (int l = Lc - 1; l >= 0; --l) {
         switch (run_len(m->dims[l])) {
-            case 4: level_up<T, DIM, 4>(c, m, l, zl, rl, st, dot_part, dot_np); break;
-            case 2: level_up<T, DIM, 2>(c, m, l, zl, rl, st, dot_part, dot_np); break;
-            default: level_up<T, DIM, 1>(c, m, l, zl, rl, st, dot_part, dot_np); break;
+            case 4: level_up<T, DIM, 4>(c, m, l, zl, zd[l], rl, st, dot_part, dot_np); break;
+            case 2: level_up<T, DIM, 2>(c, m, l, zl, zd[l], rl, st, dot_part, dot_np); break;
+            default: level_up<T, DIM, 1>(c, m, l, zl, zd[l], rl, st, dot_part, dot_np); break;
         }
     }
     PANO_HIP(hipGetLastError());

@@ -32,8 +32,8 @@
 //
 // Every phase touches only the thread's own positions plus LDS values written
 // in an earlier phase; the kernel puts a barrier after each half-sweep:
-// prologue | begin + half-sweep 0 | 1 | ... | NS-1 | end + begin of the next
-// step + half-sweep 0 | ...
+// [coarse prologue |] prologue | begin + half-sweep 0 | 1 | ... | NS-1 | end +
+// begin of the next step + half-sweep 0 | ...
 #pragma once
 
 #include <utility>
@@ -55,6 +55,11 @@ struct TbShape {
     static constexpr int KC = (CPOS + NT - 1) / NT;  // position slots per thread and parity
     static constexpr int KP = 2 * KC;
     static constexpr int LDS_FLOATS = SLOTS * 2 * PLANE;
+    // With the prolongation in the fill: the coarse cells under the tile +
+    // halo and one more on each side, CW x CH of one coarse plane (the coarse
+    // cell of pair m, row y is (m, y >> 1)).
+    static constexpr int CW = PW + 2, CH = PH / 2 + 2, CPL = CW * CH;
+    static constexpr int CKN = (CPL + NT - 1) / NT;  // coarse cells per thread
     // Position f of a row parity -> (row pair rp, pair pr). The first
     // (PH/2)*TXP positions are the tile's own pairs, TXP consecutive ones per
     // row pair, so with TXP = 32 every half-wave reads and writes 32
@@ -100,6 +105,13 @@ __device__ __forceinline__ float tb_opaque(float v) {
     return v;
 }
 __host__ inline float tb_opaque(float v) { return v; }
+// An int the optimiser may not treat as loop-invariant: what is derived from
+// it is recomputed where it is used instead of being held in registers.
+__device__ __forceinline__ int tb_opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+__host__ inline int tb_opaque(int v) { return v; }
 
 // Up to four floats as named scalars, read and written at indices that are
 // compile-time constants after unrolling: unlike an array, a select between
@@ -122,11 +134,34 @@ struct Reg4 {
 // while this one stores). r: right-hand side. With DOT, acc[0] and acc[1]
 // collect sum(v*r) over the tile's cells of the last two half-sweeps.
 // Needs D.d >= 2 (every cell then has a z neighbour, so nc >= 1).
-template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT>
+//
+// PL >= 0: the half-sweeps start from xin + P zco instead of xin, P the
+// trilinear prolongation from the coarse level zco (layout PL, dims Dc, set
+// by init_pro), with the arithmetic of prolong_run / k_prolong3_lds, so the
+// result has the bits of the prolongation kernel followed by the smoother.
+// The prolongation of a fine cell (z, y, x = 2m + side) is
+//   wz0 * Q_side(K0) + wz1 * Q_side(K1),
+//   Q_side(K) = wy0 * px_side(K, J0) + wy1 * px_side(K, J1),
+//   px_side(K, J) = w0 * c(K, J, m) + w1 * c(K, J, m -/+ 1),
+// with (K0, K1) = (z >> 1, the other coarse plane next to z) and the edge
+// weights of p1. Q depends on the fine plane only through K, so the thread
+// keeps Q of the two coarse planes M, M+1 that fine plane z needs (M =
+// floor((z - 1) / 2)) and computes Q of a new coarse plane every second step,
+// from a copy of that plane's CW x CH region in LDS. The region is loaded
+// with clamped coarse coordinates: at a grid edge the neighbour the weights
+// ignore (weight 0) then holds the cell's own value, as in the kernel.
+template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT, int PL = -1>
 struct TbThread {
     using S = TbShape<NS, TXP, TY, NTHREADS>;
     static constexpr int PW = S::PW, PH = S::PH, PLANE = S::PLANE, SLOTS = S::SLOTS, NT = S::NT, KC = S::KC,
                          KP = S::KP, HP = S::HP, CPOS = S::CPOS;
+    static constexpr bool PRO = PL >= 0;
+    static_assert(!(PRO && ZERO_X), "the prolongation adds to a given field");
+    static_assert(!PRO || 3 * S::CPL <= 2 * PLANE, "three coarse planes fit one plane slot");
+    static constexpr int CW = S::CW, CPL = S::CPL, CKN = S::CKN;
+    // x ring, then (PRO) one coarse plane region, written at the start of odd
+    // steps and read after their first barrier
+    static constexpr int LDS_FLOATS = S::LDS_FLOATS + (PRO ? CPL : 0);
 
     float* lds;
     Dims D;
@@ -137,7 +172,10 @@ struct TbThread {
     int tid, zs, ze, t0, hw, n2;
     int e_[KP];              // LDS index of the position (row * PW + pair)
     unsigned goff[KP];       // in-plane element offset y*wh + m (0 outside the grid)
-    int act[KP];             // bit h + 4p: half-sweep h updates the cell when its x parity is p
+    // bits 0..7: h + 4p, half-sweep h updates the cell when its x parity is p;
+    // PRO: bits 8..10 the edge flags (y, x-, x+) of its prolongation, bits
+    // 11..22 the LDS offset of its coarse cell in a region
+    int act[KP];
     bool ind[KP], tile[KP];  // inside the grid; inside the tile
     float rin[KP][2], rbd[KP][2];  // 1/(s*nc) for x parity p: interior plane / one z side missing
     // A cell updated at step t has x parity p = (CF + row parity + t) & 1 in
@@ -149,6 +187,14 @@ struct TbThread {
     float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
     bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
     double acc[2];
+    // PRO: the coarse level and its geometry (plane size chw; colour offset cn2)
+    const float* zco;
+    Dims Dc;
+    int chw, cn2, Ib, Jb;  // plane size, colour offset, region origin (coarse x, y)
+    float cpf[CKN];        // the thread's region cells of the next coarse plane to store
+    // Q of coarse planes K0, K1 of the next fine plane to fill, for the cells
+    // x = 2m, 2m+1 (K0 = z >> 1, K1 the other coarse plane next to z)
+    float q0[2][KP], q1[2][KP];
 
     static constexpr int mod(int a) { return ((a % SLOTS) + SLOTS) % SLOTS; }
 
@@ -199,6 +245,84 @@ struct TbThread {
             act[k] = a;
         }
     }
+    // PRO: the coarse level (after init).
+    __host__ __device__ __forceinline__ void init_pro(const float* zco_, const Dims& Dc_, int bx, int by) {
+        zco = zco_;
+        Dc = Dc_;
+        chw = PL == REDBLACK ? Dc.h * (Dc.w >> 1) : Dc.h * Dc.w;
+        cn2 = Dc.d * chw;
+        const int m0 = bx * TXP, y0 = by * TY;
+        Ib = m0 - HP - 1;
+        Jb = ((y0 - NS) >> 1) - 1;  // y0 - NS is even
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int q = k / KC, f = tid + (k % KC) * NT;
+            int rp, pr;
+            S::pos_of(f < CPOS ? f : 0, rp, pr);
+            const int y = y0 - NS + 2 * rp + q, m = m0 - HP + pr;
+            const int I = y >> 1, J = (y & 1) ? I + 1 : I - 1;  // p1 along y
+            const int ey = J < 0 || J >= Dc.h, el = m == 0, er = m + 1 >= Dc.w;
+            act[k] |= ((ey | (el << 1) | (er << 2)) << 8) | (((rp + 1) * CW + pr + 1) << 11);
+        }
+    }
+    // the thread's region cells of coarse plane K (clamped to the level)
+    __host__ __device__ __forceinline__ void load_c(int K, float (&v)[CKN]) const {
+        K = K < 0 ? 0 : (K >= Dc.d ? Dc.d - 1 : K);
+        const int tv = tb_opaque(tid);  // offsets recomputed per load (no registers held)
+#pragma unroll
+        for (int i = 0; i < CKN; ++i) {
+            const int e = tv + i * NT < CPL ? tv + i * NT : 0;
+            int J = Jb + e / CW, I = Ib + e % CW;
+            J = J < 0 ? 0 : (J >= Dc.h ? Dc.h - 1 : J);
+            I = I < 0 ? 0 : (I >= Dc.w ? Dc.w - 1 : I);
+            if constexpr (PL == REDBLACK)
+                v[i] = zco[((I + J + K) & 1) * cn2 + K * chw + J * (Dc.w >> 1) + (I >> 1)];
+            else
+                v[i] = zco[K * chw + J * Dc.w + I];
+        }
+    }
+    __host__ __device__ __forceinline__ void store_c(float* dst, const float (&v)[CKN]) const {
+#pragma unroll
+        for (int i = 0; i < CKN; ++i)
+            if (tid + i * NT < CPL) dst[tid + i * NT] = v[i];
+    }
+    // Q of the coarse plane whose region is at C, both cells of every position.
+    // Positions k (row 2rp) and k + KC (row 2rp + 1) share pair and row pair,
+    // so their coarse row J0 = y >> 1 (x-interpolated once) and x weights;
+    // J1 is the row above (even y) / below (odd y).
+    __host__ __device__ __forceinline__ void q_of(const float* C, float (&qv)[2][KP]) const {
+        const float w34 = 0.75f, w14 = 0.25f, one = 1.0f, zero = 0.0f;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int ae = tb_opaque(act[k]), ao = tb_opaque(act[k + KC]);  // weights recomputed per call
+            const int fe = (ae >> 8) & 7, fo = (ao >> 8) & 7;
+            const float* r0 = C + (ae >> 11);
+            const float wl0 = (fe & 2) ? one : w34, wl1 = (fe & 2) ? zero : w14;
+            const float wr0 = (fe & 4) ? one : w34, wr1 = (fe & 4) ? zero : w14;
+            const float am = r0[-CW - 1], bm = r0[-CW], cm = r0[-CW + 1];  // row J0 - 1
+            const float a0 = r0[-1], b0 = r0[0], c0 = r0[1];                // row J0
+            const float ap = r0[CW - 1], bp = r0[CW], cp = r0[CW + 1];      // row J0 + 1
+            const float pl0 = wl0 * b0 + wl1 * a0, pr0 = wr0 * b0 + wr1 * c0;
+            const float plm = wl0 * bm + wl1 * am, prm = wr0 * bm + wr1 * cm;
+            const float plp = wl0 * bp + wl1 * ap, prp = wr0 * bp + wr1 * cp;
+            const float ye0 = (fe & 1) ? one : w34, ye1 = (fe & 1) ? zero : w14;
+            const float yo0 = (fo & 1) ? one : w34, yo1 = (fo & 1) ? zero : w14;
+            qv[0][k] = ye0 * pl0 + ye1 * plm;
+            qv[1][k] = ye0 * pr0 + ye1 * prm;
+            qv[0][k + KC] = yo0 * pl0 + yo1 * plp;
+            qv[1][k + KC] = yo0 * pr0 + yo1 * prp;
+        }
+    }
+    __host__ __device__ __forceinline__ void q_swap() {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const float a = q0[0][k], b = q0[1][k];
+            q0[0][k] = q1[0][k];
+            q0[1][k] = q1[1][k];
+            q1[0][k] = a;
+            q1[1][k] = b;
+        }
+    }
 
     // x of plane z into xpf, b of plane z into v. The loads are unconditional
     // (positions outside the grid and unused planes read a valid clamped
@@ -229,28 +353,83 @@ struct TbThread {
         }
         return z >= 0 && z < D.d && z >= zs - NS + 1 && z < ze + NS - 1;
     }
-    // xpf into slot SL
+    // xpf (plane z) into slot SL; PRO: plus its prolongation, from Q of its
+    // coarse planes K0, K1 in q0, q1
     template <int SL>
-    __host__ __device__ __forceinline__ void fill() {
+    __host__ __device__ __forceinline__ void fill(int z) {
         float* base = lds + SL * 2 * PLANE;
+        float wz0 = 0.0f, wz1 = 0.0f;
+        if constexpr (PRO) {
+            const int I = z >> 1, J = (z & 1) ? I + 1 : I - 1;  // p1 along z
+            const bool ez = J < 0 || J >= Dc.d;
+            wz0 = ez ? 1.0f : 0.75f;
+            wz1 = ez ? 0.0f : 0.25f;
+        }
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             if ((k % KC) * NT + tid < CPOS) {
                 const bool ok = xok && ind[k];
-                base[e_[k]] = ok ? xpf[0][k] : 0.0f;
-                base[PLANE + e_[k]] = ok ? xpf[1][k] : 0.0f;
+                float v0 = xpf[0][k], v1 = xpf[1][k];
+                if constexpr (PRO) {
+                    const float pl = wz0 * q0[0][k] + wz1 * q1[0][k];  // cell x = 2m
+                    const float pr = wz0 * q0[1][k] + wz1 * q1[1][k];  // cell x = 2m + 1
+                    const bool c0 = ((k / KC + z) & 1) != 0;  // colour of x = 2m (row parity k / KC)
+                    v0 = v0 + (c0 ? pr : pl);
+                    v1 = v1 + (c0 ? pl : pr);
+                }
+                base[e_[k]] = ok ? v0 : 0.0f;
+                base[PLANE + e_[k]] = ok ? v1 : 0.0f;
             }
         }
     }
     __host__ __device__ __forceinline__ int first_step() const { return t0; }
     __host__ __device__ __forceinline__ int last_step() const { return ze + NS - 2; }
 
+    // PRO, before prologue (a barrier between): the regions of coarse planes
+    // A .. A+2 (A = floor((t0 - 2) / 2)) into the last x slot, which no step
+    // fills before step t0 + SLOTS - 3 (reads of it from the prologue end
+    // before that step's barriers); A+3 into cpf (stored by the first odd
+    // step); x of plane t0-1 into xpf.
+    __host__ __device__ __forceinline__ void prologue_coarse() {
+        const int A = (t0 - 2) >> 1;
+        float c0[CKN], c1[CKN], c2[CKN];
+        load_c(A, c0);
+        load_c(A + 1, c1);
+        load_c(A + 2, c2);
+        load_c(A + 3, cpf);
+        load_x(t0 - 1);
+        float* sc = lds + (SLOTS - 1) * 2 * PLANE;
+        store_c(sc, c0);
+        store_c(sc + CPL, c1);
+        store_c(sc + 2 * CPL, c2);
+    }
     // planes t0-1, t0 into slots 0, 1; b of plane t0; prefetch x of t0+1, b of t0+1
     __host__ __device__ __forceinline__ void prologue() {
-        load_x(t0 - 1);
-        fill<0>();
+        if constexpr (PRO) {
+            // A = t0/2 - 1 (t0 even) or (t0 - 3)/2 (t0 odd); (K0, K1) of plane z
+            // is (z >> 1, z >> 1 + 1) for odd z, (z >> 1, z >> 1 - 1) for even z
+            const float* sc = lds + (SLOTS - 1) * 2 * PLANE;
+            if (t0 & 1) {
+                q_of(sc + CPL, q0);  // plane t0 - 1: (A+1, A)
+                q_of(sc, q1);
+                fill<0>(t0 - 1);
+                q_of(sc + 2 * CPL, q1);  // plane t0: (A+1, A+2)
+            } else {
+                q_of(sc, q0);  // plane t0 - 1: (A, A+1)
+                q_of(sc + CPL, q1);
+                fill<0>(t0 - 1);
+                q_swap();  // plane t0: (A+1, A)
+            }
+        } else {
+            load_x(t0 - 1);
+            fill<0>(t0 - 1);
+        }
         load_x(t0);
-        fill<1>();
+        fill<1>(t0);
+        if constexpr (PRO) {
+            // t0 even: step t0 fills plane t0 + 1 with (A+1, A+2)
+            if (!(t0 & 1)) q_of(lds + (SLOTS - 1) * 2 * PLANE + 2 * CPL, q1);
+        }
 #pragma unroll
         for (int j = 1; j < NS; ++j)
 #pragma unroll
@@ -267,10 +446,22 @@ struct TbThread {
     }
     // Step t with t - t0 = J (mod SLOTS): plane t+1 (prefetched) into its
     // slot, which only this thread's own reads of step t touch; prefetch plane
-    // t+2.
+    // t+2. PRO: for odd t plane t+1 has (K0, K1) = ((t+1)/2, (t-1)/2), the two
+    // of the step before swapped; the step stores coarse plane (t+3)/2 into
+    // the coarse slot (its last reads, at step t-2, are barriers behind),
+    // prefetches plane (t+5)/2 and, after its first barrier (q_next), computes
+    // Q((t+3)/2) into q1, which the fill at step t dropped: the next (even)
+    // step fills plane t+2 with ((t+1)/2, (t+3)/2).
     template <int J>
     __host__ __device__ __forceinline__ void begin_step(int t) {
-        fill<mod(J + 2)>();
+        if constexpr (PRO) {
+            if (t & 1) {
+                store_c(lds + S::LDS_FLOATS, cpf);
+                load_c((t + 5) >> 1, cpf);
+                q_swap();
+            }
+        }
+        fill<mod(J + 2)>(t + 1);
         load_x(t + 2);
         step_recips(t);
     }
@@ -289,6 +480,9 @@ struct TbThread {
     }
     template <int H, int J>
     __host__ __device__ __forceinline__ void stage(int t) {
+        if constexpr (PRO && H == 1) {  // q_next: after the barrier behind the coarse slot store
+            if (t & 1) q_of(lds + S::LDS_FLOATS, q1);
+        }
         const int z = t - H;
         const int lo = zs - (NS - 1 - H) > 0 ? zs - (NS - 1 - H) : 0;
         const int hi = ze + (NS - 1 - H) < D.d ? ze + (NS - 1 - H) : D.d;
@@ -562,13 +756,21 @@ __device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_se
 // One workgroup of the one-pass smoother (four half-sweeps) on tile
 // (bx, by, bz) of a level; acc: its tile's z.r sums of the last two
 // half-sweeps (with DOT).
-template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT>
+// PL >= 0: the sweeps start from xin + P zco (zco: the coarse level, layout
+// PL, dims Dc).
+template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT, int PL = -1>
 __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, int by, int bz, int zc,
                                              const float* xin, float* xout, const float* r, float s,
-                                             const float* rinv, double* acc) {
-    using TH = TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT>;
+                                             const float* rinv, double* acc, const float* zco = nullptr,
+                                             const Dims& Dc = Dims{}) {
+    using TH = TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>;
     TH th;
     th.init(lds, D, bx, by, bz, zc, xin, xout, r, s, rinv, threadIdx.x);
+    if constexpr (TH::PRO) {
+        th.init_pro(zco, Dc, bx, by);
+        th.prologue_coarse();
+        __syncthreads();
+    }
     th.prologue();
     __syncthreads();
     const int t1 = th.last_step();

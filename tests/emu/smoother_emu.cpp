@@ -46,6 +46,49 @@ static void ref_sweeps(const Dims& D, int NS, int CF, bool zero_x, std::vector<f
     }
 }
 
+// Host restatement of the trilinear prolongation x += P zc (k_prolong3_lds /
+// prolong_run): per fine cell, x then y then z weights, 3/4 and 1/4 with the
+// far weight moved onto the near cell at a grid edge.
+static void hp1(int i, int n, int& I0, int& I1, bool& e) {
+    int I = i >> 1, J = (i & 1) ? I + 1 : I - 1;
+    I0 = I;
+    e = J < 0 || J >= n;
+    I1 = e ? I : J;
+}
+static int cix(int LC, const Dims& Dc, int K, int J, int I) {
+    if (LC == REDBLACK) {
+        int wh = Dc.w >> 1, hw = Dc.h * wh;
+        return ((I + J + K) & 1) * (Dc.d * hw) + K * hw + J * wh + (I >> 1);
+    }
+    return (K * Dc.h + J) * Dc.w + I;
+}
+static void ref_prolong(const Dims& D, const Dims& Dc, int LC, std::vector<float>& x, const std::vector<float>& zc) {
+    const float w34 = 0.75f, w14 = 0.25f;
+    for (int z = 0; z < D.d; ++z)
+        for (int y = 0; y < D.h; ++y)
+            for (int xx = 0; xx < D.w; ++xx) {
+                int K0, K1, J0, J1;
+                bool ez, ey;
+                hp1(z, Dc.d, K0, K1, ez);
+                hp1(y, Dc.h, J0, J1, ey);
+                const int m = xx >> 1, side = xx & 1;
+                const bool e = side ? m + 1 >= Dc.w : m == 0;
+                const int Is = e ? m : (side ? m + 1 : m - 1);
+                const float w0 = e ? 1.0f : w34, w1 = e ? 0.0f : w14;
+                const float wy0 = ey ? 1.0f : w34, wy1 = ey ? 0.0f : w14;
+                const float wz0 = ez ? 1.0f : w34, wz1 = ez ? 0.0f : w14;
+                float px[2][2];
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        const int K = a ? K1 : K0, J = b ? J1 : J0;
+                        px[a][b] = w0 * zc[cix(LC, Dc, K, J, m)] + w1 * zc[cix(LC, Dc, K, J, Is)];
+                    }
+                const float v = wz0 * (wy0 * px[0][0] + wy1 * px[0][1]) + wz1 * (wy0 * px[1][0] + wy1 * px[1][1]);
+                const int i = rbi(D, z, y, xx);
+                x[i] = x[i] + v;
+            }
+}
+
 template <int NS, int J, int H = 0, typename V>
 static void stages(V& th, int t) {
     if constexpr (H < NS) {
@@ -65,9 +108,11 @@ static void round_(V& th, int& t, int t1) {
     }
 }
 
-template <int NS, int TXP, int TY, int CF, bool ZX, bool DOT, int NT = 512>
+// PL >= 0: the sweeps start from xin + P zc (coarse level of half the
+// extents, layout PL)
+template <int NS, int TXP, int TY, int CF, bool ZX, bool DOT, int NT = 512, int PL = -1>
 static int run(const Dims& D, int want, int zmin, unsigned seed) {
-    using TH = TbThread<NS, TXP, TY, NT, CF, ZX, DOT>;
+    using TH = TbThread<NS, TXP, TY, NT, CF, ZX, DOT, PL>;
     const int n = D.d * D.h * D.w;
     std::mt19937 rng(seed);
     std::uniform_real_distribution<float> U(-1, 1);
@@ -77,6 +122,13 @@ static int run(const Dims& D, int want, int zmin, unsigned seed) {
     float s = 0.37f;
     RInv<float> ri = make_rinv<float>(s);
     xref = xin;
+    const Dims Dc{D.d / 2, D.h / 2, D.w / 2, D.bc};
+    std::vector<float> zc;
+    if (PL >= 0) {
+        zc.resize((size_t)Dc.d * Dc.h * Dc.w);
+        for (auto& v : zc) v = U(rng);
+        ref_prolong(D, Dc, PL, xref, zc);
+    }
     double dref[2] = {0, 0};
     ref_sweeps(D, NS, CF, ZX, xref, r, s, ri, dref);
     TbGrid g = tb_grid<TXP, TY>(D, want, zmin);
@@ -84,10 +136,14 @@ static int run(const Dims& D, int want, int zmin, unsigned seed) {
     for (int L = 0; L < g.nb; ++L) {
         int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
         if (bz * g.zc >= D.d) { printf("empty chunk\n"); return 1; }
-        std::vector<float> lds(TH::S::LDS_FLOATS, -777.0f);
+        std::vector<float> lds(TH::LDS_FLOATS, -777.0f);
         std::vector<TH> th(TH::S::NT);
-        for (int i = 0; i < TH::S::NT; ++i)
+        for (int i = 0; i < TH::S::NT; ++i) {
             th[i].init(lds.data(), D, bx, by, bz, g.zc, xin.data(), xout.data(), r.data(), s, ri.v, i);
+            if constexpr (TH::PRO) th[i].init_pro(zc.data(), Dc, bx, by);
+        }
+        if constexpr (TH::PRO)
+            for (auto& t : th) t.prologue_coarse();
         for (auto& t : th) t.prologue();
         int t = th[0].first_step(), t1 = th[0].last_step();
         while (t <= t1) round_<NS, TH::SLOTS>(th, t, t1);
@@ -101,8 +157,9 @@ static int run(const Dims& D, int want, int zmin, unsigned seed) {
         }
     double e0 = DOT ? fabs(dsum[0] - dref[0]) / (fabs(dref[0]) + 1e-30) : 0;
     double e1 = DOT ? fabs(dsum[1] - dref[1]) / (fabs(dref[1]) + 1e-30) : 0;
-    printf("NS=%d CF=%d ZX=%d DOT=%d dims %dx%dx%d bc=%d grid gx=%d gy=%d zc=%d nb=%d: %d mismatches, dot rel err %.2e %.2e\n",
-           NS, CF, ZX, DOT, D.d, D.h, D.w, D.bc, g.gx, g.gy, g.zc, g.nb, bad, e0, e1);
+    printf("NS=%d CF=%d ZX=%d DOT=%d PL=%d tile %dx%d dims %dx%dx%d bc=%d grid gx=%d gy=%d zc=%d nb=%d: %d mismatches, "
+           "dot rel err %.2e %.2e\n",
+           NS, CF, ZX, DOT, PL, TXP, TY, D.d, D.h, D.w, D.bc, g.gx, g.gy, g.zc, g.nb, bad, e0, e1);
     return bad || e0 > 1e-12 || e1 > 1e-12;
 }
 
@@ -189,6 +246,20 @@ int main() {
             fails += run<4, 32, 8, 1, false, false, 320>(D, 2048, zmin, seed++);
             // a small generic tile (several workgroups on every shape)
             fails += run<4, 8, 4, 1, false, true, 64>(D, 64, zmin, seed++);
+        }
+    }
+    // the prolongation in the post-smoothing: even extents (a coarse level of
+    // half of each), widths divisible by 4 for a red-black coarse level
+    const Dims cp[] = {{16, 16, 16, 0}, {10, 6, 8, 48}, {20, 36, 72, 0}, {2, 40, 136, 32},
+                       {32, 32, 128, 63}, {40, 18, 68, 2}, {6, 70, 12, 12}};
+    for (auto D : cp) {
+        for (int zmin : {8, 2, 1}) {
+            fails += run<4, 32, 32, 1, false, true, 768, NATURAL>(D, 512, zmin, seed++);
+            fails += run<4, 32, 32, 1, false, false, 768, REDBLACK>(D, 1024, zmin, seed++);
+            fails += run<4, 32, 8, 1, false, true, 320, REDBLACK>(D, 2048, zmin, seed++);
+            fails += run<4, 32, 8, 1, false, false, 320, NATURAL>(D, 2048, zmin, seed++);
+            // two positions per thread and row parity
+            fails += run<4, 16, 16, 1, false, true, 128, REDBLACK>(D, 64, zmin, seed++);
         }
     }
     const Dims c2[] = {{1, 64, 64, 0}, {1, 40, 70, 2}, {1, 100, 200, 15}, {1, 8, 6, 4}, {1, 33, 130, 9}};

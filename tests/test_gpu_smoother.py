@@ -70,3 +70,34 @@ def test_gpu_fused_smoothing_pcg(ctx, dims, bc, levels, monkeypatch):
     assert abs(inf_f.iterations - inf_u.iterations) <= 1
     scale = np.abs(xu).max()
     assert np.abs(xf - xu).max() <= 1e-4 * scale
+
+
+@pytest.mark.parametrize("dims,bc,levels", [c for c in CASES if len(c[0]) == 3])
+def test_gpu_prolongation_in_post_smoothing_bitwise(ctx, dims, bc, levels, monkeypatch):
+    """The prolongation inside the one-pass post-smoothing (default) against the
+    prolongation kernel followed by the one-pass post-smoothing (PANO_MG_PRO=0):
+    the same V-cycle, and the same MG-PCG solve, bit for bit (the post-smoothing's
+    z.r partials add the same terms per workgroup in the same order)."""
+    print("prolongation in post-smoothing", dims, bc, levels, flush=True)
+    monkeypatch.delenv("PANO_MG_FUSED", raising=False)
+    g = pa.Grid3d(dims, np.float32, bc=bc)
+    r = np.random.default_rng(dims[1]).standard_normal(g.num_elem_2()).astype(np.float32)
+    if not any(bc):
+        r -= r.mean(dtype=np.float64).astype(np.float32)
+    faces = pa.gen_bandlimited_faces(g, 7, 16, 8.0)
+    b = g.new_simplex_2()
+    pa.neg_divergence(b, g.new_simplex_1().upload(faces))
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("PANO_MG_PRO", flag)
+        z = g.new_simplex_2()
+        pa.precond_apply(pa.Multigrid(levels, 2), z, g.new_simplex_2().upload(r), dt=0.4)
+        x, rr, a, s = (g.new_simplex_2() for _ in range(4))
+        info = pa.precond_conjugate_gradient(pa.Multigrid(levels, 2), x, b, 200, 1e-6, rr, a, s,
+                                             pa.PoissonOperator(g, 0.5 / dims[-1]), tol_kind="rel")
+        out[flag] = (z.download(), x.download(), info.iterations, info.converged)
+    (z1, x1, i1, c1), (z0, x0, i0, c0) = out["1"], out["0"]
+    assert np.all(np.isfinite(z1)) and c1 and c0
+    assert np.array_equal(z1, z0)
+    assert i1 == i0
+    assert np.array_equal(x1, x0)
