@@ -28,7 +28,9 @@
 // Thread positions are split by row parity (rows 2i+q, q = 0, 1), so the x
 // parity of the cell a half-sweep updates is the same across a position slot:
 // the activity test and the reciprocal 1/(s*nc) come from per-position
-// tables selected by that parity.
+// tables selected by that parity. With an even number of slots the parity of
+// step t - t0 = J is that of t0 + J, so the workgroup runs one of two copies
+// of the loop (t0 even or odd) in which every such choice is a constant.
 //
 // Every phase touches only the thread's own positions plus LDS values written
 // in an earlier phase; the kernel puts a barrier after each half-sweep:
@@ -113,20 +115,13 @@ __device__ __forceinline__ int tb_opaque(int v) {
 }
 __host__ inline int tb_opaque(int v) { return v; }
 
-// Up to four floats as named scalars, read and written at indices that are
-// compile-time constants after unrolling: unlike an array, a select between
-// entries of two of them stays a select of values (an array entry selected
-// this way can become a load from a selected address, in scratch memory).
-struct Reg4 {
-    float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
-    __host__ __device__ __forceinline__ float get(int i) const { return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3; }
-    __host__ __device__ __forceinline__ void set(int i, float x) {
-        if (i == 0) v0 = x;
-        else if (i == 1) v1 = x;
-        else if (i == 2) v2 = x;
-        else v3 = x;
-    }
-};
+// f(std::integral_constant<int, i>) for i = 0 .. N-1: a loop whose index is a
+// compile-time constant in the body from the start (table entries it selects
+// are constant indices, never a select between two entries).
+template <typename F, int... I>
+__host__ __device__ __forceinline__ void tb_static_for(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
 
 // Per-thread state and phases of one workgroup. xin: the field before the
 // half-sweeps (unused with ZERO_X: the sweeps start from 0); xout: the result
@@ -178,12 +173,6 @@ struct TbThread {
     int act[KP];
     bool ind[KP], tile[KP];  // inside the grid; inside the tile
     float rin[KP][2], rbd[KP][2];  // 1/(s*nc) for x parity p: interior plane / one z side missing
-    // A cell updated at step t has x parity p = (CF + row parity + t) & 1 in
-    // every half-sweep of the step (half-sweep h updates colour CF+h of plane
-    // t-h), so ra / rb hold 1/(s*nc) of that cell (interior plane / one z side
-    // missing) for the whole step (set by begin_step).
-    static_assert(KP <= 4, "per-step reciprocals are held in a Reg4");
-    Reg4 ra, rb;
     float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
     bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
     double acc[2];
@@ -353,9 +342,9 @@ struct TbThread {
         }
         return z >= 0 && z < D.d && z >= zs - NS + 1 && z < ze + NS - 1;
     }
-    // xpf (plane z) into slot SL; PRO: plus its prolongation, from Q of its
-    // coarse planes K0, K1 in q0, q1
-    template <int SL>
+    // xpf (plane z, of parity ZP) into slot SL; PRO: plus its prolongation,
+    // from Q of its coarse planes K0, K1 in q0, q1
+    template <int SL, int ZP>
     __host__ __device__ __forceinline__ void fill(int z) {
         float* base = lds + SL * 2 * PLANE;
         float wz0 = 0.0f, wz1 = 0.0f;
@@ -373,7 +362,8 @@ struct TbThread {
                 if constexpr (PRO) {
                     const float pl = wz0 * q0[0][k] + wz1 * q1[0][k];  // cell x = 2m
                     const float pr = wz0 * q0[1][k] + wz1 * q1[1][k];  // cell x = 2m + 1
-                    const bool c0 = ((k / KC + z) & 1) != 0;  // colour of x = 2m (row parity k / KC)
+                    // colour of x = 2m (row parity k / KC): a constant after unrolling
+                    const bool c0 = ((k / KC + ZP) & 1) != 0;
                     v0 = v0 + (c0 ? pr : pl);
                     v1 = v1 + (c0 ? pl : pr);
                 }
@@ -403,32 +393,35 @@ struct TbThread {
         store_c(sc + CPL, c1);
         store_c(sc + 2 * CPL, c2);
     }
-    // planes t0-1, t0 into slots 0, 1; b of plane t0; prefetch x of t0+1, b of t0+1
+    // planes t0-1, t0 into slots 0, 1; b of plane t0; prefetch x of t0+1, b of
+    // t0+1. PAR = t0 & 1: the ring has an even number of slots, so the parity
+    // of every step is a compile-time constant within a round.
+    template <int PAR>
     __host__ __device__ __forceinline__ void prologue() {
         if constexpr (PRO) {
             // A = t0/2 - 1 (t0 even) or (t0 - 3)/2 (t0 odd); (K0, K1) of plane z
             // is (z >> 1, z >> 1 + 1) for odd z, (z >> 1, z >> 1 - 1) for even z
             const float* sc = lds + (SLOTS - 1) * 2 * PLANE;
-            if (t0 & 1) {
+            if constexpr (PAR) {
                 q_of(sc + CPL, q0);  // plane t0 - 1: (A+1, A)
                 q_of(sc, q1);
-                fill<0>(t0 - 1);
+                fill<0, 0>(t0 - 1);
                 q_of(sc + 2 * CPL, q1);  // plane t0: (A+1, A+2)
             } else {
                 q_of(sc, q0);  // plane t0 - 1: (A, A+1)
                 q_of(sc + CPL, q1);
-                fill<0>(t0 - 1);
+                fill<0, 1>(t0 - 1);
                 q_swap();  // plane t0: (A+1, A)
             }
         } else {
             load_x(t0 - 1);
-            fill<0>(t0 - 1);
+            fill<0, PAR ^ 1>(t0 - 1);
         }
         load_x(t0);
-        fill<1>(t0);
-        if constexpr (PRO) {
-            // t0 even: step t0 fills plane t0 + 1 with (A+1, A+2)
-            if (!(t0 & 1)) q_of(lds + (SLOTS - 1) * 2 * PLANE + 2 * CPL, q1);
+        fill<1, PAR>(t0);
+        if constexpr (PRO && !PAR) {
+            // step t0 fills plane t0 + 1 with (A+1, A+2)
+            q_of(lds + (SLOTS - 1) * 2 * PLANE + 2 * CPL, q1);
         }
 #pragma unroll
         for (int j = 1; j < NS; ++j)
@@ -444,45 +437,28 @@ struct TbThread {
         load_x(t0 + 1);
         bok = load_b(t0 + 1, bpf);
     }
-    // Step t with t - t0 = J (mod SLOTS): plane t+1 (prefetched) into its
-    // slot, which only this thread's own reads of step t touch; prefetch plane
-    // t+2. PRO: for odd t plane t+1 has (K0, K1) = ((t+1)/2, (t-1)/2), the two
-    // of the step before swapped; the step stores coarse plane (t+3)/2 into
-    // the coarse slot (its last reads, at step t-2, are barriers behind),
-    // prefetches plane (t+5)/2 and, after its first barrier (q_next), computes
-    // Q((t+3)/2) into q1, which the fill at step t dropped: the next (even)
-    // step fills plane t+2 with ((t+1)/2, (t+3)/2).
-    template <int J>
+    // Step t (t - t0 = J mod SLOTS, TP = t & 1): plane t+1 (prefetched) into
+    // its slot, which only this thread's own reads of step t touch; prefetch
+    // plane t+2. PRO: for odd t plane t+1 has (K0, K1) = ((t+1)/2, (t-1)/2),
+    // the two of the step before swapped; the step stores coarse plane (t+3)/2
+    // into the coarse slot (its last reads, at step t-2, are barriers behind),
+    // prefetches plane (t+5)/2 and, after its first barrier (stage 1),
+    // computes Q((t+3)/2) into q1, which the fill at step t dropped: the next
+    // (even) step fills plane t+2 with ((t+1)/2, (t+3)/2).
+    template <int J, int TP>
     __host__ __device__ __forceinline__ void begin_step(int t) {
-        if constexpr (PRO) {
-            if (t & 1) {
-                store_c(lds + S::LDS_FLOATS, cpf);
-                load_c((t + 5) >> 1, cpf);
-                q_swap();
-            }
+        if constexpr (PRO && TP) {
+            store_c(lds + S::LDS_FLOATS, cpf);
+            load_c((t + 5) >> 1, cpf);
+            q_swap();
         }
-        fill<mod(J + 2)>(t + 1);
+        fill<mod(J + 2), TP ^ 1>(t + 1);
         load_x(t + 2);
-        step_recips(t);
     }
-    // half-sweep H on plane t-H (t - t0 = J mod SLOTS)
-    // The step's reciprocals (ra, rb) from rin / rbd.
-    __host__ __device__ __forceinline__ void step_recips(int t) {
-#pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            const int p = (CF + k / KC + t) & 1;
-            // selects of opaque values (not a load from a selected table entry)
-            const float i0 = tb_opaque(rin[k][0]), i1 = tb_opaque(rin[k][1]);
-            const float d0 = tb_opaque(rbd[k][0]), d1 = tb_opaque(rbd[k][1]);
-            ra.set(k, p ? i1 : i0);
-            rb.set(k, p ? d1 : d0);
-        }
-    }
-    template <int H, int J>
+    // half-sweep H on plane t-H (t - t0 = J mod SLOTS, TP = t & 1)
+    template <int H, int J, int TP>
     __host__ __device__ __forceinline__ void stage(int t) {
-        if constexpr (PRO && H == 1) {  // q_next: after the barrier behind the coarse slot store
-            if (t & 1) q_of(lds + S::LDS_FLOATS, q1);
-        }
+        if constexpr (PRO && H == 1 && TP) q_of(lds + S::LDS_FLOATS, q1);  // after the barrier behind the store
         const int z = t - H;
         const int lo = zs - (NS - 1 - H) > 0 ? zs - (NS - 1 - H) : 0;
         const int hi = ze + (NS - 1 - H) < D.d ? ze + (NS - 1 - H) : D.d;
@@ -496,13 +472,13 @@ struct TbThread {
         // both z neighbours exist or their sides are open: interior reciprocals
         const bool zin = (z > 0 || (D.bc & 16)) && (z < D.d - 1 || (D.bc & 32));
         const bool own = DOT && H >= NS - 2 && z >= zs && z < ze;
-#pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            const int p = (c + k / KC + z) & 1;  // x parity of the colour-c cell (row parity k / KC)
+        tb_static_for([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int p = (c + k / KC + TP + H) & 1;  // x parity of the colour-c cell (row parity k / KC)
             if (act[k] & (1 << (H + 4 * p))) {
                 const int e = e_[k];
                 const float rv = bq[H][c][k];
-                const float a = zin ? ra.get(k) : rb.get(k);  // 1/(s*nc) of this cell (step_recips)
+                const float a = zin ? rin[k][p] : rbd[k][p];  // 1/(s*nc) of this cell
                 float v;
                 if constexpr (ZERO_X && H == 0) {
                     // first half-sweep from z = 0: every neighbour still holds +0, whose
@@ -522,7 +498,7 @@ struct TbThread {
                 xc[e] = v;
                 if (own && tile[k]) acc[H - (NS - 2)] += (double)v * (double)rv;
             }
-        }
+        }, std::make_integer_sequence<int, KP>{});
     }
     // store plane t-NS+1 (final after half-sweep NS-1 of step t); advance the
     // b ring; prefetch b of plane t+2
@@ -740,24 +716,31 @@ inline bool tb_on(bool enabled, int nu, const Dims& D) {
     return enabled && nu == 2 && tb_blocks(D) <= grid3_rb(D).nb;
 }
 
-// One step (t - t0 = J mod SLOTS): begin, half-sweeps 0..NS-1 with a barrier
-// after each, end.
-template <int J, typename TH, int... H>
+// One step (t - t0 = J mod SLOTS, TP = t & 1): begin, half-sweeps 0..NS-1
+// with a barrier after each, end.
+template <int J, int TP, typename TH, int... H>
 __device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
-    th.template begin_step<J>(t);
-    ((th.template stage<H, J>(t), __syncthreads()), ...);
+    th.template begin_step<J, TP>(t);
+    ((th.template stage<H, J, TP>(t), __syncthreads()), ...);
     th.template end_step<J>(t);
 }
-// Steps t .. t+SLOTS-1 (up to t1), then t advanced past them.
-template <int NS, typename TH, int... J>
+// Steps t .. t+SLOTS-1 (up to t1), then t advanced past them; PAR = t0 & 1.
+template <int NS, int PAR, typename TH, int... J>
 __device__ __forceinline__ void tb_round(TH& th, int& t, int t1, std::integer_sequence<int, J...>) {
-    ((t <= t1 ? (tb_step<J>(th, t, std::make_integer_sequence<int, NS>{}), ++t) : t), ...);
+    ((t <= t1 ? (tb_step<J, (PAR + J) & 1>(th, t, std::make_integer_sequence<int, NS>{}), ++t) : t), ...);
+}
+template <int PAR, typename TH>
+__device__ __forceinline__ void tb_run(TH& th) {
+    th.template prologue<PAR>();
+    __syncthreads();
+    const int t1 = th.last_step();
+    int t = th.first_step();
+    while (t <= t1) tb_round<4, PAR>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
 }
 // One workgroup of the one-pass smoother (four half-sweeps) on tile
 // (bx, by, bz) of a level; acc: its tile's z.r sums of the last two
-// half-sweeps (with DOT).
-// PL >= 0: the sweeps start from xin + P zco (zco: the coarse level, layout
-// PL, dims Dc).
+// half-sweeps (with DOT). PL >= 0: the sweeps start from xin + P zco (zco:
+// the coarse level, layout PL, dims Dc).
 template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT, int PL = -1>
 __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, int by, int bz, int zc,
                                              const float* xin, float* xout, const float* r, float s,
@@ -771,11 +754,8 @@ __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, 
         th.prologue_coarse();
         __syncthreads();
     }
-    th.prologue();
-    __syncthreads();
-    const int t1 = th.last_step();
-    int t = th.first_step();
-    while (t <= t1) tb_round<4>(th, t, t1, std::make_integer_sequence<int, TH::SLOTS>{});
+    if (th.first_step() & 1) tb_run<1>(th);
+    else tb_run<0>(th);
     acc[0] = th.acc[0];
     acc[1] = th.acc[1];
 }

@@ -89,23 +89,31 @@ static void ref_prolong(const Dims& D, const Dims& Dc, int LC, std::vector<float
             }
 }
 
-template <int NS, int J, int H = 0, typename V>
+template <int NS, int J, int TP, int H = 0, typename V>
 static void stages(V& th, int t) {
     if constexpr (H < NS) {
-        for (auto& tt : th) tt.template stage<H, J>(t);
-        stages<NS, J, H + 1>(th, t);
+        for (auto& tt : th) tt.template stage<H, J, TP>(t);
+        stages<NS, J, TP, H + 1>(th, t);
     }
 }
-template <int NS, int SL, int J = 0, typename V>
+// steps t .. t+SL-1 (up to t1) of a workgroup whose first step has parity PAR
+template <int NS, int SL, int PAR, int J = 0, typename V>
 static void round_(V& th, int& t, int t1) {
     if constexpr (J < SL) {
         if (t > t1) return;
-        for (auto& tt : th) tt.template begin_step<J>(t);
-        stages<NS, J>(th, t);
+        constexpr int TP = (PAR + J) & 1;
+        for (auto& tt : th) tt.template begin_step<J, TP>(t);
+        stages<NS, J, TP>(th, t);
         for (auto& tt : th) tt.template end_step<J>(t);
         ++t;
-        round_<NS, SL, J + 1>(th, t, t1);
+        round_<NS, SL, PAR, J + 1>(th, t, t1);
     }
+}
+template <int NS, int PAR, typename V>
+static void run_steps(V& th) {
+    for (auto& t : th) t.template prologue<PAR>();
+    int t = th[0].first_step(), t1 = th[0].last_step();
+    while (t <= t1) round_<NS, V::value_type::SLOTS, PAR>(th, t, t1);
 }
 
 // PL >= 0: the sweeps start from xin + P zc (coarse level of half the
@@ -144,9 +152,8 @@ static int run(const Dims& D, int want, int zmin, unsigned seed) {
         }
         if constexpr (TH::PRO)
             for (auto& t : th) t.prologue_coarse();
-        for (auto& t : th) t.prologue();
-        int t = th[0].first_step(), t1 = th[0].last_step();
-        while (t <= t1) round_<NS, TH::SLOTS>(th, t, t1);
+        if (th[0].first_step() & 1) run_steps<NS, 1>(th);
+        else run_steps<NS, 0>(th);
         for (auto& tt : th) { dsum[0] += tt.acc[0]; dsum[1] += tt.acc[1]; }
     }
     int bad = 0;
