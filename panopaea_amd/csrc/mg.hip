@@ -202,16 +202,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 }
 
 // The 2D form: one workgroup per tile of a 2D level.
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int PL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_smooth_tb2(
     Dims D, TbGrid g, const float* xin, float* xout, const float* r, float s, RInv<float> ri, double* part,
-    const SolverState* st) {
+    const float* zco, Dims Dc, const SolverState* st) {
     if (st->done) return;
-    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>::LDS_FLOATS];
+    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int L = xcd_remap(blockIdx.x, g.nb);
     const int bx = L % g.gx, by = L / g.gx;
     double acc[2];
-    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, D, bx, by, xin, xout, r, s, ri.v, acc);
+    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, D, bx, by, xin, xout, r, s, ri.v, acc, zco, Dc);
     if constexpr (DOT) {
         const double a0 = block_sum(acc[0]);
         const double a1 = block_sum(acc[1]);
@@ -223,21 +223,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 }
 
 // The four half-sweeps (colours CF, CF^1, ...) of a 3D f32 level in one pass;
-// with part, the z.r partials (2 per workgroup, *np set). PL >= 0 (3D only):
-// the sweeps start from xin + P zco, zco the coarse level (layout PL, dims Dc).
+// with part, the z.r partials (2 per workgroup, *np set). PL >= 0: the
+// sweeps start from xin + P zco, zco the coarse level (layout PL, dims Dc).
 template <int CF, bool ZERO_X, bool DOT, int PL = -1>
 static void smooth_tb(Ctx* c, const char* name, double bytes, const Dims& D, const float* xin, float* xout,
                       const float* r, float s, const RInv<float>& ri, double* part, int* np, const SolverState* st,
                       const float* zco = nullptr, const Dims& Dc = Dims{}) {
-    if constexpr (PL < 0) {
-        if (D.d <= 1) {
-            using C = Tb2D;
-            const TbGrid g = tb2_grid(D);
-            klaunch(c, name, bytes, k_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT>, dim3(g.nb),
-                    dim3(C::NT), D, g, xin, xout, r, s, ri, part, st);
-            if (np) *np = 2 * g.nb;
-            return;
-        }
+    if (D.d <= 1) {
+        using C = Tb2D;
+        const TbGrid g = tb2_grid(D);
+        klaunch(c, name, bytes, k_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZERO_X, DOT, PL>, dim3(g.nb),
+                dim3(C::NT), D, g, xin, xout, r, s, ri, part, zco, Dc, st);
+        if (np) *np = 2 * g.nb;
+        return;
     }
     with_tb(D, [&](auto cfg) {
         using C = decltype(cfg);
@@ -458,7 +456,7 @@ pano_status mg_setup(Ctx* c, const Dims& D, int levels, int nu, double dt, MG<T>
         }
     }
     const char* pro = getenv("PANO_MG_PRO");
-    m->pro = std::is_same<T, float>::value && D.d > 1 && !(pro && pro[0] == '0');
+    m->pro = std::is_same<T, float>::value && !(pro && pro[0] == '0');
     for (int l = 0; l + 1 < levels; ++l) {
         m->w[l] = nullptr;
         if (!pro_level(m, l)) continue;
@@ -938,13 +936,13 @@ static void level_up(Ctx* c, MG<T>* m, int l, T* const* zl, const T* zd, const T
     const double unit = (double)D.d * D.h * D.w * sizeof(T);
     const char* nh = l == 0 ? "smooth_l0" : "";
     const char* np = l == 0 ? "prolong_l0" : "";
-    // 3D f32 with the prolongation in the post-smoothing: the four half-sweeps
+    // f32 with the prolongation in the post-smoothing: the four half-sweeps
     // from zd + P z_{l+1} into z in one pass (zd: the pre-smoothed z, a
     // separate buffer, as other workgroups still read their halo from it while
     // this one stores). Timed bytes: the prolongation pass (z and z_c in, z
     // out) and two red-black sweeps of 3 field passes each; the pass moves
     // about 3.
-    if constexpr (DIM == 3 && std::is_same<T, float>::value) {
+    if constexpr (std::is_same<T, float>::value) {
         if (zd != zl[l]) {
             const char* npost = l == 0 ? "smooth_post_l0" : "";
             const double bytes = (8 + cfrac) * unit;

@@ -22,7 +22,7 @@ struct MG {
     int coarse_n[3] = {1, 1, 1};                         // coarsest extents x, y, z
     bool tb = true;  // 3D f32 levels: pre/post-smoothing as one pass each (PANO_MG_FUSED=0 turns it off)
     bool fdm_split = false;  // 3D coarsest level: passes split over many workgroups (PANO_FDM_SPLIT=0: off)
-    // 3D f32 one-pass levels above the coarsest: the prolongation runs inside
+    // f32 one-pass levels above the coarsest: the prolongation runs inside
     // the post-smoothing (PANO_MG_PRO=0: off), which then reads the
     // pre-smoothed z from w[l] (the down leg's z) while it writes z.
     bool pro = false;

@@ -123,6 +123,91 @@ __host__ __device__ __forceinline__ void tb_static_for(F&& f, std::integer_seque
     (f(std::integral_constant<int, I>{}), ...);
 }
 
+// The coarse level of the prolongation fused into the one-pass smoothers
+// (layout PL, dims Dc; a 2D level has Dc.d = 1): a coarse plane's region of
+// CW x CH cells around the tile + halo (region origin Ib, Jb in coarse x,
+// y), loaded with clamped coordinates, and Q of a position's two cells from
+// a region in LDS. Position bits in act (8..10 the edge flags y, x-, x+;
+// 11..22 the LDS offset of the position's coarse cell in a region).
+template <int NS, int TXP, int TY, int NTHREADS, int PL>
+struct TbPro {
+    using S = TbShape<NS, TXP, TY, NTHREADS>;
+    static constexpr int CW = S::CW, CPL = S::CPL, CKN = S::CKN, KC = S::KC, KP = S::KP, NT = S::NT, HP = S::HP,
+                         CPOS = S::CPOS;
+    const float* zco;
+    Dims Dc;
+    int chw, cn2, Ib, Jb;  // plane size, colour offset, region origin
+
+    __host__ __device__ __forceinline__ void init(const float* zco_, const Dims& Dc_, int bx, int by, int tid,
+                                                  int (&act)[KP]) {
+        zco = zco_;
+        Dc = Dc_;
+        chw = PL == REDBLACK ? Dc.h * (Dc.w >> 1) : Dc.h * Dc.w;
+        cn2 = Dc.d * chw;
+        const int m0 = bx * TXP, y0 = by * TY;
+        Ib = m0 - HP - 1;
+        Jb = ((y0 - NS) >> 1) - 1;  // y0 - NS is even
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int q = k / KC, f = tid + (k % KC) * NT;
+            int rp, pr;
+            S::pos_of(f < CPOS ? f : 0, rp, pr);
+            const int y = y0 - NS + 2 * rp + q, m = m0 - HP + pr;
+            const int I = y >> 1, J = (y & 1) ? I + 1 : I - 1;  // p1 along y
+            const int ey = J < 0 || J >= Dc.h, el = m == 0, er = m + 1 >= Dc.w;
+            act[k] |= ((ey | (el << 1) | (er << 2)) << 8) | (((rp + 1) * CW + pr + 1) << 11);
+        }
+    }
+    // the thread's region cells of coarse plane K (clamped to the level)
+    __host__ __device__ __forceinline__ void load(int K, float (&v)[CKN], int tid) const {
+        K = K < 0 ? 0 : (K >= Dc.d ? Dc.d - 1 : K);
+        const int tv = tb_opaque(tid);  // offsets recomputed per load (no registers held)
+#pragma unroll
+        for (int i = 0; i < CKN; ++i) {
+            const int e = tv + i * NT < CPL ? tv + i * NT : 0;
+            int J = Jb + e / CW, I = Ib + e % CW;
+            J = J < 0 ? 0 : (J >= Dc.h ? Dc.h - 1 : J);
+            I = I < 0 ? 0 : (I >= Dc.w ? Dc.w - 1 : I);
+            if constexpr (PL == REDBLACK)
+                v[i] = zco[((I + J + K) & 1) * cn2 + K * chw + J * (Dc.w >> 1) + (I >> 1)];
+            else
+                v[i] = zco[K * chw + J * Dc.w + I];
+        }
+    }
+    __host__ __device__ __forceinline__ void store(float* dst, const float (&v)[CKN], int tid) const {
+#pragma unroll
+        for (int i = 0; i < CKN; ++i)
+            if (tid + i * NT < CPL) dst[tid + i * NT] = v[i];
+    }
+    // Q of the coarse plane whose region is at C, both cells of every position.
+    // Positions k (row 2rp) and k + KC (row 2rp + 1) share pair and row pair,
+    // so their coarse row J0 = y >> 1 (x-interpolated once) and x weights;
+    // J1 is the row above (even y) / below (odd y).
+    __host__ __device__ __forceinline__ void q_of(const int (&act)[KP], const float* C, float (&qv)[2][KP]) const {
+        const float w34 = 0.75f, w14 = 0.25f, one = 1.0f, zero = 0.0f;
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            const int ae = tb_opaque(act[k]), ao = tb_opaque(act[k + KC]);  // weights recomputed per call
+            const int fe = (ae >> 8) & 7, fo = (ao >> 8) & 7;
+            const float* r0 = C + (ae >> 11);
+            const float wl0 = (fe & 2) ? one : w34, wl1 = (fe & 2) ? zero : w14;
+            const float wr0 = (fe & 4) ? one : w34, wr1 = (fe & 4) ? zero : w14;
+            const float am = r0[-CW - 1], bm = r0[-CW], cm = r0[-CW + 1];  // row J0 - 1
+            const float a0 = r0[-1], b0 = r0[0], c0 = r0[1];                // row J0
+            const float ap = r0[CW - 1], bp = r0[CW], cp = r0[CW + 1];      // row J0 + 1
+            const float pl0 = wl0 * b0 + wl1 * a0, pr0 = wr0 * b0 + wr1 * c0;
+            const float plm = wl0 * bm + wl1 * am, prm = wr0 * bm + wr1 * cm;
+            const float plp = wl0 * bp + wl1 * ap, prp = wr0 * bp + wr1 * cp;
+            const float ye0 = (fe & 1) ? one : w34, ye1 = (fe & 1) ? zero : w14;
+            const float yo0 = (fo & 1) ? one : w34, yo1 = (fo & 1) ? zero : w14;
+            qv[0][k] = ye0 * pl0 + ye1 * plm;
+            qv[1][k] = ye0 * pr0 + ye1 * prm;
+            qv[0][k + KC] = yo0 * pl0 + yo1 * plp;
+            qv[1][k + KC] = yo0 * pr0 + yo1 * prp;
+        }
+    }
+};
+
 // Per-thread state and phases of one workgroup. xin: the field before the
 // half-sweeps (unused with ZERO_X: the sweeps start from 0); xout: the result
 // (may equal xin only with ZERO_X: other workgroups still read xin's halo
@@ -176,11 +261,8 @@ struct TbThread {
     float xpf[2][KP], bpf[2][KP], bq[NS][2][KP];
     bool xok, bok;  // whether the plane held in xpf / bpf is used (else it reads as 0)
     double acc[2];
-    // PRO: the coarse level and its geometry (plane size chw; colour offset cn2)
-    const float* zco;
-    Dims Dc;
-    int chw, cn2, Ib, Jb;  // plane size, colour offset, region origin (coarse x, y)
-    float cpf[CKN];        // the thread's region cells of the next coarse plane to store
+    TbPro<NS, TXP, TY, NTHREADS, PL> pro;  // PRO: the coarse level
+    float cpf[CKN];  // the thread's region cells of the next coarse plane to store
     // Q of coarse planes K0, K1 of the next fine plane to fill, for the cells
     // x = 2m, 2m+1 (K0 = z >> 1, K1 the other coarse plane next to z)
     float q0[2][KP], q1[2][KP];
@@ -235,73 +317,14 @@ struct TbThread {
         }
     }
     // PRO: the coarse level (after init).
-    __host__ __device__ __forceinline__ void init_pro(const float* zco_, const Dims& Dc_, int bx, int by) {
-        zco = zco_;
-        Dc = Dc_;
-        chw = PL == REDBLACK ? Dc.h * (Dc.w >> 1) : Dc.h * Dc.w;
-        cn2 = Dc.d * chw;
-        const int m0 = bx * TXP, y0 = by * TY;
-        Ib = m0 - HP - 1;
-        Jb = ((y0 - NS) >> 1) - 1;  // y0 - NS is even
-#pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            const int q = k / KC, f = tid + (k % KC) * NT;
-            int rp, pr;
-            S::pos_of(f < CPOS ? f : 0, rp, pr);
-            const int y = y0 - NS + 2 * rp + q, m = m0 - HP + pr;
-            const int I = y >> 1, J = (y & 1) ? I + 1 : I - 1;  // p1 along y
-            const int ey = J < 0 || J >= Dc.h, el = m == 0, er = m + 1 >= Dc.w;
-            act[k] |= ((ey | (el << 1) | (er << 2)) << 8) | (((rp + 1) * CW + pr + 1) << 11);
-        }
+    __host__ __device__ __forceinline__ void init_pro(const float* zco, const Dims& Dc, int bx, int by) {
+        pro.init(zco, Dc, bx, by, tid, act);
     }
-    // the thread's region cells of coarse plane K (clamped to the level)
-    __host__ __device__ __forceinline__ void load_c(int K, float (&v)[CKN]) const {
-        K = K < 0 ? 0 : (K >= Dc.d ? Dc.d - 1 : K);
-        const int tv = tb_opaque(tid);  // offsets recomputed per load (no registers held)
-#pragma unroll
-        for (int i = 0; i < CKN; ++i) {
-            const int e = tv + i * NT < CPL ? tv + i * NT : 0;
-            int J = Jb + e / CW, I = Ib + e % CW;
-            J = J < 0 ? 0 : (J >= Dc.h ? Dc.h - 1 : J);
-            I = I < 0 ? 0 : (I >= Dc.w ? Dc.w - 1 : I);
-            if constexpr (PL == REDBLACK)
-                v[i] = zco[((I + J + K) & 1) * cn2 + K * chw + J * (Dc.w >> 1) + (I >> 1)];
-            else
-                v[i] = zco[K * chw + J * Dc.w + I];
-        }
-    }
+    __host__ __device__ __forceinline__ void load_c(int K, float (&v)[CKN]) const { pro.load(K, v, tid); }
     __host__ __device__ __forceinline__ void store_c(float* dst, const float (&v)[CKN]) const {
-#pragma unroll
-        for (int i = 0; i < CKN; ++i)
-            if (tid + i * NT < CPL) dst[tid + i * NT] = v[i];
+        pro.store(dst, v, tid);
     }
-    // Q of the coarse plane whose region is at C, both cells of every position.
-    // Positions k (row 2rp) and k + KC (row 2rp + 1) share pair and row pair,
-    // so their coarse row J0 = y >> 1 (x-interpolated once) and x weights;
-    // J1 is the row above (even y) / below (odd y).
-    __host__ __device__ __forceinline__ void q_of(const float* C, float (&qv)[2][KP]) const {
-        const float w34 = 0.75f, w14 = 0.25f, one = 1.0f, zero = 0.0f;
-#pragma unroll
-        for (int k = 0; k < KC; ++k) {
-            const int ae = tb_opaque(act[k]), ao = tb_opaque(act[k + KC]);  // weights recomputed per call
-            const int fe = (ae >> 8) & 7, fo = (ao >> 8) & 7;
-            const float* r0 = C + (ae >> 11);
-            const float wl0 = (fe & 2) ? one : w34, wl1 = (fe & 2) ? zero : w14;
-            const float wr0 = (fe & 4) ? one : w34, wr1 = (fe & 4) ? zero : w14;
-            const float am = r0[-CW - 1], bm = r0[-CW], cm = r0[-CW + 1];  // row J0 - 1
-            const float a0 = r0[-1], b0 = r0[0], c0 = r0[1];                // row J0
-            const float ap = r0[CW - 1], bp = r0[CW], cp = r0[CW + 1];      // row J0 + 1
-            const float pl0 = wl0 * b0 + wl1 * a0, pr0 = wr0 * b0 + wr1 * c0;
-            const float plm = wl0 * bm + wl1 * am, prm = wr0 * bm + wr1 * cm;
-            const float plp = wl0 * bp + wl1 * ap, prp = wr0 * bp + wr1 * cp;
-            const float ye0 = (fe & 1) ? one : w34, ye1 = (fe & 1) ? zero : w14;
-            const float yo0 = (fo & 1) ? one : w34, yo1 = (fo & 1) ? zero : w14;
-            qv[0][k] = ye0 * pl0 + ye1 * plm;
-            qv[1][k] = ye0 * pr0 + ye1 * prm;
-            qv[0][k + KC] = yo0 * pl0 + yo1 * plp;
-            qv[1][k + KC] = yo0 * pr0 + yo1 * prp;
-        }
-    }
+    __host__ __device__ __forceinline__ void q_of(const float* C, float (&qv)[2][KP]) const { pro.q_of(act, C, qv); }
     __host__ __device__ __forceinline__ void q_swap() {
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
@@ -350,7 +373,7 @@ struct TbThread {
         float wz0 = 0.0f, wz1 = 0.0f;
         if constexpr (PRO) {
             const int I = z >> 1, J = (z & 1) ? I + 1 : I - 1;  // p1 along z
-            const bool ez = J < 0 || J >= Dc.d;
+            const bool ez = J < 0 || J >= pro.Dc.d;
             wz0 = ez ? 1.0f : 0.75f;
             wz1 = ez ? 0.0f : 0.25f;
         }
@@ -539,12 +562,19 @@ struct TbThread {
 // stores the tile. Same positions, activity tables and arithmetic as the 3D
 // form with four neighbours (nsum over x-, x+, y-, y+). Every cell of a 2D
 // level with an even width has an x neighbour, so nc >= 1.
-template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT>
+// PL >= 0: the half-sweeps start from xin + P zco (the prolongation from the
+// coarse level zco, layout PL, dims Dc, with the arithmetic of prolong_run):
+// init stores the coarse region of the tile into LDS and keeps x; after a
+// barrier, fill_pro adds Q (the y and x interpolation) to each cell of x and
+// fills the plane.
+template <int NS, int TXP, int TY, int NTHREADS, int CF, bool ZERO_X, bool DOT, int PL = -1>
 struct Tb2Thread {
     using S = TbShape<NS, TXP, TY, NTHREADS>;
     static constexpr int PW = S::PW, PH = S::PH, PLANE = S::PLANE, NT = S::NT, KC = S::KC, KP = S::KP, HP = S::HP,
                          CPOS = S::CPOS;
-    static constexpr int LDS_FLOATS = 2 * PLANE;
+    static constexpr bool PRO = PL >= 0;
+    static_assert(!(PRO && ZERO_X), "the prolongation adds to a given field");
+    static constexpr int LDS_FLOATS = 2 * PLANE + (PRO ? S::CPL : 0);  // both colours, then (PRO) the region
 
     float* lds;
     Dims D;
@@ -557,9 +587,12 @@ struct Tb2Thread {
     float rin[KP][2];
     float bv[2][KP];
     double acc[2];
+    TbPro<NS, TXP, TY, NTHREADS, PL> pro;  // PRO: the coarse level
+    float xk[2][KP];                       // PRO: x until fill_pro
 
     __host__ __device__ __forceinline__ void init(float* lds_, const Dims& D_, int bx, int by, const float* xin,
-                                                  const float* r, float s_, const float* rinv, int tid_) {
+                                                  const float* r, float s_, const float* rinv, int tid_,
+                                                  const float* zco = nullptr, const Dims& Dc = Dims{}) {
         lds = lds_;
         D = D_;
         s = s_;
@@ -593,8 +626,9 @@ struct Tb2Thread {
             }
             act[k] = a;
         }
+        if constexpr (PRO) pro.init(zco, Dc, bx, by, tid, act);
         // all loads first (clamped, valid addresses), then the LDS fill and b
-        float xv[2][KP], b0[2][KP];
+        float xv[2][KP], b0[2][KP], cv[S::CKN];
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             xv[0][k] = xv[1][k] = 0.0f;
@@ -605,15 +639,37 @@ struct Tb2Thread {
             b0[0][k] = r[goff[k]];
             b0[1][k] = r[n2 + goff[k]];
         }
+        if constexpr (PRO) pro.load(0, cv, tid);
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             const bool ok = ind[k];
-            if ((k % KC) * NT + tid < CPOS) {
+            if constexpr (PRO) {
+                xk[0][k] = xv[0][k];
+                xk[1][k] = xv[1][k];
+            } else if ((k % KC) * NT + tid < CPOS) {
                 lds[e_[k]] = ok ? xv[0][k] : 0.0f;
                 lds[PLANE + e_[k]] = ok ? xv[1][k] : 0.0f;
             }
             bv[0][k] = ok ? b0[0][k] : 0.0f;
             bv[1][k] = ok ? b0[1][k] : 0.0f;
+        }
+        if constexpr (PRO) pro.store(lds + 2 * PLANE, cv, tid);
+    }
+    // PRO, after a barrier behind init: x + P zco into the plane. The cell x =
+    // 2m of row y has colour y & 1, the position's row parity k / KC.
+    __host__ __device__ __forceinline__ void fill_pro() {
+        float qv[2][KP];
+        pro.q_of(act, lds + 2 * PLANE, qv);
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            if ((k % KC) * NT + tid < CPOS) {
+                const bool ok = ind[k];
+                const bool c0 = (k / KC) != 0;
+                const float v0 = xk[0][k] + (c0 ? qv[1][k] : qv[0][k]);
+                const float v1 = xk[1][k] + (c0 ? qv[0][k] : qv[1][k]);
+                lds[e_[k]] = ok ? v0 : 0.0f;
+                lds[PLANE + e_[k]] = ok ? v1 : 0.0f;
+            }
         }
     }
     // half-sweep H (colour (CF + H) & 1)
@@ -761,13 +817,20 @@ __device__ __forceinline__ void tb_workgroup(float* lds, const Dims& D, int bx, 
 }
 
 // One workgroup of the 2D one-pass smoother on tile (bx, by) of a level.
-template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT>
+// PL >= 0: the sweeps start from xin + P zco (zco: the coarse level, layout
+// PL, dims Dc).
+template <int TXP, int TY, int NT, int CF, bool ZERO_X, bool DOT, int PL = -1>
 __device__ __forceinline__ void tb2_workgroup(float* lds, const Dims& D, int bx, int by, const float* xin, float* xout,
-                                              const float* r, float s, const float* rinv, double* acc) {
-    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>;
+                                              const float* r, float s, const float* rinv, double* acc,
+                                              const float* zco = nullptr, const Dims& Dc = Dims{}) {
+    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>;
     TH th;
-    th.init(lds, D, bx, by, xin, r, s, rinv, threadIdx.x);
+    th.init(lds, D, bx, by, xin, r, s, rinv, threadIdx.x, zco, Dc);
     __syncthreads();
+    if constexpr (TH::PRO) {
+        th.fill_pro();
+        __syncthreads();
+    }
     th.template stage<0>();
     __syncthreads();
     th.template stage<1>();

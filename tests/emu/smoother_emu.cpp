@@ -83,7 +83,9 @@ static void ref_prolong(const Dims& D, const Dims& Dc, int LC, std::vector<float
                         const int K = a ? K1 : K0, J = b ? J1 : J0;
                         px[a][b] = w0 * zc[cix(LC, Dc, K, J, m)] + w1 * zc[cix(LC, Dc, K, J, Is)];
                     }
-                const float v = wz0 * (wy0 * px[0][0] + wy1 * px[0][1]) + wz1 * (wy0 * px[1][0] + wy1 * px[1][1]);
+                // 2D (one plane): no z interpolation
+                const float v = D.d == 1 ? wy0 * px[0][0] + wy1 * px[0][1]
+                                         : wz0 * (wy0 * px[0][0] + wy1 * px[0][1]) + wz1 * (wy0 * px[1][0] + wy1 * px[1][1]);
                 const int i = rbi(D, z, y, xx);
                 x[i] = x[i] + v;
             }
@@ -196,9 +198,9 @@ static void ref_sweeps2(const Dims& D, int NS, int CF, bool zero_x, std::vector<
     }
 }
 
-template <int TXP, int TY, int NT, int CF, bool ZX, bool DOT>
+template <int TXP, int TY, int NT, int CF, bool ZX, bool DOT, int PL = -1>
 static int run2(const Dims& D, unsigned seed) {
-    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZX, DOT>;
+    using TH = Tb2Thread<4, TXP, TY, NT, CF, ZX, DOT, PL>;
     const int n = D.h * D.w;
     std::mt19937 rng(seed);
     std::uniform_real_distribution<float> U(-1, 1);
@@ -208,6 +210,13 @@ static int run2(const Dims& D, unsigned seed) {
     float s = 0.37f;
     RInv<float> ri = make_rinv<float>(s);
     xref = xin;
+    const Dims Dc{1, D.h / 2, D.w / 2, D.bc};
+    std::vector<float> zc;
+    if (PL >= 0) {
+        zc.resize((size_t)Dc.h * Dc.w);
+        for (auto& v : zc) v = U(rng);
+        ref_prolong(D, Dc, PL, xref, zc);
+    }
     double dref[2] = {0, 0};
     ref_sweeps2(D, 4, CF, ZX, xref, r, s, ri, dref);
     const int gx = ((D.w >> 1) + TXP - 1) / TXP, gy = (D.h + TY - 1) / TY;
@@ -216,7 +225,9 @@ static int run2(const Dims& D, unsigned seed) {
         int bx = L % gx, by = L / gx;
         std::vector<float> lds(TH::LDS_FLOATS, -777.0f);
         std::vector<TH> th(NT);
-        for (int i = 0; i < NT; ++i) th[i].init(lds.data(), D, bx, by, xin.data(), r.data(), s, ri.v, i);
+        for (int i = 0; i < NT; ++i) th[i].init(lds.data(), D, bx, by, xin.data(), r.data(), s, ri.v, i, zc.data(), Dc);
+        if constexpr (TH::PRO)
+            for (auto& t : th) t.fill_pro();
         for (auto& t : th) t.template stage<0>();
         for (auto& t : th) t.template stage<1>();
         for (auto& t : th) t.template stage<2>();
@@ -232,8 +243,8 @@ static int run2(const Dims& D, unsigned seed) {
         }
     double e0 = DOT ? fabs(dsum[0] - dref[0]) / (fabs(dref[0]) + 1e-30) : 0;
     double e1 = DOT ? fabs(dsum[1] - dref[1]) / (fabs(dref[1]) + 1e-30) : 0;
-    printf("2D CF=%d ZX=%d DOT=%d dims %dx%d bc=%d tiles %dx%d: %d mismatches, dot rel err %.2e %.2e\n", CF, ZX, DOT,
-           D.h, D.w, D.bc, gx, gy, bad, e0, e1);
+    printf("2D CF=%d ZX=%d DOT=%d PL=%d tile %dx%d dims %dx%d bc=%d tiles %dx%d: %d mismatches, dot rel err %.2e %.2e\n",
+           CF, ZX, DOT, PL, TXP, TY, D.h, D.w, D.bc, gx, gy, bad, e0, e1);
     return bad || e0 > 1e-12 || e1 > 1e-12;
 }
 
@@ -275,6 +286,16 @@ int main() {
         fails += run2<32, 32, 768, 1, false, true>(D, seed++);
         fails += run2<32, 32, 768, 1, false, false>(D, seed++);
         fails += run2<8, 4, 64, 1, false, true>(D, seed++);
+    }
+    // 2D with the prolongation in the post-smoothing (even extents; widths
+    // divisible by 4 for a red-black coarse level)
+    const Dims c2p[] = {{1, 64, 64, 0}, {1, 40, 72, 2}, {1, 100, 200, 15}, {1, 8, 8, 4}, {1, 34, 132, 9},
+                        {1, 66, 36, 12}};
+    for (auto D : c2p) {
+        fails += run2<32, 32, 768, 1, false, true, NATURAL>(D, seed++);
+        fails += run2<32, 32, 768, 1, false, false, REDBLACK>(D, seed++);
+        fails += run2<8, 4, 64, 1, false, true, REDBLACK>(D, seed++);
+        fails += run2<16, 16, 128, 1, false, false, NATURAL>(D, seed++);
     }
     printf("%s (%d failing cases)\n", fails ? "FAIL" : "OK", fails);
     return fails != 0;

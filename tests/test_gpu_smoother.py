@@ -72,7 +72,7 @@ def test_gpu_fused_smoothing_pcg(ctx, dims, bc, levels, monkeypatch):
     assert np.abs(xf - xu).max() <= 1e-4 * scale
 
 
-@pytest.mark.parametrize("dims,bc,levels", [c for c in CASES if len(c[0]) == 3])
+@pytest.mark.parametrize("dims,bc,levels", CASES)
 def test_gpu_prolongation_in_post_smoothing_bitwise(ctx, dims, bc, levels, monkeypatch):
     """The prolongation inside the one-pass post-smoothing (default) against the
     prolongation kernel followed by the one-pass post-smoothing (PANO_MG_PRO=0):
@@ -80,7 +80,7 @@ def test_gpu_prolongation_in_post_smoothing_bitwise(ctx, dims, bc, levels, monke
     z.r partials add the same terms per workgroup in the same order)."""
     print("prolongation in post-smoothing", dims, bc, levels, flush=True)
     monkeypatch.delenv("PANO_MG_FUSED", raising=False)
-    g = pa.Grid3d(dims, np.float32, bc=bc)
+    g = (pa.Grid2d if len(dims) == 2 else pa.Grid3d)(dims, np.float32, bc=bc)
     r = np.random.default_rng(dims[1]).standard_normal(g.num_elem_2()).astype(np.float32)
     if not any(bc):
         r -= r.mean(dtype=np.float64).astype(np.float32)
