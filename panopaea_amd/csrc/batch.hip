@@ -318,16 +318,18 @@ __global__ __launch_bounds__(256) void kb_prolong(BTab T, char* A,
 }
 
 // the 2D form over tb_[l][3]: tile (bx, by) of E.G, fields as below
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int PL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb2(
-    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base) {
+    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base, int xo) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
-    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT>::LDS_FLOATS];
+    __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int bx = lb % E.G.gx, by = lb / E.G.gx;
     double acc[2];
-    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, E.D, bx, by, fld<float>(A, E.o[0]), fld<float>(A, E.o[1]),
-                                                fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc);
+    float* out = fld<float>(A, E.o[ZERO_X ? xo : 1]);
+    tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, E.D, bx, by, ZERO_X ? out : fld<float>(A, E.o[0]), out,
+                                                    fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc,
+                                                    fld<float>(A, E.o[3]), E.Dc);
     if constexpr (DOT) {
         const double a0 = block_sum(acc[0]);
         const double a1 = block_sum(acc[1]);
@@ -339,21 +341,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 }
 
 // one-pass smoothing (the four half-sweeps of a leg) over tb_[l][config]:
-// o[0] = input z (post: the prolonged z in the residual scratch; pre: unused,
-// the sweeps start from 0), o[1] = z_l (output), o[2] = r_l. Tile (bx, by, bz)
-// of E.G as in the single-grid kernel. With DOT, the tile's z.r sums of the
-// last two half-sweeps go to p0[base + b] and p2[base + b].
-template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT>
+// o[0] = the post-smoothing's input z (with the prolongation in the
+// post-smoothing, PL >= 0: the pre-smoothed z, to which P z_{l+1} is added,
+// o[3] = z_{l+1}; else the prolonged z in the residual scratch), o[1] = z_l
+// (the post-smoothing's output), o[2] = r_l. The pre-smoothing (ZERO_X, from
+// 0) writes o[xo]. Tile (bx, by, bz) of E.G as in the single-grid kernel.
+// With DOT, the tile's z.r sums of the last two half-sweeps go to
+// p0[base + b] and p2[base + b].
+template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int PL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb(
-    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base) {
+    BTab T, char* A, const SolverState* st, double* p0, double* p2, int base, int xo) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
-    __shared__ float lds[TbShape<4, TXP, TY, NT>::LDS_FLOATS];
+    __shared__ float lds[TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int bx = lb % E.G.gx, q = lb / E.G.gx, by = q % E.G.gy, bz = q / E.G.gy;
     double acc[2];
-    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT>(lds, E.D, bx, by, bz, E.zc, fld<float>(A, E.o[0]),
-                                               fld<float>(A, E.o[1]), fld<float>(A, E.o[2]), (float)E.s, E.rinv,
-                                               acc);
+    float* out = fld<float>(A, E.o[ZERO_X ? xo : 1]);
+    tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, E.D, bx, by, bz, E.zc, ZERO_X ? out : fld<float>(A, E.o[0]),
+                                                   out, fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc,
+                                                   fld<float>(A, E.o[3]), E.Dc);
     if constexpr (DOT) {
         const double a0 = block_sum(acc[0]);
         const double a1 = block_sum(acc[1]);
@@ -474,6 +480,10 @@ pano_status hgrid_layout(Ctx* c, HGrid& g, int64_t base, bool proj) {
         int64_t n = (int64_t)L.d * L.h * L.w;
         g.o_zl[l] = take(n * 4);
         g.o_rl[l] = take(n * 4);
+    }
+    for (int l = 0; l + 1 < g.levels; ++l) {
+        Dims L = level_dims(D, l);
+        g.o_wl[l] = take((int64_t)L.d * L.h * L.w * 4);
     }
     Dims C = level_dims(D, g.levels - 1);
     g.o_t0 = take((int64_t)C.d * C.h * C.w * 8);
@@ -598,6 +608,8 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     {
         const char* fused = getenv("PANO_MG_FUSED");
         tb_enabled_ = !(fused && fused[0] == '0');
+        const char* pro = getenv("PANO_MG_PRO");
+        pro_ = !(pro && pro[0] == '0');
     }
     plane_.h.clear();
     zcol_.h.clear();
@@ -707,9 +719,10 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                     t.zc = tg.zc;
                     t.start = stb[l][k];
                     stb[l][k] += t.G.nb;
-                    t.o[0] = g.o_res;
+                    t.o[0] = pro_ ? g.o_wl[l] : g.o_res;
                     t.o[1] = g.o_zl[l];
                     t.o[2] = g.o_rl[l];
+                    t.o[3] = g.o_zl[l + 1];
                     tb_[l][k].h.push_back(t);
                     if (l == 0) dot_src.emplace_back(k + 1, (int)tb_[l][k].h.size() - 1);
                 } else {
@@ -719,11 +732,15 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                     rbs_[l].h.push_back(h);
                     if (l == 0) dot_src.emplace_back(0, (int)rbs_[l].h.size() - 1);
                 }
-                BEntry pr = r;
-                pr.o[0] = g.o_zl[l];
-                pr.o[1] = g.o_zl[l + 1];
-                pr.o[2] = tb ? g.o_res : g.o_zl[l];
-                prl_[l].h.push_back(pr);
+                // the down leg's z: where the pre-smoothing leaves it
+                const int64_t zd = tb && pro_ ? g.o_wl[l] : g.o_zl[l];
+                if (!(tb && pro_)) {
+                    BEntry pr = r;
+                    pr.o[0] = g.o_zl[l];
+                    pr.o[1] = g.o_zl[l + 1];
+                    pr.o[2] = tb ? g.o_res : g.o_zl[l];
+                    prl_[l].h.push_back(pr);
+                }
                 // residual and restriction: one pass on 2D levels (the single-grid rule),
                 // else the residual over pair tiles and the restriction over coarse tiles
                 if (tb_enabled_ && L.d <= 1) {
@@ -734,12 +751,13 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                     q.G.nb = q.G.gx * q.G.gy;
                     q.start = srr2[l];
                     srr2[l] += q.G.nb;
-                    q.o[0] = g.o_zl[l];
+                    q.o[0] = zd;
                     q.o[1] = g.o_rl[l];
                     q.o[2] = g.o_rl[l + 1];
                     rr2_[l].h.push_back(q);
                 } else {
                     BEntry rs = r;
+                    rs.o[0] = zd;
                     rs.start = srsd[l];
                     srsd[l] += rs.G.nb;
                     rsd_[l].h.push_back(rs);
@@ -912,38 +930,44 @@ pano_status BatchCore::enqueue_project(int ping) {
 // prolonged z, with the z.r partials when dot) for each configuration present.
 void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
     const SolverState* st = states;
+    // with the prolongation in the post-smoothing (pro_) the pre-smoothing
+    // writes the down leg's z (o[0]) and the post-smoothing adds P z_{l+1}
+    // (natural layout when level l+1 is the coarsest) to it
+    const int xo = pro_ ? 0 : 1;
+    const bool nat = l + 2 == levels_;
     auto go = [&](auto cfg, int k) {
         using C = decltype(cfg);
+        constexpr bool TWO_D = std::is_same<C, Tb2D>::value;
         const Table& t = tb_[l][k];
         if (t.nbt == 0) return;
         const dim3 g(t.nbt), b(C::NT);
-        if (!post)
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 0, true, false>), g, b, 0, c->stream,
-                               t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
-        else if (dot)
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, true>), g, b, 0, c->stream,
-                               t.view(), arena, st, part0_, part2_, tb_base_[k]);
-        else
-            hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, 1, false, false>), g, b, 0, c->stream,
-                               t.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
+        auto launch = [&](auto cf, auto zx, auto dt, auto pl, double* p0, double* p2, int base) {
+            constexpr int CF = decltype(cf)::value, PL = decltype(pl)::value;
+            constexpr bool ZX = decltype(zx)::value, DOT = decltype(dt)::value;
+            if constexpr (TWO_D)
+                hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, CF, ZX, DOT, PL>), g, b, 0, c->stream,
+                                   t.view(), arena, st, p0, p2, base, xo);
+            else
+                hipLaunchKernelGGL((kb_smooth_tb<C::TXP, C::TY, C::NT, C::WPE, CF, ZX, DOT, PL>), g, b, 0, c->stream,
+                                   t.view(), arena, st, p0, p2, base, xo);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        auto post_launch = [&](auto pl) {
+            if (dot) launch(I1{}, F_{}, T_{}, pl, part0_, part2_, tb_base_[k]);
+            else launch(I1{}, F_{}, F_{}, pl, (double*)nullptr, (double*)nullptr, 0);
+        };
+        if (!post) launch(I0{}, T_{}, F_{}, std::integral_constant<int, -1>{}, (double*)nullptr, (double*)nullptr, 0);
+        else if (!pro_) post_launch(std::integral_constant<int, -1>{});
+        else if (nat) post_launch(std::integral_constant<int, NATURAL>{});
+        else post_launch(std::integral_constant<int, REDBLACK>{});
     };
     go(TbLarge{}, 0);
     go(TbMedium{}, 1);
     go(TbSmall{}, 2);
-    const Table& t2 = tb_[l][3];  // 2D grids
-    if (t2.nbt > 0) {
-        using C = Tb2D;
-        const dim3 g(t2.nbt), b(C::NT);
-        if (!post)
-            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 0, true, false>), g, b, 0, c->stream,
-                               t2.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
-        else if (dot)
-            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 1, false, true>), g, b, 0, c->stream,
-                               t2.view(), arena, st, part0_, part2_, tb_base_[3]);
-        else
-            hipLaunchKernelGGL((kb_smooth_tb2<C::TXP, C::TY, C::NT, C::WPE, 1, false, false>), g, b, 0, c->stream,
-                               t2.view(), arena, st, (double*)nullptr, (double*)nullptr, 0);
-    }
+    go(Tb2D{}, 3);  // 2D grids
 }
 
 pano_status BatchCore::vcycle(bool with_dot) {

@@ -74,6 +74,7 @@ struct HGrid {
     int64_t o_rho[2], o_u[2];
     int64_t o_x, o_r, o_z, o_p, o_res;
     int64_t o_zl[16], o_rl[16];
+    int64_t o_wl[16];  // levels above the coarsest: the down leg's z (pre-smoothing output)
     int64_t o_t0;      // FP64 coarse temp
     int64_t bytes;     // total bytes of this grid
     const double *Vx, *Vy, *Vz, *lx, *ly, *lz;  // coarsest eigenbases
@@ -161,6 +162,10 @@ private:
     bool any3d_ = false;
     bool split_first_[16] = {};  // first sweep as red cells + black half-sweep at this level
     bool tb_enabled_ = true;      // one-pass smoothing on 3D levels (PANO_MG_FUSED=0 turns it off)
+    // the prolongation inside the one-pass post-smoothing (PANO_MG_PRO=0 turns
+    // it off): those grids' pre-smoothing writes o_wl[l], which the residual
+    // and the post-smoothing read, and they have no prolongation launch
+    bool pro_ = true;
     int tb_base_[kTbConfigs] = {};  // first partial slot of each configuration's level-0 launch
     size_t fdm_lds_ = 0;
     double* part0_ = nullptr;
