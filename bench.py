@@ -444,7 +444,8 @@ def bench_batch(args, **_):
         "config": {"workload": wl["name"], "grids": len(grids), "cells": cells,
                    "sizes": sorted(set(g["dims"][0] for g in grids)),
                    "precond": "multigrid 4 levels, 2+2 red-black GS", "tol": "1e-6 rel L-inf",
-                   "tiles": {"natural": list(tiles["nat"]), "red_black": list(tiles["rb"])},
+                   "tiles": {"natural": list(tiles["nat"]), "red_black": list(tiles["rb"]),
+                             "one_pass_2d": ["32x32", "32x16", "16x16", "64x16"][tiles["tb"]]},
                    "cg_iterations_max_per_step": its.max(axis=1).tolist()},
         "roofline": {"bound": "hbm", "kernel": "whole step (model bytes / wall time)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -253,6 +253,8 @@ typedef struct {
     int32_t autotune;        /* time the candidate tile shapes and keep the fastest */
     int32_t tile_nat[2];     /* threads (x, y) of natural-layout tiles when not autotuning (x*y = 256) */
     int32_t tile_rb[2];      /* threads (x, y) of red-black pair tiles when not autotuning */
+    int32_t tile_tb;         /* one-pass smoothing tiles of 2D levels when not autotuning: 0..3 for
+                                32x32, 32x16, 16x16, 64x16 x-pairs by rows */
 } pano_batch_params;
 pano_status pano_batch_create(pano_ctx, const pano_batch_grid* grids, int32_t n, const pano_batch_params* params,
                               pano_batch* out);
@@ -263,9 +265,12 @@ pano_status pano_batch_get_state(pano_batch, int32_t i, float* density, float* f
 /* One advect+project step of every grid; iterations (optional, n entries)
  * receives each grid's CG iteration count. Synchronous. */
 pano_status pano_batch_step(pano_batch, int32_t* iterations);
-/* The tile shapes in use and the autotuner's timings (us, 2 families x 4
- * candidates: 64x4, 32x8, 128x2, 16x16). */
-pano_status pano_batch_tiles(pano_batch, int32_t* tile_nat, int32_t* tile_rb, double* tune_us);
+/* The tile shapes in use (any output may be NULL) and the autotuner's
+ * timings (us, 12 entries: natural and red-black families x 4 candidates
+ * 64x4, 32x8, 128x2, 16x16 threads, then the 2D one-pass family x 4
+ * candidates 32x32, 32x16, 16x16, 64x16 x-pairs by rows; 0 when not
+ * autotuned). */
+pano_status pano_batch_tiles(pano_batch, int32_t* tile_nat, int32_t* tile_rb, int32_t* tile_tb, double* tune_us);
 
 /* ---- serving: continuous batching of projection solves ---------------------- */
 /* No reference counterpart. A request is one projection of a velocity field

@@ -368,10 +368,11 @@ class Batch:
     """Many independent uneven grids advanced by one set of launches
     (pano_batch_*): grids = [dict(dims=(y, x) or (z, y, x), bc=..., dt=...)].
     f32 fields, MG-PCG to a relative L-inf tolerance, steps captured as
-    hipGraphs, tile shapes autotuned or given."""
+    hipGraphs, tile shapes autotuned or given (tile_tb: the 2D one-pass
+    smoothing tiles, 0..3 for 32x32, 32x16, 16x16, 64x16 x-pairs by rows)."""
 
     def __init__(self, grids, mg_levels=4, mg_nu=2, tol=1e-6, max_iterations=500, use_graph=True, autotune=False,
-                 tile_nat=(64, 4), tile_rb=(64, 4), ctx=None):
+                 tile_nat=(64, 4), tile_rb=(64, 4), tile_tb=0, ctx=None):
         self.ctx = ctx or default_context()
         self.grids = [dict(g) for g in grids]
         arr = (_lib.BatchGrid * len(grids))()
@@ -389,6 +390,7 @@ class Batch:
         P.use_graph, P.autotune = int(use_graph), int(autotune)
         P.tile_nat[0], P.tile_nat[1] = tile_nat
         P.tile_rb[0], P.tile_rb[1] = tile_rb
+        P.tile_tb = int(tile_tb)
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().pano_batch_create(self.ctx.handle, arr, len(grids), ctypes.byref(P), ctypes.byref(h)))
         self.handle = h
@@ -417,9 +419,9 @@ class Batch:
         return list(it)
 
     def tiles(self):
-        tn, tr, us = (ctypes.c_int32 * 2)(), (ctypes.c_int32 * 2)(), (ctypes.c_double * 8)()
-        _lib.check(_lib.lib().pano_batch_tiles(self.handle, tn, tr, us))
-        return {"nat": tuple(tn), "rb": tuple(tr), "tune_us": list(us)}
+        tn, tr, tt, us = (ctypes.c_int32 * 2)(), (ctypes.c_int32 * 2)(), ctypes.c_int32(), (ctypes.c_double * 12)()
+        _lib.check(_lib.lib().pano_batch_tiles(self.handle, tn, tr, ctypes.byref(tt), us))
+        return {"nat": tuple(tn), "rb": tuple(tr), "tb": tt.value, "tune_us": list(us)}
 
     def close(self):
         if getattr(self, "handle", None):

@@ -71,7 +71,8 @@ class BatchGrid(ctypes.Structure):
 
 class BatchParams(ctypes.Structure):
     _fields_ = [("mg_levels", c_i32), ("mg_nu", c_i32), ("tol", c_f64), ("max_iterations", c_i32),
-                ("use_graph", c_i32), ("autotune", c_i32), ("tile_nat", c_i32 * 2), ("tile_rb", c_i32 * 2)]
+                ("use_graph", c_i32), ("autotune", c_i32), ("tile_nat", c_i32 * 2), ("tile_rb", c_i32 * 2),
+                ("tile_tb", c_i32)]
 
 
 class ServerParams(ctypes.Structure):
@@ -136,7 +137,7 @@ _SIGS = {
     "pano_batch_set_state": ([c_void_p, c_i32, c_void_p, c_void_p], c_i32),
     "pano_batch_get_state": ([c_void_p, c_i32, c_void_p, c_void_p], c_i32),
     "pano_batch_step": ([c_void_p, _P(c_i32)], c_i32),
-    "pano_batch_tiles": ([c_void_p, _P(c_i32), _P(c_i32), _P(c_f64)], c_i32),
+    "pano_batch_tiles": ([c_void_p, _P(c_i32), _P(c_i32), _P(c_i32), _P(c_f64)], c_i32),
     "pano_server_create": ([c_void_p, _P(ServerParams), _P(c_void_p)], c_i32),
     "pano_server_destroy": ([c_void_p], c_i32),
     "pano_server_submit": ([c_void_p, _P(ServerRequest), _P(c_i64)], c_i32),

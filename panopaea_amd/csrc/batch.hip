@@ -322,7 +322,7 @@ template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int P
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb2(
     BTab T, char* A, const SolverState* st, double* p0, double* p2, int base, int xo) {
     BATCH_ENTRY(T);
-    if (st[E.sidx].done) return;
+    if (st && st[E.sidx].done) return;  // st null: tile-shape timing
     __shared__ float lds[Tb2Thread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int bx = lb % E.G.gx, by = lb / E.G.gx;
     double acc[2];
@@ -352,7 +352,7 @@ template <int TXP, int TY, int NT, int WPE, int CF, bool ZERO_X, bool DOT, int P
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void kb_smooth_tb(
     BTab T, char* A, const SolverState* st, double* p0, double* p2, int base, int xo) {
     BATCH_ENTRY(T);
-    if (st[E.sidx].done) return;
+    if (st && st[E.sidx].done) return;  // st null: tile-shape timing
     __shared__ float lds[TbThread<4, TXP, TY, NT, CF, ZERO_X, DOT, PL>::LDS_FLOATS];
     const int bx = lb % E.G.gx, q = lb / E.G.gx, by = q % E.G.gy, bz = q / E.G.gy;
     double acc[2];
@@ -710,8 +710,9 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 if (tb) {
                     const int k = tb_config(L);
                     BEntry t = r;
-                    const TbGrid tg = L.d <= 1 ? tb2_grid(L)
-                                               : with_tb(L, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(L); });
+                    const TbGrid tg =
+                        L.d <= 1 ? with_tb2(shapes.tb2, [&](auto cfg) { return tb2_grid(L, cfg.TXP, cfg.TY); })
+                                 : with_tb(L, [&](auto cfg) { return tb_grid_of<decltype(cfg)>(L); });
                     t.G.gx = tg.gx;
                     t.G.gy = tg.gy;
                     t.G.gz = tg.nch;
@@ -928,8 +929,7 @@ pano_status BatchCore::enqueue_project(int ping) {
 // half-sweeps write the partials of z.r (black into part0_, red into part2_).
 // One-pass smoothing launches of a level (pre: from z = 0; post: from the
 // prolonged z, with the z.r partials when dot) for each configuration present.
-void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
-    const SolverState* st = states;
+void BatchCore::smooth_tb_level(int l, bool post, bool dot, const SolverState* st) {
     // with the prolongation in the post-smoothing (pro_) the pre-smoothing
     // writes the down leg's z (o[0]) and the post-smoothing adds P z_{l+1}
     // (natural layout when level l+1 is the coarsest) to it
@@ -937,7 +937,8 @@ void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
     const bool nat = l + 2 == levels_;
     auto go = [&](auto cfg, int k) {
         using C = decltype(cfg);
-        constexpr bool TWO_D = std::is_same<C, Tb2D>::value;
+        constexpr bool TWO_D = std::is_same<C, Tb2D>::value || std::is_same<C, Tb2D_32x16>::value ||
+                               std::is_same<C, Tb2D_16x16>::value || std::is_same<C, Tb2D_64x16>::value;
         const Table& t = tb_[l][k];
         if (t.nbt == 0) return;
         const dim3 g(t.nbt), b(C::NT);
@@ -967,14 +968,17 @@ void BatchCore::smooth_tb_level(int l, bool post, bool dot) {
     go(TbLarge{}, 0);
     go(TbMedium{}, 1);
     go(TbSmall{}, 2);
-    go(Tb2D{}, 3);  // 2D grids
+    with_tb2(shapes.tb2, [&](auto cfg) {  // 2D grids
+        go(cfg, 3);
+        return 0;
+    });
 }
 
 pano_status BatchCore::vcycle(bool with_dot) {
     const int Lc = levels_ - 1;
     dim3 blk(shapes.rb_x, shapes.rb_y);
     for (int l = 0; l < Lc; ++l) {
-        smooth_tb_level(l, false, false);
+        smooth_tb_level(l, false, false, states);
         if (split_first_[l]) {  // bandwidth-bound levels: red cells, then the black half-sweep
             BL(kb_red_init, rbs_[l], arena, (const SolverState*)states);
             BL(kb_smooth_half, rbs_[l], arena, 1, (double*)nullptr, (const SolverState*)states);
@@ -1015,7 +1019,7 @@ pano_status BatchCore::vcycle(bool with_dot) {
     for (int l = Lc - 1; l >= 0; --l) {
         if (l + 1 == Lc) BL(kb_prolong<NATURAL>, prl_[l], arena, (const SolverState*)states);
         else BL(kb_prolong<REDBLACK>, prl_[l], arena, (const SolverState*)states);
-        smooth_tb_level(l, true, l == 0 && with_dot);
+        smooth_tb_level(l, true, l == 0 && with_dot, states);
         for (int k = 0; k < nu; ++k) {
             bool last = l == 0 && k == nu - 1 && with_dot;
             BL(kb_smooth_half, rbs_[l], arena, 1, last ? part0_ : (double*)nullptr, (const SolverState*)states);
@@ -1062,6 +1066,13 @@ pano_status BatchCore::enqueue_iteration() {
     return PANO_OK;
 }
 
+pano_status BatchCore::tune_smooth_tb() {
+    smooth_tb_level(0, false, false, nullptr);
+    smooth_tb_level(0, true, true, nullptr);
+    PANO_HIP(hipGetLastError());
+    return PANO_OK;
+}
+
 pano_status BatchCore::tune_smooth() {
     dim3 blk(shapes.rb_x, shapes.rb_y);
     BL(kb_smooth_half, rb_[0], arena, 0, (double*)nullptr, (const SolverState*)nullptr);
@@ -1084,7 +1095,7 @@ struct Batch {
     hipGraphExec_t pre[2] = {nullptr, nullptr}, iter = nullptr, post[2] = {nullptr, nullptr};
     std::vector<SolverState> hstates;
     int hint = 0;
-    double tune_us[2][8];
+    double tune_us[3][8];
     int tune_n = 0;
 };
 
@@ -1132,9 +1143,10 @@ static pano_status build_graphs(Batch* B) {
 }
 
 // Times one launch family for a candidate tile shape: 5 replays of the
-// level-0 half-sweep (red-black family) or of the density advection and
-// projection (natural family), measured with HIP events. The kernels run
-// without solver states, on whatever the fields hold.
+// density advection and projection (family 0, natural tiles), of the level-0
+// half-sweep (1, red-black tiles) or of the level-0 one-pass pre- and
+// post-smoothing (2, the 2D one-pass tiles), measured with HIP events. The
+// kernels run without solver states, on whatever the fields hold.
 static pano_status time_shape(Batch* B, int family, double* us) {
     Ctx* c = B->c;
     PANO_CHECK(B->core.build(B->grids, B->ids));
@@ -1146,8 +1158,10 @@ static pano_status time_shape(Batch* B, int family, double* us) {
         if (family == 0) {
             PANO_CHECK(K.enqueue_advect(0));
             PANO_CHECK(K.enqueue_project(1));
-        } else {
+        } else if (family == 1) {
             PANO_CHECK(K.tune_smooth());
+        } else {
+            PANO_CHECK(K.tune_smooth_tb());
         }
         return PANO_OK;
     };
@@ -1168,17 +1182,23 @@ static const int kShapes[4][2] = {{64, 4}, {32, 8}, {128, 2}, {16, 16}};
 
 static pano_status autotune(Batch* B) {
     B->tune_n = 4;
-    for (int fam = 0; fam < 2; ++fam) {
+    TileShapes& T = B->core.shapes;
+    for (int fam = 0; fam < 3; ++fam) {
+        auto set = [&](int k) {
+            if (fam == 0) {
+                T.nat_x = kShapes[k][0];
+                T.nat_y = kShapes[k][1];
+            } else if (fam == 1) {
+                T.rb_x = kShapes[k][0];
+                T.rb_y = kShapes[k][1];
+            } else {
+                T.tb2 = k;
+            }
+        };
         int best = 0;
         double best_us = 1e300;
         for (int k = 0; k < 4; ++k) {
-            if (fam == 0) {
-                B->core.shapes.nat_x = kShapes[k][0];
-                B->core.shapes.nat_y = kShapes[k][1];
-            } else {
-                B->core.shapes.rb_x = kShapes[k][0];
-                B->core.shapes.rb_y = kShapes[k][1];
-            }
+            set(k);
             double us;
             PANO_CHECK(time_shape(B, fam, &us));
             B->tune_us[fam][k] = us;
@@ -1187,13 +1207,7 @@ static pano_status autotune(Batch* B) {
                 best = k;
             }
         }
-        if (fam == 0) {
-            B->core.shapes.nat_x = kShapes[best][0];
-            B->core.shapes.nat_y = kShapes[best][1];
-        } else {
-            B->core.shapes.rb_x = kShapes[best][0];
-            B->core.shapes.rb_y = kShapes[best][1];
-        }
+        set(best);
     }
     // the timing ran kernels over the uninitialised fields: clear them
     PANO_CHECK(fill_zero(B->c, B->arena, (size_t)B->arena_bytes));
@@ -1232,10 +1246,15 @@ pano_status pano_batch_create(pano_ctx ctx, const pano_batch_grid* grids, int32_
             delete h;
             return fail(PANO_ERR_INVALID, "tile shapes must have 256 threads");
         }
+        if (P->tile_tb < 0 || P->tile_tb >= kTb2Shapes) {
+            delete h;
+            return fail(PANO_ERR_INVALID, "one-pass tile index must be 0..%d", kTb2Shapes - 1);
+        }
         B->core.shapes.nat_x = a[0];
         B->core.shapes.nat_y = a[1];
         B->core.shapes.rb_x = a[2];
         B->core.shapes.rb_y = a[3];
+        B->core.shapes.tb2 = P->tile_tb;
     }
     int64_t off = 0;
     std::vector<SolveCfg> cfg(n);
@@ -1387,7 +1406,8 @@ pano_status pano_batch_step(pano_batch h, int32_t* iterations) {
     return conv ? PANO_OK : PANO_NOT_CONVERGED;
 }
 
-pano_status pano_batch_tiles(pano_batch h, int32_t* tile_nat, int32_t* tile_rb, double* tune_us) {
+pano_status pano_batch_tiles(pano_batch h, int32_t* tile_nat, int32_t* tile_rb, int32_t* tile_tb,
+                             double* tune_us) {
     if (!h) return fail(PANO_ERR_INVALID, "batch is null");
     Batch* B = &h->b;
     if (tile_nat) {
@@ -1398,8 +1418,9 @@ pano_status pano_batch_tiles(pano_batch h, int32_t* tile_nat, int32_t* tile_rb, 
         tile_rb[0] = B->core.shapes.rb_x;
         tile_rb[1] = B->core.shapes.rb_y;
     }
+    if (tile_tb) *tile_tb = B->core.shapes.tb2;
     if (tune_us)
-        for (int f = 0; f < 2; ++f)
+        for (int f = 0; f < 3; ++f)
             for (int k = 0; k < 4; ++k) tune_us[f * 4 + k] = B->tune_n ? B->tune_us[f][k] : 0.0;
     return PANO_OK;
 }

@@ -80,10 +80,11 @@ struct HGrid {
     const double *Vx, *Vy, *Vz, *lx, *ly, *lz;  // coarsest eigenbases
 };
 
-// Tile shapes (threads per workgroup along x / y) of the two launch families.
+// Tile shapes of the three launch families.
 struct TileShapes {
-    int nat_x = 64, nat_y = 4;  // natural-layout cell and face tiles
-    int rb_x = 64, rb_y = 4;    // red-black x-pair tiles
+    int nat_x = 64, nat_y = 4;  // natural-layout cell and face tiles (threads along x / y)
+    int rb_x = 64, rb_y = 4;    // red-black x-pair tiles (threads along x / y)
+    int tb2 = 0;                // one-pass smoothing tiles of 2D levels (with_tb2 index)
 };
 
 // Byte size of a grid's fields and their offsets from `base`. A projection
@@ -122,6 +123,8 @@ public:
     pano_status ensure_slots(int n);
     // Level-0 red-black sweeps without solver states (tile-shape timing only).
     pano_status tune_smooth();
+    // Level-0 one-pass pre- and post-smoothing without solver states (timing only).
+    pano_status tune_smooth_tb();
 
 private:
     // A launch table lives in the core's one device table buffer: its
@@ -183,7 +186,7 @@ private:
     pano_status ensure_partials(size_t n);
     void fin(int op, const double* p0, const double* p1, const Table& t, int use_max);
     pano_status vcycle(bool with_dot);
-    void smooth_tb_level(int l, bool post, bool dot);
+    void smooth_tb_level(int l, bool post, bool dot, const SolverState* st);
 };
 
 }  // namespace pano

@@ -731,6 +731,29 @@ struct TbSmall {
 struct Tb2D {
     static constexpr int TXP = 32, TY = 32, NT = 768, WPE = 8;
 };
+// The batched engine's other 2D tiles (its autotuner picks one of Tb2D and
+// these): 32 x 16, 16 x 16 and 64 x 16 x-pairs by rows, one thread per
+// position pair (NT >= (TY/2 + 4) * (TXP + 4)).
+struct Tb2D_32x16 {
+    static constexpr int TXP = 32, TY = 16, NT = 448, WPE = 8;
+};
+struct Tb2D_16x16 {
+    static constexpr int TXP = 16, TY = 16, NT = 256, WPE = 8;
+};
+struct Tb2D_64x16 {
+    static constexpr int TXP = 64, TY = 16, NT = 832, WPE = 8;
+};
+constexpr int kTb2Shapes = 4;  // Tb2D, Tb2D_32x16, Tb2D_16x16, Tb2D_64x16
+// Calls f with the 2D tile configuration of index k (0 .. kTb2Shapes-1).
+template <class F>
+inline auto with_tb2(int k, F&& f) {
+    switch (k) {
+        case 1: return f(Tb2D_32x16{});
+        case 2: return f(Tb2D_16x16{});
+        case 3: return f(Tb2D_64x16{});
+        default: return f(Tb2D{});
+    }
+}
 constexpr int kTbConfigs = 4;  // index: 0 large, 1 medium, 2 small (3D), 3 2D
 
 template <class C>
@@ -751,10 +774,10 @@ inline int tb_config(const Dims& D) {
     return n >= (int64_t(1) << 23) ? 0 : (n >= (int64_t(1) << 20) ? 1 : 2);
 }
 // 2D tiles of a level: x-pair tiles by row tiles (TbGrid with one chunk)
-inline TbGrid tb2_grid(const Dims& D) {
+inline TbGrid tb2_grid(const Dims& D, int txp = Tb2D::TXP, int ty = Tb2D::TY) {
     TbGrid g;
-    g.gx = ((D.w >> 1) + Tb2D::TXP - 1) / Tb2D::TXP;
-    g.gy = (D.h + Tb2D::TY - 1) / Tb2D::TY;
+    g.gx = ((D.w >> 1) + txp - 1) / txp;
+    g.gy = (D.h + ty - 1) / ty;
     g.zc = 1;
     g.nch = 1;
     g.nb = g.gx * g.gy;

@@ -296,6 +296,14 @@ int main() {
         fails += run2<32, 32, 768, 1, false, false, REDBLACK>(D, seed++);
         fails += run2<8, 4, 64, 1, false, true, REDBLACK>(D, seed++);
         fails += run2<16, 16, 128, 1, false, false, NATURAL>(D, seed++);
+        // the batched engine's other 2D tiles (32x16 / 448, 16x16 / 256, 64x16 / 832)
+        fails += run2<32, 16, 448, 0, true, false>(D, seed++);
+        fails += run2<32, 16, 448, 1, false, true, REDBLACK>(D, seed++);
+        fails += run2<16, 16, 256, 0, true, false>(D, seed++);
+        fails += run2<16, 16, 256, 1, false, true, NATURAL>(D, seed++);
+        fails += run2<64, 16, 832, 0, true, false>(D, seed++);
+        fails += run2<64, 16, 832, 1, false, true, REDBLACK>(D, seed++);
+        fails += run2<64, 16, 832, 1, false, false, -1>(D, seed++);
     }
     printf("%s (%d failing cases)\n", fails ? "FAIL" : "OK", fails);
     return fails != 0;

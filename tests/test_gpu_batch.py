@@ -57,10 +57,12 @@ def test_gpu_batch_alone_equals_in_mix_autotuned(ctx):
     B = pa.Batch(GRIDS, autotune=True)
     t = B.tiles()
     B.close()
-    assert len(t["tune_us"]) == 8 and all(v > 0 for v in t["tune_us"])
+    assert len(t["tune_us"]) == 12 and all(v > 0 for v in t["tune_us"])
+    assert 0 <= t["tb"] < 4
     for k in (1, 2):
-        alone, _ = run_batch([GRIDS[k]], 2, seeds=[100 + k], autotune=False, tile_nat=t["nat"], tile_rb=t["rb"])
-        mixed, _ = run_batch(GRIDS, 2, autotune=False, tile_nat=t["nat"], tile_rb=t["rb"])
+        alone, _ = run_batch([GRIDS[k]], 2, seeds=[100 + k], autotune=False, tile_nat=t["nat"], tile_rb=t["rb"],
+                             tile_tb=t["tb"])
+        mixed, _ = run_batch(GRIDS, 2, autotune=False, tile_nat=t["nat"], tile_rb=t["rb"], tile_tb=t["tb"])
         assert np.array_equal(alone[0][0], mixed[k][0]) and np.array_equal(alone[0][1], mixed[k][1])
 
 
@@ -140,3 +142,23 @@ def test_gpu_batch64_full_workload_properties(ctx):
         scale = np.abs(faces).max()
         assert np.abs(div).max() <= 1e-6 * scale, (k, g["dims"], np.abs(div).max(), scale)
     B.close()
+
+
+@pytest.mark.parametrize("tile_tb", [1, 2, 3])
+def test_gpu_batch_one_pass_tiles(ctx, tile_tb):
+    """Every 2D one-pass smoothing tile shape the autotuner may pick: the same
+    steps as the default 32x32 tiles (the smoothing's values do not depend on
+    the tiles; only the order in which the z.r partials of the tiles are
+    added does, so the solves agree to rounding), and the grid alone equals
+    the grid in the mix bit for bit."""
+    ref, its_ref = run_batch(GRIDS, 2, tile_tb=0)
+    out, its = run_batch(GRIDS, 2, tile_tb=tile_tb)
+    for s in range(2):
+        for k in range(len(GRIDS)):
+            assert abs(its[s][k] - its_ref[s][k]) <= 1
+    for (r0, v0), (r1, v1) in zip(ref, out):
+        assert np.all(np.isfinite(r1)) and np.all(np.isfinite(v1))
+        assert np.abs(r1 - r0).max() <= 1e-4 * max(np.abs(r0).max(), 1e-30)
+        assert np.abs(v1 - v0).max() <= 1e-4 * max(np.abs(v0).max(), 1e-30)
+    alone, _ = run_batch([GRIDS[1]], 2, seeds=[101], tile_tb=tile_tb)
+    assert np.array_equal(alone[0][0], out[1][0]) and np.array_equal(alone[0][1], out[1][1])
