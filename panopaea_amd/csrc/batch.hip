@@ -835,6 +835,10 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
         }
     }
     fdm_lds_ = kFdmLdsBytes;
+    // 2D coarsest passes: the largest need of the grids in the launch
+    rows2_lds_ = cols2_lds_ = 0;
+    for (const BFdm& F : rows2_.h) rows2_lds_ = std::max(rows2_lds_, fdm2_rows_lds(F.nx));
+    for (const BFdm& F : cols2_.h) cols2_lds_ = std::max(cols2_lds_, fdm2_cols_lds(F.ny));
     // pack every table into the staging buffer, then one copy
     size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
                    table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
@@ -1001,11 +1005,11 @@ pano_status BatchCore::vcycle(bool with_dot) {
     }
     if (!rows2_.h.empty()) {
         const SolverState* cst = states;
-        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), fdm_lds_, c->stream, rows2_.view(), arena, 0,
+        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), rows2_lds_, c->stream, rows2_.view(), arena, 0,
                            cst);
-        hipLaunchKernelGGL(kb_fdm2_cols, dim3(cols2_.nbt), dim3(256), fdm_lds_, c->stream, cols2_.view(), arena,
+        hipLaunchKernelGGL(kb_fdm2_cols, dim3(cols2_.nbt), dim3(256), cols2_lds_, c->stream, cols2_.view(), arena,
                            cst);
-        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), fdm_lds_, c->stream, rows2_.view(), arena, 2,
+        hipLaunchKernelGGL(kb_fdm2_rows, dim3(rows2_.nbt), dim3(256), rows2_lds_, c->stream, rows2_.view(), arena, 2,
                            cst);
     }
     if (!plane_.h.empty()) {

@@ -171,6 +171,7 @@ private:
     bool pro_ = true;
     int tb_base_[kTbConfigs] = {};  // first partial slot of each configuration's level-0 launch
     size_t fdm_lds_ = 0;
+    size_t rows2_lds_ = 0, cols2_lds_ = 0;  // dynamic LDS of the 2D coarsest passes
     double* part0_ = nullptr;
     double* part1_ = nullptr;
     double* part2_ = nullptr;

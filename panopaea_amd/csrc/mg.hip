@@ -760,12 +760,12 @@ static pano_status coarse_solve(Ctx* c, MG<T>* m, T* z, const T* r, const Solver
     const double *lx = m->fdm_lam[0], *ly = m->fdm_lam[1], *lz = m->fdm_lam[2];
     if (DIM == 2 && nx <= kFdmPlane && ny <= kFdmPlane) {
         // rows (forward x), columns (y, divide, inverse y), rows (inverse x)
-        size_t lds = kFdmLdsBytes;
+        const size_t lr = fdm2_rows_lds(nx), lc = fdm2_cols_lds(ny);
         unsigned nr = (unsigned)((ny + fdm2_rows_per(nx) - 1) / fdm2_rows_per(nx));
         unsigned ncol = (unsigned)((nx + fdm2_cols_per(ny) - 1) / fdm2_cols_per(ny));
-        hipLaunchKernelGGL((k_fdm2_rows<T, double>), dim3(nr), dim3(256), lds, S, nx, ny, 0, r, t0, Vx, st);
-        hipLaunchKernelGGL(k_fdm2_cols, dim3(ncol), dim3(256), lds, S, nx, ny, t0, Vy, lx, ly, s, eps, st);
-        hipLaunchKernelGGL((k_fdm2_rows<double, T>), dim3(nr), dim3(256), lds, S, nx, ny, 1, (const double*)t0, z,
+        hipLaunchKernelGGL((k_fdm2_rows<T, double>), dim3(nr), dim3(256), lr, S, nx, ny, 0, r, t0, Vx, st);
+        hipLaunchKernelGGL(k_fdm2_cols, dim3(ncol), dim3(256), lc, S, nx, ny, t0, Vy, lx, ly, s, eps, st);
+        hipLaunchKernelGGL((k_fdm2_rows<double, T>), dim3(nr), dim3(256), lr, S, nx, ny, 1, (const double*)t0, z,
                            Vx, st);
         PANO_HIP(hipGetLastError());
         return PANO_OK;

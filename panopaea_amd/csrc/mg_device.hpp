@@ -605,8 +605,20 @@ __device__ __forceinline__ void fdm_plane_body(double* lds, int nx, int ny, int 
 // Rows per workgroup: fdm2_rows_per(nx); columns per workgroup: fdm2_cols_per(ny).
 __host__ __device__ inline int fdm2_rows_per(int nx) { return nx >= 256 ? 1 : 256 / nx; }
 __host__ __device__ inline int fdm2_cols_per(int ny) { return ny >= 256 ? 1 : 256 / ny; }
+// Dynamic LDS bytes of the passes: the workgroup's lines, then the eigenbasis
+// when it is staged (stage_basis), so small levels leave room for several
+// workgroups per CU.
+__host__ __device__ inline size_t fdm2_rows_lds(int nx) {
+    const size_t R = fdm2_rows_per(nx);
+    return (R * nx + (nx * nx <= kFdmVStage ? (size_t)nx * nx : 0)) * sizeof(double);
+}
+__host__ __device__ inline size_t fdm2_cols_lds(int ny) {
+    const size_t C = fdm2_cols_per(ny);
+    return (2 * C * ny + (ny * ny <= kFdmVStage ? (size_t)ny * ny : 0)) * sizeof(double);
+}
 
-// rows [blk*R, blk*R + R) of pass 0 (Tin -> double t) or pass 2 (double t -> Tout); lds: R*nx doubles
+// rows [blk*R, blk*R + R) of pass 0 (Tin -> double t) or pass 2 (double t -> Tout); lds: R*nx doubles,
+// then the staged basis (fdm2_rows_lds)
 template <typename Tin, typename Tout>
 __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int blk, int inverse, const Tin* src,
                                                Tout* dst, const double* Vx) {
@@ -616,8 +628,8 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     const int P = nr * nx;
     lds_fill(lds, P, [&](int e) { return (double)src[(int64_t)j0 * nx + e]; });
     // inverse: the staged transpose W (W[k*nx+i] = Vx[i*nx+k]) when it fits
-    const double* W = inverse ? stage_basis_t(lds + 2 * kFdmPlane, 0, Vx, nx) : nullptr;
-    if (!W) Vx = stage_basis(lds + 2 * kFdmPlane, 0, Vx, nx);
+    const double* W = inverse ? stage_basis_t(lds + R * nx, 0, Vx, nx) : nullptr;
+    if (!W) Vx = stage_basis(lds + R * nx, 0, Vx, nx);
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int jl = e / nx, i = e - jl * nx;
@@ -633,7 +645,8 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     }
 }
 
-// columns [blk*C, blk*C + C) of pass 1, in place on t; lds: 2*ny*C doubles
+// columns [blk*C, blk*C + C) of pass 1, in place on t; lds: 2*ny*C doubles, then the staged basis
+// (fdm2_cols_lds)
 __device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int blk, double* t, const double* Vy,
                                                const double* lx, const double* ly, double s, double eps) {
     const int C = fdm2_cols_per(ny);
@@ -655,7 +668,7 @@ __device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int 
             if (e < P) a[j * C + ci] = v[u];
         }
     }
-    Vy = stage_basis(lds + 2 * kFdmPlane, 0, Vy, ny);
+    Vy = stage_basis(lds + 2 * ny * C, 0, Vy, ny);
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int j = e / nc, ci = e - j * nc;
