@@ -186,6 +186,34 @@ __device__ __forceinline__ double block_max(double v) {
     return s;
 }
 
+// The same two reductions for a one-dimensional workgroup (blockDim.y =
+// blockDim.z = 1): identical arithmetic, without reading the workgroup's
+// other extents (a memory load in the middle of the kernel).
+__device__ __forceinline__ double block_sum_x(double v) {
+    __shared__ double redx_s[16];
+    const int tid = threadIdx.x;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((tid & 63) == 0) redx_s[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += redx_s[i];
+    return s;
+}
+__device__ __forceinline__ double block_max_x(double v) {
+    __shared__ double redmx_s[16];
+    const int tid = threadIdx.x;
+    v = wave_max(v);
+    __syncthreads();
+    if ((tid & 63) == 0) redmx_s[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s = nan_max(s, redmx_s[i]);
+    return s;
+}
+
 // NaN-propagating |x| max for norms: a NaN anywhere makes the norm NaN.
 __host__ __device__ __forceinline__ double absmax_acc(double m, double x) {
     double a = fabs(x);

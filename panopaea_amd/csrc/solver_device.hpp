@@ -25,27 +25,44 @@ struct SolveCfg {
 
 // Reduces the FP64 partials part0[0..np) (+ part1[0..np) when part1 is not
 // null) in a fixed order and applies `op` to the solver state. Run by one
-// workgroup (all threads).
+// one-dimensional workgroup (all threads). The state is read once, at the
+// start, and the done test comes at the end, after the reduction (the
+// partials' loads no longer wait for it): a finished solve (done) changes
+// nothing; its partials are read and left unused.
 __device__ __forceinline__ void finalize_range(int op, const double* part0, const double* part1, int np,
                                                SolverState* st, const SolveCfg& cfg, int use_max) {
-    if (op >= FIN_SIGMA0 && st->done) return;  // uniform: every thread returns
-    // eight independent accumulators per thread (loads in flight), combined in a fixed order
+    const SolverState s0 = *st;
+    // eight independent accumulators per thread, combined in a fixed order; the
+    // loads are unconditional (indices past the end read the last partial and
+    // count as 0), so all of a thread's loads are in flight together
     double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int stride = blockDim.x * 8;
     for (int base = threadIdx.x; base < np; base += stride) {
+        double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            int i = base + u * blockDim.x;
-            double v = i < np ? part0[i] : 0.0;
-            if (part1 && i < np) v += part1[i];
-            a8[u] = use_max ? absmax_acc(a8[u], v) : a8[u] + v;
+            const int i = base + u * blockDim.x;
+            v[u] = part0[i < np ? i : np - 1];
+        }
+        if (part1) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = base + u * blockDim.x;
+                v[u] += part1[i < np ? i : np - 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double w = base + u * (int)blockDim.x < np ? v[u] : 0.0;
+            a8[u] = use_max ? absmax_acc(a8[u], w) : a8[u] + w;
         }
     }
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc = use_max ? absmax_acc(acc, a8[u]) : acc + a8[u];
-    double tot = use_max ? block_max(acc) : block_sum(acc);
+    double tot = use_max ? block_max_x(acc) : block_sum_x(acc);  // one-dimensional workgroup
     if (threadIdx.x != 0) return;
+    if (op >= FIN_SIGMA0 && s0.done) return;
     switch (op) {
         case FIN_STATS:  // tot = sum(b): the mean to remove
             st->mean = tot / (double)cfg.n;
@@ -53,7 +70,7 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
         case FIN_THRESH: {  // tot = ||b||_inf (after mean removal): the solve's initial state
             SolverState s = {};
             s.max_iter = cfg.max_iter;
-            s.mean = cfg.remove_mean ? st->mean : 0.0;  // FIN_STATS ran first when removing it
+            s.mean = cfg.remove_mean ? s0.mean : 0.0;  // FIN_STATS ran first when removing it
             s.b_linf = tot;
             s.rnorm = tot;
             double thr = cfg.tol_kind == PANO_TOL_ABS_LINF ? cfg.tol : cfg.tol * tot;
@@ -75,36 +92,40 @@ __device__ __forceinline__ void finalize_range(int op, const double* part0, cons
         case FIN_SIGMA0:
             st->sigma = tot;
             break;
-        case FIN_ALPHA:
+        case FIN_ALPHA: {
+            const double alpha = s0.sigma / tot;
             st->pq = tot;
-            st->alpha = st->sigma / tot;
-            if (!(st->alpha == st->alpha) || fabs(st->alpha) > 1.7e308) {
+            st->alpha = alpha;
+            if (!(alpha == alpha) || fabs(alpha) > 1.7e308) {
                 st->nonfinite = 1;
                 st->done = 1;
             }
             break;
-        case FIN_RNORM:
+        }
+        case FIN_RNORM: {
+            const int iter = s0.iter + 1;
             st->rnorm = tot;
-            st->iter += 1;
+            st->iter = iter;
             if (!(tot == tot)) {
                 st->nonfinite = 1;
                 st->done = 1;
-            } else if (tot < st->threshold) {  // pcg.rs:58-63
+            } else if (tot < s0.threshold) {  // pcg.rs:58-63
                 st->converged = 1;
                 st->done = 1;
-            } else if (st->iter >= st->max_iter) {
+            } else if (iter >= s0.max_iter) {
                 st->done = 1;
             }
             break;
+        }
         case FIN_BETA: {
-            double sn = tot;
-            st->beta = sn / st->sigma;
-            st->sigma = sn;
-            if (!(st->beta == st->beta)) {
+            const double beta = tot / s0.sigma;
+            st->beta = beta;
+            st->sigma = tot;
+            if (!(beta == beta)) {
                 st->nonfinite = 1;
                 st->done = 1;
             } else {
-                st->xit = st->iter;  // the search that follows adds this update's step to x
+                st->xit = s0.iter;  // the search that follows adds this update's step to x
             }
             break;
         }
