@@ -669,10 +669,18 @@ __global__ __launch_bounds__(256) void k_resid64_rb(Dims D, Grid3 G, float* r32,
 // x += (double)e
 __global__ __launch_bounds__(256) void k_add_correction(int64_t n, double* x, const float* e) {
     int64_t base = (int64_t)blockIdx.x * kFlatPer + threadIdx.x;
+    double xv[kItems];
+    float ev[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // all loads before any store (clamped, valid indices)
+        const int64_t i = base + (int64_t)k * kBlock, j = i < n ? i : n - 1;
+        xv[k] = x[j];
+        ev[k] = e[j];
+    }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t i = base + (int64_t)k * kBlock;
-        if (i < n) x[i] = x[i] + (double)e[i];
+        const int64_t i = base + (int64_t)k * kBlock;
+        if (i < n) x[i] = xv[k] + (double)ev[k];
     }
 }
 

@@ -147,12 +147,18 @@ static __global__ void k_state_init(SolverState* st, int max_iter) {
 template <typename T>
 __device__ __forceinline__ void flat_stats(int lb, int64_t n, const T* r, double* sum, double* mx) {
     const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T rv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // all loads first (clamped, valid indices)
+        const int i = base + k * kBlock;
+        rv[k] = ldx(r, i < n ? i : (int)n - 1);
+    }
     double s = 0.0, m = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
         if (i < n) {
-            double v = (double)ldx(r, i);
+            double v = (double)rv[k];
             s += v;
             m = absmax_acc(m, v);
         }
@@ -164,12 +170,18 @@ __device__ __forceinline__ void flat_stats(int lb, int64_t n, const T* r, double
 template <typename T>
 __device__ __forceinline__ double flat_sub_mean(int lb, int64_t n, T* r, double mean) {
     const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T rv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // all loads before any store (clamped, valid indices)
+        const int i = base + k * kBlock;
+        rv[k] = ldx(r, i < n ? i : (int)n - 1);
+    }
     double m = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
         if (i < n) {
-            T v = (T)((double)ldx(r, i) - mean);
+            T v = (T)((double)rv[k] - mean);
             stx(r, i, v);
             m = absmax_acc(m, (double)v);
         }
@@ -193,9 +205,9 @@ __device__ __forceinline__ void flat_copy(int lb, int64_t n, T* p, const T* z) {
     const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
     T v[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
+    for (int k = 0; k < kItems; ++k) {  // all loads first (clamped, valid indices)
         const int i = base + k * kBlock;
-        if (i < n) v[k] = ldx(z, i);
+        v[k] = ldx(z, i < n ? i : (int)n - 1);
     }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -208,14 +220,21 @@ __device__ __forceinline__ void flat_copy(int lb, int64_t n, T* p, const T* z) {
 template <typename T>
 __device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T* z, const T* r) {
     const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T zv[kItems], rv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // all loads before any store (clamped, valid indices)
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        zv[k] = ldx(z, j);
+        rv[k] = ldx(r, j);
+    }
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
         if (i < n) {
-            T zv = ldx(z, i);
-            stx(p, i, zv);
-            s += (double)zv * (double)ldx(r, i);
+            stx(p, i, zv[k]);
+            s += (double)zv[k] * (double)rv[k];
         }
     }
     return block_sum(s);
@@ -224,11 +243,19 @@ __device__ __forceinline__ double flat_copy_dot(int lb, int64_t n, T* p, const T
 template <typename T>
 __device__ __forceinline__ double flat_dot(int lb, int64_t n, const T* a, const T* b) {
     const int base = lb * kFlatPer + (int)threadIdx.x;  // 32-bit: fields hold < 2^30 elements
+    T av[kItems], bv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // all loads first (clamped, valid indices)
+        const int i = base + k * kBlock;
+        const int j = i < n ? i : (int)n - 1;
+        av[k] = ldx(a, j);
+        bv[k] = ldx(b, j);
+    }
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = base + k * kBlock;
-        if (i < n) s += (double)ldx(a, i) * (double)ldx(b, i);
+        if (i < n) s += (double)av[k] * (double)bv[k];
     }
     return block_sum(s);
 }
