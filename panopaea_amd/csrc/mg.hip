@@ -568,27 +568,11 @@ __host__ __device__ inline int fdm_split_w(int n) {
     return w < 1 ? 1 : w;
 }
 // doubles staged for two bases of sizes n0^2 and n1^2 (the first, then the
-// second if both fit), as fdm_split_stage stages them
+// second if both fit), as the kernels below stage them
 __host__ __device__ inline int fdm_split_cap(int n0, int n1) {
     const int a = n0 * n0 <= kFdmSplitV ? n0 * n0 : 0;
     const int b = a + n1 * n1 <= kFdmSplitV ? n1 * n1 : 0;
     return a + b;
-}
-// V (or its transpose with T) into dst + used when it fits within cap; returns
-// the staged copy or V itself (nullptr for a transpose that does not fit)
-template <bool TRANS>
-__device__ __forceinline__ const double* fdm_split_stage(double* dst, int& used, int cap, const double* V, int n) {
-    if (used + n * n > cap) return TRANS ? nullptr : V;
-    double* d = dst + used;
-    if (TRANS)
-        lds_fill(d, n * n, [&](int e) {
-            const int q = e / n, i = e - q * n;
-            return V[i * n + q];
-        });
-    else
-        lds_fill(d, n * n, [&](int e) { return V[e]; });
-    used += n * n;
-    return d;
 }
 inline size_t fdm_split_lds_fwd(int nx, int ny) {
     return (size_t)(nx * ny + ny * fdm_split_w(ny) + fdm_split_cap(nx, ny)) * sizeof(double);
@@ -614,11 +598,16 @@ __global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_fwd_cols(int nx, int n
     double* t = a + P;     // its x transform, columns i0.. : t[q * cw + il]
     double* vs = t + ny * cw;
     const Tin* src = in + (int64_t)K * P;
-    lds_fill(a, P, [&](int e) { return (double)src[e]; });
-    int used = 0;
+    // the plane and the bases that fit (Vx first, then Vy: fdm_split_cap),
+    // all loads of a round in flight together
     const int cap = fdm_split_cap(nx, ny);
-    const double* vx = fdm_split_stage<false>(vs, used, cap, Vx, nx);
-    const double* vy = fdm_split_stage<false>(vs, used, cap, Vy, ny);
+    const int nsx = nx * nx <= cap ? nx * nx : 0;
+    const int nsy = nsx + ny * ny <= cap ? ny * ny : 0;
+    lds_fill3(
+        a, P, [&](int e) { return (double)src[e]; }, vs, nsx, [&](int e) { return Vx[e]; }, vs + nsx, nsy,
+        [&](int e) { return Vy[e]; });
+    const double* vx = nsx ? vs : Vx;
+    const double* vy = nsy ? vs + nsx : Vy;
     __syncthreads();
     for (int e = threadIdx.x; e < ny * nc; e += blockDim.x) {
         const int q = e / nc, il = e - q * nc, i = i0 + il;
@@ -657,8 +646,9 @@ __global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_z_cols(int nx, int ny,
         const int k = e / cw, il = e - k * cw;
         return data[k * zs + (int64_t)j * nx + i0 + imin(il, nc - 1)];
     });
-    int used = 0;
-    const double* vz = fdm_split_stage<false>(vs, used, fdm_split_cap(nz, 0), Vz, nz);
+    const bool sz = nz * nz <= fdm_split_cap(nz, 0);
+    if (sz) lds_fill(vs, nz * nz, [&](int e) { return Vz[e]; });
+    const double* vz = sz ? vs : Vz;
     __syncthreads();
     for (int e = threadIdx.x; e < nz * nc; e += blockDim.x) {
         const int k = e / nc, il = e - k * nc, i = i0 + il;
@@ -692,11 +682,19 @@ __global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_inv_rows(int nx, int n
     double* u = a + P;     // its inverse y transform, rows j0.. : u[jl * nx + i]
     double* vs = u + rw * nx;
     const double* src = in + (int64_t)K * P;
-    lds_fill(a, P, [&](int e) { return src[e]; });
-    int used = 0;
+    // the plane, Vy and the transpose W[q*nx+i] = Vx[i*nx+q] when they fit
+    // (fdm_split_cap), all loads of a round in flight together
     const int cap = fdm_split_cap(ny, nx);
-    const double* vy = fdm_split_stage<false>(vs, used, cap, Vy, ny);
-    const double* wx = fdm_split_stage<true>(vs, used, cap, Vx, nx);  // W[q*nx+i] = Vx[i*nx+q]
+    const int nsy = ny * ny <= cap ? ny * ny : 0;
+    const int nsw = nsy + nx * nx <= cap ? nx * nx : 0;
+    lds_fill3(
+        a, P, [&](int e) { return src[e]; }, vs, nsy, [&](int e) { return Vy[e]; }, vs + nsy, nsw,
+        [&](int e) {
+            const int q = e / nx, i = e - q * nx;
+            return Vx[i * nx + q];
+        });
+    const double* vy = nsy ? vs : Vy;
+    const double* wx = nsw ? vs + nsy : nullptr;
     __syncthreads();
     for (int e = threadIdx.x; e < nr * nx; e += blockDim.x) {
         const int jl = e / nx, i = e - jl * nx, j = j0 + jl;

@@ -222,28 +222,49 @@ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T d
     const int yl = y > 0 ? y - 1 : y, yr = y + 1 < D.h ? y + 1 : y;
     // index of x_c (c = 0: cell 2mm, 1: cell 2mm+1) of pair mm in row yy of plane zz
     auto at = [&](int zz, int yy, int mm, int c) { return (((yy + zz) & 1) ^ c) * n2 + zz * hw + yy * wh + mm; };
-    T cv[NZ + 2][2], xl[NZ], xr[NZ], ylv[NZ][2], yrv[NZ][2];
+    // every index first, then every load: the compiler forms some of these
+    // 32-bit multiply-adds with an instruction that also reads a (don't-care)
+    // high register, and interleaved with the loads that register can be a
+    // pending load's destination, which makes the rest of the loads wait
+    int ic[NZ + 2][2], il[NZ], ir[NZ], iyl[NZ][2], iyr[NZ][2];
 #pragma unroll
     for (int kk = 0; kk < NZ + 2; ++kk) {
         // plane z0 + kk - 1, clamped into the grid (a missing z neighbour's value is not used)
         int zz = z0 + kk - 1;
         zz = zz < 0 ? 0 : (zz >= D.d ? D.d - 1 : zz);
+        ic[kk][0] = at(zz, y, m, 0);
+        ic[kk][1] = at(zz, y, m, 1);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        const int zz = zclamp(D, z0 + k);
+        il[k] = at(zz, y, ml, 1);
+        ir[k] = at(zz, y, mr, 0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            iyl[k][c] = at(zz, yl, m, c);
+            iyr[k][c] = at(zz, yr, m, c);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    T cv[NZ + 2][2], xl[NZ], xr[NZ], ylv[NZ][2], yrv[NZ][2];
+#pragma unroll
+    for (int kk = 0; kk < NZ + 2; ++kk) {
         if (DIM == 3 || (kk >= 1 && kk <= NZ)) {
-            cv[kk][0] = ldx(p, at(zz, y, m, 0));
-            cv[kk][1] = ldx(p, at(zz, y, m, 1));
+            cv[kk][0] = ldx(p, ic[kk][0]);
+            cv[kk][1] = ldx(p, ic[kk][1]);
         } else {
             cv[kk][0] = cv[kk][1] = (T)0;
         }
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
-        const int zz = zclamp(D, z0 + k);
-        xl[k] = ldx(p, at(zz, y, ml, 1));
-        xr[k] = ldx(p, at(zz, y, mr, 0));
+        xl[k] = ldx(p, il[k]);
+        xr[k] = ldx(p, ir[k]);
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            ylv[k][c] = ldx(p, at(zz, yl, m, c));
-            yrv[k][c] = ldx(p, at(zz, yr, m, c));
+            ylv[k][c] = ldx(p, iyl[k][c]);
+            yrv[k][c] = ldx(p, iyr[k][c]);
         }
     }
     double s = 0.0;
@@ -455,6 +476,37 @@ __device__ __forceinline__ void lds_fill(double* dst, int n, Src src) {
         for (int u = 0; u < U; ++u) {
             const int e = base + u * blockDim.x;
             if (e < n) dst[e] = v[u];
+        }
+    }
+}
+
+// Three fills at once (n0, n1, n2 >= 0): each round issues a thread's loads
+// of all three before it stores any, so a workgroup staging a plane and two
+// bases waits for one memory round trip, not three. The loads are
+// unconditional (a branch per load would wait for each one): a source must
+// be readable at index 0 also when its segment is empty, which then stores
+// nothing.
+template <typename S0, typename S1, typename S2>
+__device__ __forceinline__ void lds_fill3(double* d0, int n0, S0 s0, double* d1, int n1, S1 s1, double* d2, int n2,
+                                          S2 s2) {
+    constexpr int U = 8;
+    const int nm = n0 > n1 ? (n0 > n2 ? n0 : n2) : (n1 > n2 ? n1 : n2);
+    const int c0 = n0 > 0 ? n0 - 1 : 0, c1 = n1 > 0 ? n1 - 1 : 0, c2 = n2 > 0 ? n2 - 1 : 0;
+    for (int base = threadIdx.x; base < nm; base += U * blockDim.x) {
+        double v0[U], v1[U], v2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * blockDim.x;
+            v0[u] = s0(e < c0 ? e : c0);
+            v1[u] = s1(e < c1 ? e : c1);
+            v2[u] = s2(e < c2 ? e : c2);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = base + u * blockDim.x;
+            if (e < n0) d0[e] = v0[u];
+            if (e < n1) d1[e] = v1[u];
+            if (e < n2) d2[e] = v2[u];
         }
     }
 }
