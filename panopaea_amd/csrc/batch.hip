@@ -174,7 +174,8 @@ __global__ __launch_bounds__(256) void kb_search(BTab T, char* A, const SolverSt
     flat_search(lb, n, fld<float>(A, E.o[3]), (const float*)fld<float>(A, E.o[2]), st[E.sidx].beta);
 }
 
-// red-black pair kernels over rb_[l]: o[0] = z_l, o[1] = r_l, o[2] = res, o[3] = p (level 0)
+// red-black pair kernels over rb_[l] (the operator over ap_): o[0] = z_l, o[1] = r_l, o[2] = res, o[3] = p
+// (level 0)
 __global__ __launch_bounds__(256) void kb_apply(BTab T, char* A, const SolverState* st, double* part) {
     BATCH_ENTRY(T);
     if (st[E.sidx].done) return;
@@ -182,10 +183,12 @@ __global__ __launch_bounds__(256) void kb_apply(BTab T, char* A, const SolverSta
     local_xyz(E.G, lb, m, y, zb);
     const Dims& D = E.D;
     double s = 0.0;
-    if (m < (D.w >> 1) && y < D.h) {
-        float* q = fld<float>(A, E.o[0]);
-        const float* p = fld<float>(A, E.o[3]);
-        float dt = (float)E.s;
+    float* q = fld<float>(A, E.o[0]);
+    const float* p = fld<float>(A, E.o[3]);
+    const float dt = (float)E.s;
+    if (D.d <= 1) {  // runs of kApplyRunY rows (ap_)
+        if (m < (D.w >> 1) && y * kApplyRunY < D.h) s = apply_run_y<float, kApplyRunY>(D, q, p, dt, m, y * kApplyRunY);
+    } else if (m < (D.w >> 1) && y < D.h) {
         s = with_run(D, [&](auto dc, auto rc) {
             return apply_run<float, decltype(dc)::value, decltype(rc)::value>(D, q, p, dt, m, y, zb * rc);
         });
@@ -591,6 +594,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     reset(fy_);
     reset(fz_);
     reset(flat_);
+    reset(ap_);
     for (int p = 0; p < 2; ++p) {
         reset(div_[p]);
         reset(prc_[p]);
@@ -619,7 +623,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     any3d_ = false;
     for (int l = 0; l < 16; ++l) split_first_[l] = false;
     int snat = 0, sfx = 0, sfy = 0, sfz = 0, sflat = 0, srb[16] = {0}, sco[16] = {0}, splane = 0, szcol = 0;
-    int srbs[16] = {0}, stb[16][kTbConfigs] = {};
+    int srbs[16] = {0}, stb[16][kTbConfigs] = {}, sap = 0;
     int srsd[16] = {0}, srr2[16] = {0};
     std::vector<std::pair<int, int>> dot_src;  // per grid: (config + 1 or 0 for half-sweeps, entry index) at level 0
     int srow = 0, scol = 0;
@@ -705,6 +709,14 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
                 r.o[2] = g.o_res;
                 r.o[3] = g.o_p;
                 rb_[l].h.push_back(r);
+                if (l == 0) {
+                    BEntry a = r;
+                    if (L.d <= 1) a.G = grid3t(L.w >> 1, (L.h + kApplyRunY - 1) / kApplyRunY, 1, shapes.rb_x,
+                                               shapes.rb_y);
+                    a.start = sap;
+                    sap += a.G.nb;
+                    ap_.h.push_back(a);
+                }
                 // smoothing: one pass per leg on 3D levels (the single-grid rule), else half-sweeps
                 const bool tb = tb_on(tb_enabled_, nu, L);
                 if (tb) {
@@ -841,6 +853,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     for (const BFdm& F : cols2_.h) cols2_lds_ = std::max(cols2_lds_, fdm2_cols_lds(F.ny));
     // pack every table into the staging buffer, then one copy
     size_t total = table_bytes(nat_) + table_bytes(fx_) + table_bytes(fy_) + table_bytes(fz_) + table_bytes(flat_) +
+                   table_bytes(ap_) +
                    table_bytes(plane_) + table_bytes(zcol_) + table_bytes(rows2_) + table_bytes(cols2_);
     for (int p = 0; p < 2; ++p)
         total += table_bytes(div_[p]) + table_bytes(prc_[p]);
@@ -870,6 +883,7 @@ pano_status BatchCore::build(const std::vector<HGrid>& grids, const std::vector<
     PANO_CHECK(place(fy_));
     PANO_CHECK(place(fz_));
     PANO_CHECK(place(flat_));
+    PANO_CHECK(place(ap_));
     for (int p = 0; p < 2; ++p) {
         PANO_CHECK(place(div_[p]));
         PANO_CHECK(place(prc_[p]));
@@ -1057,8 +1071,8 @@ pano_status BatchCore::enqueue_init() {
 pano_status BatchCore::enqueue_iteration() {
     if (flat_.h.empty()) return PANO_OK;
     dim3 blk(shapes.rb_x, shapes.rb_y);
-    BL(kb_apply, rb_[0], arena, (const SolverState*)states, part0_);
-    fin(FIN_ALPHA, part0_, nullptr, rb_[0], 0);
+    BL(kb_apply, ap_, arena, (const SolverState*)states, part0_);
+    fin(FIN_ALPHA, part0_, nullptr, ap_, 0);
     blk = dim3(kBlock, 1);
     BL(kb_update, flat_, arena, (const SolverState*)states, part1_);
     fin(FIN_RNORM, part1_, nullptr, flat_, 1);

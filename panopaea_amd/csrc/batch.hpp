@@ -151,6 +151,7 @@ private:
     Table prc_[2];         // natural tiles of the projection: o[0] = u[ping], o[1] = x
     Table flat_;           // flat vector blocks (2048 cells per block)
     Table rb_[16];         // red-black pair tiles per level
+    Table ap_;             // level-0 operator tiles: rb_[0]'s, with runs of kApplyRunY rows on 2D grids
     Table rbs_[16];        // the same, for the grids smoothed by half-sweeps at the level
     Table tb_[16][kTbConfigs];  // one-pass smoothing tiles (3D levels, by configuration)
     Table dot_;            // per grid: where the last post-smoothing left its z.r partials

@@ -287,6 +287,76 @@ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T d
     return s;
 }
 
+// Rows per thread of the 2D operator (apply_run_y): a thread covering one
+// row of one pair is short next to a workgroup's fixed round trips (the
+// batched engine's entry lookup, the stop flag), so 2D runs go along y.
+constexpr int kApplyRunY = 4;
+// Tiles of the red-black operator: x-pairs by rows by plane runs in 3D,
+// x-pairs by runs of kApplyRunY rows in 2D.
+inline Grid3 grid3_ap(const Dims& D) {
+    if (D.d > 1) return grid3_rb(D);
+    return grid3(D.w >> 1, (D.h + kApplyRunY - 1) / kApplyRunY, 1);
+}
+
+// The 2D form of apply_run over NY rows y0 .. y0+NY-1 of pair m, with
+// sum(p*q) in the order (row, x0 then x1). Each distinct value is loaded
+// once: both cells of the pair on rows y0-1 .. y0+NY (the run's cells and
+// their y neighbours: the y neighbours of x0 / x1 are the x0 / x1 of the rows
+// above and below) and per run row the x1 of pair m-1 and the x0 of pair m+1.
+// Each q is poisson_from of the same values as in apply_run, so the same bits.
+template <typename T, int NY>
+__device__ __forceinline__ double apply_run_y(const Dims& D, T* q, const T* p, T dt, int m, int y0) {
+    const int wh = D.w >> 1, n2 = D.h * wh;
+    const int ml = m > 0 ? m - 1 : m, mr = m + 1 < wh ? m + 1 : m;
+    auto at = [&](int yy, int mm, int c) { return ((yy & 1) ^ c) * n2 + yy * wh + mm; };
+    // every index first, then every load (as in apply_run)
+    int ic[NY + 2][2], il[NY], ir[NY];
+#pragma unroll
+    for (int kk = 0; kk < NY + 2; ++kk) {
+        // row y0 + kk - 1, clamped into the grid (a missing y neighbour's value is not used)
+        int yy = y0 + kk - 1;
+        yy = yy < 0 ? 0 : (yy >= D.h ? D.h - 1 : yy);
+        ic[kk][0] = at(yy, m, 0);
+        ic[kk][1] = at(yy, m, 1);
+    }
+#pragma unroll
+    for (int k = 0; k < NY; ++k) {
+        const int yy = y0 + k < D.h ? y0 + k : D.h - 1;
+        il[k] = at(yy, ml, 1);
+        ir[k] = at(yy, mr, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    T cv[NY + 2][2], xl[NY], xr[NY];
+#pragma unroll
+    for (int kk = 0; kk < NY + 2; ++kk) {
+        cv[kk][0] = ldx(p, ic[kk][0]);
+        cv[kk][1] = ldx(p, ic[kk][1]);
+    }
+#pragma unroll
+    for (int k = 0; k < NY; ++k) {
+        xl[k] = ldx(p, il[k]);
+        xr[k] = ldx(p, ir[k]);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NY; ++k) {
+        const int y = y0 + k;
+        if (y >= D.h) continue;
+        const RBRow R = rb_row<2>(D, 0, y, m);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const T pc = cv[k + 1][c];
+            const T a = c == 0 ? xl[k] : cv[k + 1][0];  // x-1
+            const T b = c == 0 ? cv[k + 1][1] : xr[k];  // x+1
+            const T v = poisson_from<T, 2>(pc, a, b, cv[k][c], cv[k + 2][c], (T)0, (T)0, c == 0 ? R.has0 : R.has1,
+                                           D.bc, dt);
+            stx(q, c == 0 ? R.i0 : R.i1, v);
+            s += (double)pc * (double)v;
+        }
+    }
+    return s;
+}
+
 // Red cells of the first sweep from z = 0: z = (r + s*0) * rinv[nc]; the
 // black half-sweep that follows completes the sweep.
 template <typename T, int DIM, int NZ>
@@ -511,6 +581,39 @@ __device__ __forceinline__ void lds_fill3(double* d0, int n0, S0 s0, double* d1,
     }
 }
 
+// Two fills at once with U0 / U1 loads per thread and round: a thread issues
+// all its loads of a round before it stores any, so with enough loads per
+// round (a staged basis of up to U1 * blockDim.x doubles) a workgroup waits
+// for one memory round trip. Unconditional loads as in lds_fill3.
+template <int U0, int U1, typename S0, typename S1>
+__device__ __forceinline__ void lds_fill2(double* d0, int n0, S0 s0, double* d1, int n1, S1 s1) {
+    const int B = blockDim.x;
+    const int c0 = n0 > 0 ? n0 - 1 : 0, c1 = n1 > 0 ? n1 - 1 : 0;
+    for (int r0 = threadIdx.x, r1 = threadIdx.x; r0 < n0 || r1 < n1; r0 += U0 * B, r1 += U1 * B) {
+        double v0[U0], v1[U1];
+#pragma unroll
+        for (int u = 0; u < U0; ++u) {
+            const int e = r0 + u * B;
+            v0[u] = s0(e < c0 ? e : c0);
+        }
+#pragma unroll
+        for (int u = 0; u < U1; ++u) {
+            const int e = r1 + u * B;
+            v1[u] = s1(e < c1 ? e : c1);
+        }
+#pragma unroll
+        for (int u = 0; u < U0; ++u) {
+            const int e = r0 + u * B;
+            if (e < n0) d0[e] = v0[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U1; ++u) {
+            const int e = r1 + u * B;
+            if (e < n1) d1[e] = v1[u];
+        }
+    }
+}
+
 __device__ __forceinline__ const double* stage_basis(double* dst, int used, const double* V, int n) {
     if (used + n * n > kFdmVStage) return V;
     lds_fill(dst + used, n * n, [&](int e) { return V[e]; });
@@ -678,10 +781,23 @@ __device__ __forceinline__ void fdm2_rows_body(double* lds, int nx, int ny, int 
     const int j0 = blk * R;
     const int nr = min(R, ny - j0);
     const int P = nr * nx;
-    lds_fill(lds, P, [&](int e) { return (double)src[(int64_t)j0 * nx + e]; });
-    // inverse: the staged transpose W (W[k*nx+i] = Vx[i*nx+k]) when it fits
-    const double* W = inverse ? stage_basis_t(lds + R * nx, 0, Vx, nx) : nullptr;
-    if (!W) Vx = stage_basis(lds + R * nx, 0, Vx, nx);
+    // the rows and the basis in one round of loads (inverse: the transpose W,
+    // W[k*nx+i] = Vx[i*nx+k]), the basis when it fits (kFdmVStage)
+    const bool sv = nx * nx <= kFdmVStage;
+    const Tin* rows = src + (int64_t)j0 * nx;
+    double* vs = lds + R * nx;
+    if (inverse)
+        lds_fill2<1, 16>(
+            lds, P, [&](int e) { return (double)rows[e]; }, vs, sv ? nx * nx : 0,
+            [&](int e) {
+                const int q = e / nx, i = e - q * nx;
+                return Vx[i * nx + q];
+            });
+    else
+        lds_fill2<1, 16>(
+            lds, P, [&](int e) { return (double)rows[e]; }, vs, sv ? nx * nx : 0, [&](int e) { return Vx[e]; });
+    const double* W = inverse && sv ? vs : nullptr;
+    if (!inverse && sv) Vx = vs;
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int jl = e / nx, i = e - jl * nx;
@@ -707,20 +823,17 @@ __device__ __forceinline__ void fdm2_cols_body(double* lds, int nx, int ny, int 
     const int P = ny * nc;
     double* a = lds;
     double* b = lds + ny * C;
-    for (int base = threadIdx.x; base < P; base += 8 * blockDim.x) {  // up to 8 loads in flight per thread
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = imin(base + u * blockDim.x, P - 1), j = e / nc, ci = e - j * nc;
-            v[u] = t[(int64_t)j * nx + i0 + ci];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int e = base + u * blockDim.x, j = e / nc, ci = e - j * nc;
-            if (e < P) a[j * C + ci] = v[u];
-        }
-    }
-    Vy = stage_basis(lds + 2 * ny * C, 0, Vy, ny);
+    // the columns (a[j*C+ci]; columns past the grid's last hold a copy of
+    // its value, never read) and the basis in one round of loads
+    const bool sv = ny * ny <= kFdmVStage;
+    lds_fill2<1, 16>(
+        a, ny * C,
+        [&](int e) {
+            const int j = e / C, ci = e - j * C;
+            return t[(int64_t)j * nx + i0 + imin(ci, nc - 1)];
+        },
+        lds + 2 * ny * C, sv ? ny * ny : 0, [&](int e) { return Vy[e]; });
+    if (sv) Vy = lds + 2 * ny * C;
     __syncthreads();
     for (int e = threadIdx.x; e < P; e += blockDim.x) {
         int j = e / nc, ci = e - j * nc;

@@ -67,7 +67,11 @@ __global__ __launch_bounds__(256) void k_apply_dot_rb(Dims D, Grid3 G, T* q, con
     int m, y, zb, L;
     tile_xyz(G, m, y, zb, L);
     double s = 0.0;
-    if (m < (D.w >> 1) && y < D.h) s = apply_run<T, DIM, NZ>(D, q, p, dt, m, y, zb * NZ);
+    if constexpr (DIM == 2) {  // runs of NZ rows (grid3_ap)
+        if (m < (D.w >> 1) && y * NZ < D.h) s = apply_run_y<T, NZ>(D, q, p, dt, m, y * NZ);
+    } else {
+        if (m < (D.w >> 1) && y < D.h) s = apply_run<T, DIM, NZ>(D, q, p, dt, m, y, zb * NZ);
+    }
     double t = block_sum(s);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[L] = t;
 }
@@ -192,7 +196,7 @@ static pano_status solver_setup(Ctx* c, const Dims& D, const pano_pcg_params* P,
     PANO_CHECK(solver_check(D, P));
     if (S->rb) PANO_CHECK(mg_setup<T>(c, D, P->mg_levels, P->mg_nu1, P->dt, &S->mg));
     S->nflat = (int)blocks_for(S->n, kFlatPer);
-    S->G = S->rb ? grid3_rb(D) : grid3(D.w, D.h, D.d);
+    S->G = S->rb ? grid3_ap(D) : grid3(D.w, D.h, D.d);  // the operator's tiles
     S->ntile = S->G.nb;
     const int np = solver_partials(D, S->rb);
     void *st, *p0, *p1;
@@ -301,8 +305,8 @@ static pano_status solver_iter_head(Solver<T>* S) {
     if (S->rb) {
         const int nz = run_len(D);
         if (D.d == 1)
-            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2, 1>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
-                    S->part0, cst);
+            klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 2, kApplyRunY>, Gd, B, D, S->G, S->q, (const T*)S->p,
+                    S->dt, S->part0, cst);
         else if (nz == 4)
             klaunch(c, "apply_dot", 2 * unit, k_apply_dot_rb<T, 3, 4>, Gd, B, D, S->G, S->q, (const T*)S->p, S->dt,
                     S->part0, cst);
@@ -702,7 +706,8 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     PANO_CHECK(c->workspace("mx_b64", n * sizeof(double), &b64));
     PANO_CHECK(c->workspace("mx_x64", n * sizeof(double), &x64));
     PANO_CHECK(c->workspace("mx_state", sizeof(SolverState), &st2v));
-    int np = S.nflat > S.ntile ? S.nflat : S.ntile;
+    const Grid3 Gr = grid3_rb(D);  // tiles of the FP64 residual (one row or plane run per thread)
+    int np = S.nflat > Gr.nb ? S.nflat : Gr.nb;
     PANO_CHECK(c->workspace("mx_part", (size_t)np * sizeof(double), &part2));
     S.r = (float*)r32;
     S.aux = (float*)aux;
@@ -741,14 +746,13 @@ static pano_status mixed_solve(Ctx* c, const Dims& D, const pano_mixed_params* M
     for (;;) {
         // r = b - A x in FP64 (to the FP32 buffer of the inner solve); stop test in FP64
         if (D.d > 1)
-            hipLaunchKernelGGL(k_resid64_rb<3>, dim3(S.G.nb), dim3(kTX, kTY), 0, s, D, S.G, S.r, (const double*)xx,
+            hipLaunchKernelGGL(k_resid64_rb<3>, dim3(Gr.nb), dim3(kTX, kTY), 0, s, D, Gr, S.r, (const double*)xx,
                                (const double*)bb, M->inner.dt, part);
         else
-            hipLaunchKernelGGL(k_resid64_rb<2>, dim3(S.G.nb), dim3(kTX, kTY), 0, s, D, S.G, S.r, (const double*)xx,
+            hipLaunchKernelGGL(k_resid64_rb<2>, dim3(Gr.nb), dim3(kTX, kTY), 0, s, D, Gr, S.r, (const double*)xx,
                                (const double*)bb, M->inner.dt, part);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, (int)FIN_RNORM, (const double*)part,
-                           (const double*)nullptr, S.ntile, st2,
-                           cfg2, 1);
+                           (const double*)nullptr, Gr.nb, st2, cfg2, 1);
         PANO_HIP(hipMemcpyAsync(&hs[1], st2, sizeof(SolverState), hipMemcpyDeviceToHost, s));
         PANO_HIP(hipStreamSynchronize(s));
         if (hs[1].done) break;
