@@ -334,8 +334,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                                                     fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc,
                                                     fld<float>(A, E.o[3]), E.Dc);
     if constexpr (DOT) {
-        const double a0 = block_sum(acc[0]);
-        const double a1 = block_sum(acc[1]);
+        double a0 = acc[0], a1 = acc[1];
+        block_sum2<NT>(a0, a1);
         if (threadIdx.x == 0) {
             p0[base + b_] = a0;
             p2[base + b_] = a1;
@@ -364,8 +364,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                                                    out, fld<float>(A, E.o[2]), (float)E.s, E.rinv, acc,
                                                    fld<float>(A, E.o[3]), E.Dc);
     if constexpr (DOT) {
-        const double a0 = block_sum(acc[0]);
-        const double a1 = block_sum(acc[1]);
+        double a0 = acc[0], a1 = acc[1];
+        block_sum2<NT>(a0, a1);
         if (threadIdx.x == 0) {
             p0[base + b_] = a0;
             p2[base + b_] = a1;

@@ -192,8 +192,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     double acc[2];
     tb_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, D, bx, by, bz, g.zc, xin, xout, r, s, ri.v, acc, zco, Dc);
     if constexpr (DOT) {
-        const double a0 = block_sum(acc[0]);
-        const double a1 = block_sum(acc[1]);
+        double a0 = acc[0], a1 = acc[1];
+        block_sum2<NT>(a0, a1);
         if (threadIdx.x == 0) {
             part[L] = a0;
             part[g.nb + L] = a1;
@@ -213,8 +213,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     double acc[2];
     tb2_workgroup<TXP, TY, NT, CF, ZERO_X, DOT, PL>(lds, D, bx, by, xin, xout, r, s, ri.v, acc, zco, Dc);
     if constexpr (DOT) {
-        const double a0 = block_sum(acc[0]);
-        const double a1 = block_sum(acc[1]);
+        double a0 = acc[0], a1 = acc[1];
+        block_sum2<NT>(a0, a1);
         if (threadIdx.x == 0) {
             part[L] = a0;
             part[g.nb + L] = a1;

@@ -214,6 +214,33 @@ __device__ __forceinline__ double block_max_x(double v) {
     return s;
 }
 
+// Two sums at once over a one-dimensional workgroup of NT threads: each with
+// the arithmetic of block_sum (butterfly, then the waves' sums in order from
+// thread 0), sharing one pair of barriers. Results valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+    __shared__ double red2_s[2][NT / 64];
+    const int tid = threadIdx.x;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        red2_s[0][tid >> 6] = a;
+        red2_s[1][tid >> 6] = b;
+    }
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+    if (tid == 0)
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) {
+            s0 += red2_s[0][i];
+            s1 += red2_s[1][i];
+        }
+    a = s0;
+    b = s1;
+}
+
 // NaN-propagating |x| max for norms: a NaN anywhere makes the norm NaN.
 __host__ __device__ __forceinline__ double absmax_acc(double m, double x) {
     double a = fabs(x);
