@@ -445,7 +445,7 @@ __device__ __forceinline__ T div_value(const Dims& D, const Faces<const T>& v, i
 // Neighbour values a..g = x-, x+, y-, y+, z-, z+; `has` bit i set when
 // neighbour i exists (is inside the grid).
 template <typename T, int DIM>
-__device__ __forceinline__ T poisson_from(T pc, T a, T b, T c, T e, T f, T g, int has, int bc, T dt) {
+__host__ __device__ __forceinline__ T poisson_from(T pc, T a, T b, T c, T e, T f, T g, int has, int bc, T dt) {
     const T zero = (T)0;
     T xl = (has & 1) ? a - pc : ((bc & 1) ? zero - pc : zero);
     T xr = (has & 2) ? pc - b : ((bc & 2) ? pc - zero : zero);
@@ -505,7 +505,7 @@ struct RBRow {
     int has0, has1; // neighbour existence bits of x0 and x1
 };
 template <int DIM>
-__device__ __forceinline__ RBRow rb_row(const Dims& D, int z, int y, int m) {
+__host__ __device__ __forceinline__ RBRow rb_row(const Dims& D, int z, int y, int m) {
     RBRow R;
     R.wh = D.w >> 1;
     R.hw = D.h * R.wh;

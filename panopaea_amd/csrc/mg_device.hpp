@@ -178,7 +178,7 @@ __host__ __device__ inline auto with_run(const Dims& D, F&& f) {
     if (nz == 2) return f(C3{}, R2{});
     return f(C3{}, R1{});
 }
-__device__ __forceinline__ int zclamp(const Dims& D, int z) { return z < D.d ? z : D.d - 1; }
+__host__ __device__ __forceinline__ int zclamp(const Dims& D, int z) { return z < D.d ? z : D.d - 1; }
 
 template <typename T, int DIM, int NZ>
 __device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T* r, T s, const RInv<T>& ri,
@@ -215,8 +215,13 @@ __device__ __forceinline__ double smooth_half_run(const Dims& D, T* zv, const T*
 // and y+1 (the y neighbours of x0 / x1 are the x0 / x1 of those rows, as the
 // colours alternate). Missing neighbours load a clamped, valid cell and
 // poisson_from ignores them (existence bits).
+// No instruction is scheduled across this point (the loads below it stay
+// after the index arithmetic above it); nothing on the host.
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+__host__ inline void sched_fence() {}
+
 template <typename T, int DIM, int NZ>
-__device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T dt, int m, int y, int z0) {
+__host__ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T dt, int m, int y, int z0) {
     const int wh = D.w >> 1, hw = D.h * wh, n2 = D.d * hw;
     const int ml = m > 0 ? m - 1 : m, mr = m + 1 < wh ? m + 1 : m;
     const int yl = y > 0 ? y - 1 : y, yr = y + 1 < D.h ? y + 1 : y;
@@ -246,7 +251,7 @@ __device__ __forceinline__ double apply_run(const Dims& D, T* q, const T* p, T d
             iyr[k][c] = at(zz, yr, m, c);
         }
     }
-    __builtin_amdgcn_sched_barrier(0);
+    sched_fence();
     T cv[NZ + 2][2], xl[NZ], xr[NZ], ylv[NZ][2], yrv[NZ][2];
 #pragma unroll
     for (int kk = 0; kk < NZ + 2; ++kk) {
@@ -305,7 +310,7 @@ inline Grid3 grid3_ap(const Dims& D) {
 // above and below) and per run row the x1 of pair m-1 and the x0 of pair m+1.
 // Each q is poisson_from of the same values as in apply_run, so the same bits.
 template <typename T, int NY>
-__device__ __forceinline__ double apply_run_y(const Dims& D, T* q, const T* p, T dt, int m, int y0) {
+__host__ __device__ __forceinline__ double apply_run_y(const Dims& D, T* q, const T* p, T dt, int m, int y0) {
     const int wh = D.w >> 1, n2 = D.h * wh;
     const int ml = m > 0 ? m - 1 : m, mr = m + 1 < wh ? m + 1 : m;
     auto at = [&](int yy, int mm, int c) { return ((yy & 1) ^ c) * n2 + yy * wh + mm; };
@@ -325,7 +330,7 @@ __device__ __forceinline__ double apply_run_y(const Dims& D, T* q, const T* p, T
         il[k] = at(yy, ml, 1);
         ir[k] = at(yy, mr, 0);
     }
-    __builtin_amdgcn_sched_barrier(0);
+    sched_fence();
     T cv[NY + 2][2], xl[NY], xr[NY];
 #pragma unroll
     for (int kk = 0; kk < NY + 2; ++kk) {

@@ -5,7 +5,9 @@ its memory safety:
     against the residual followed by the restriction;
   - the advection gathers (fluid_device.hpp) against the C oracle's advect /
     advect_mac, 2D and 3D, f32 and f64, one- and two-element rows,
-    displacements clamped at every wall.
+    displacements clamped at every wall;
+  - the red-black operator over runs of rows (2D) and planes (3D) against one
+    row or plane per call (mg_device.hpp).
 The device code is compiled as host code (hipcc, host-side AddressSanitizer,
 exact-size arrays) and run one workgroup or one element at a time, each phase
 over all threads before the next, on ragged shapes, every kind of boundary and
@@ -27,7 +29,7 @@ ORACLE_C = os.path.join(ROOT, "oracle", "oracle.c")
 pytestmark = pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
 
 
-@pytest.mark.parametrize("name", ["smoother_emu", "rr_emu", "adv_emu"])
+@pytest.mark.parametrize("name", ["smoother_emu", "rr_emu", "adv_emu", "op_emu"])
 def test_emulation(name, tmp_path):
     src = os.path.join(HERE, "emu", name + ".cpp")
     exe = str(tmp_path / name)
