@@ -32,10 +32,11 @@
 // step t - t0 = J is that of t0 + J, so the workgroup runs one of two copies
 // of the loop (t0 even or odd) in which every such choice is a constant.
 //
-// Every phase touches only the thread's own positions plus LDS values written
-// in an earlier phase; the kernel puts a barrier after each half-sweep:
-// [coarse prologue |] prologue | begin + half-sweep 0 | 1 | ... | NS-1 | end +
-// begin of the next step + half-sweep 0 | ...
+// Within a step a thread reads other threads' LDS values only as written in
+// earlier steps (tb_step), so the kernel puts one barrier after each step:
+// [coarse prologue |] prologue | begin + half-sweeps 0 .. NS-1 + end | begin
+// of the next step ... (with the prolongation, odd steps also have one after
+// half-sweep 0, behind the store of their coarse region).
 #pragma once
 
 #include <utility>
@@ -795,13 +796,23 @@ inline bool tb_on(bool enabled, int nu, const Dims& D) {
     return enabled && nu == 2 && tb_blocks(D) <= grid3_rb(D).nb;
 }
 
-// One step (t - t0 = J mod SLOTS, TP = t & 1): begin, half-sweeps 0..NS-1
-// with a barrier after each, end.
+// One step (t - t0 = J mod SLOTS, TP = t & 1): begin, half-sweeps 0..NS-1,
+// end, then one barrier. Within a step every LDS value a thread reads from
+// another thread's positions was written in an earlier step: half-sweep h
+// reads the in-plane neighbours of plane t-h, of the colour half-sweep h-1
+// wrote at step t-1, and its z neighbours (planes t-h-1, t-h+1) only at its
+// own positions; the fill and the store touch its own positions; a slot is
+// refilled two steps after its last in-plane read. So the step's phases need
+// no barrier between them (the CPU emulation runs each step thread by
+// thread, in both thread orders, and gets the same bits). With the
+// prolongation, an odd step's coarse region, stored by begin_step and read
+// by every thread from half-sweep 1 on, takes a barrier after half-sweep 0.
 template <int J, int TP, typename TH, int... H>
 __device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
     th.template begin_step<J, TP>(t);
-    ((th.template stage<H, J, TP>(t), __syncthreads()), ...);
+    ((th.template stage<H, J, TP>(t), (TH::PRO && TP && H == 0) ? __syncthreads() : (void)0), ...);
     th.template end_step<J>(t);
+    __syncthreads();
 }
 // Steps t .. t+SLOTS-1 (up to t1), then t advanced past them; PAR = t0 & 1.
 template <int NS, int PAR, typename TH, int... J>

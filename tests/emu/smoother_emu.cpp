@@ -1,9 +1,12 @@
 // CPU emulation of the one-pass smoother (mg_tb.hpp): the device structs
-// TbThread / Tb2Thread built as host code, one workgroup at a time, every
-// thread running each phase before any runs the next (a barrier between
-// phases), compared bit for bit with full-grid red-black half-sweeps, and the
-// z.r partial sums with double sums of the same terms. Run under
-// AddressSanitizer by tests/test_emulation.py.
+// TbThread / Tb2Thread built as host code, one workgroup at a time, compared
+// bit for bit with full-grid red-black half-sweeps, and the z.r partial sums
+// with double sums of the same terms. Every thread runs the code between two
+// of the kernel's barriers before any runs the code after the next one; in
+// 3D a step has one barrier (two on the prolongation's odd steps), and its
+// code runs thread by thread, once in increasing and once in decreasing
+// thread order, so a value one thread reads from another within a step would
+// show up as a mismatch. Run under AddressSanitizer by tests/test_emulation.py.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
@@ -91,11 +94,13 @@ static void ref_prolong(const Dims& D, const Dims& Dc, int LC, std::vector<float
             }
 }
 
-template <int NS, int J, int TP, int H = 0, typename V>
-static void stages(V& th, int t) {
+static bool g_rev = false;  // 3D steps: threads in decreasing order
+// half-sweeps H .. NS-1 of one thread
+template <int NS, int J, int TP, int H, typename T1>
+static void stages_of(T1& tt, int t) {
     if constexpr (H < NS) {
-        for (auto& tt : th) tt.template stage<H, J, TP>(t);
-        stages<NS, J, TP, H + 1>(th, t);
+        tt.template stage<H, J, TP>(t);
+        stages_of<NS, J, TP, H + 1>(tt, t);
     }
 }
 // steps t .. t+SL-1 (up to t1) of a workgroup whose first step has parity PAR
@@ -104,9 +109,25 @@ static void round_(V& th, int& t, int t1) {
     if constexpr (J < SL) {
         if (t > t1) return;
         constexpr int TP = (PAR + J) & 1;
-        for (auto& tt : th) tt.template begin_step<J, TP>(t);
-        stages<NS, J, TP>(th, t);
-        for (auto& tt : th) tt.template end_step<J>(t);
+        using TT = typename V::value_type;
+        const int nth = (int)th.size();
+        auto at = [&](int i) -> TT& { return th[g_rev ? nth - 1 - i : i]; };
+        if constexpr (TT::PRO && TP) {  // a barrier after half-sweep 0
+            for (int i = 0; i < nth; ++i) {
+                at(i).template begin_step<J, TP>(t);
+                at(i).template stage<0, J, TP>(t);
+            }
+            for (int i = 0; i < nth; ++i) {
+                stages_of<NS, J, TP, 1>(at(i), t);
+                at(i).template end_step<J>(t);
+            }
+        } else {
+            for (int i = 0; i < nth; ++i) {
+                at(i).template begin_step<J, TP>(t);
+                stages_of<NS, J, TP, 0>(at(i), t);
+                at(i).template end_step<J>(t);
+            }
+        }
         ++t;
         round_<NS, SL, PAR, J + 1>(th, t, t1);
     }
@@ -248,11 +269,10 @@ static int run2(const Dims& D, unsigned seed) {
     return bad || e0 > 1e-12 || e1 > 1e-12;
 }
 
-int main() {
+static int run3d(unsigned& seed) {
     int fails = 0;
     const Dims cases[] = {{16, 16, 16, 0}, {9, 5, 6, 32 | 16}, {20, 36, 70, 0}, {3, 40, 140, 32},
                           {32, 32, 128, 63}, {40, 17, 66, 2}};
-    unsigned seed = 1;
     for (auto D : cases) {
         for (int zmin : {8, 2, 1}) {
             // the production tiles: 32 x 32 pairs / 768 threads (levels >= 2^20 cells), 32 x 8 / 320
@@ -279,6 +299,18 @@ int main() {
             // two positions per thread and row parity
             fails += run<4, 16, 16, 1, false, true, 128, REDBLACK>(D, 64, zmin, seed++);
         }
+    }
+    return fails;
+}
+
+int main() {
+    int fails = 0;
+    unsigned seed = 1;
+    // the 3D cases run twice: steps thread by thread in increasing, then
+    // decreasing thread order
+    for (int rev = 0; rev < 2; ++rev) {
+        g_rev = rev != 0;
+        fails += run3d(seed);
     }
     const Dims c2[] = {{1, 64, 64, 0}, {1, 40, 70, 2}, {1, 100, 200, 15}, {1, 8, 6, 4}, {1, 33, 130, 9}};
     for (auto D : c2) {
