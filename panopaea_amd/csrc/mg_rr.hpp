@@ -4,18 +4,20 @@
 // column (I, J)) and a chunk of coarse planes [K0, K1). It streams through
 // the fine planes p = 2K0-1 .. 2K1:
 //  phase 1: the residual of plane p on the fine cells the tile's restriction
-//           reads (pairs I0-1 .. I0+TXC, rows 2J0-1 .. 2J0+2TYC) into LDS,
-//           from z planes p-1, p, p+1 held in LDS (three slots, with a
+//           reads (pairs I0-1 .. I0+TXC, rows 2J0-1 .. 2J0+2TYC) into one of
+//           two residual buffers in LDS (steps alternate), from z planes
+//           p-1, p, p+1 held in LDS (four slots: planes p-1 .. p+2, with a
 //           one-cell ring around the region; zero outside the grid) and b of
 //           plane p in registers;
 //  phase 2: each thread restricts plane p along x, then y, for its coarse
 //           column (the weights and order of restrict_run) into a ring of
 //           four plane values; after plane 2K+2 it combines planes
 //           2K-1 .. 2K+2 along z into r_c(K, J, I). Phase 2 also fills the
-//           slot of plane p+2 from registers loaded PD steps earlier (z and
+//           slot of plane p+3 from registers loaded PD steps earlier (z and
 //           b are loaded PD planes ahead, into PD register sets used in
 //           turn, so a step's loads have PD steps to arrive).
-// A barrier separates the phases. Each residual is residual_run's
+// A barrier separates phase 1 of a step from its phase 2; phase 2 of step p
+// and phase 1 of step p+1 need none between them (rr_steps). Each residual is residual_run's
 // expression of the same six neighbours and each coarse value restrict_run's,
 // so the output has the bits of the two separate kernels. Values the
 // restriction weights by zero (rows, columns and planes beyond the grid) are
@@ -47,7 +49,8 @@ struct RrShape {
     // stores all its positions without a branch (the pad is never read)
     static constexpr int ZS = KZ * NT;
     static constexpr int RS = KR * NT;
-    static constexpr int LDS_FLOATS = 3 * 2 * ZS + 2 * RS;
+    static constexpr int ZSLOTS = 4;  // z planes p-1 .. p+2 at step p
+    static constexpr int LDS_FLOATS = ZSLOTS * 2 * ZS + 2 * 2 * RS;
     // Position f -> (row, pair) of a region holding the TXC tile pairs at
     // pr = H .. H+TXC-1: the tile pairs of every row first, TXC consecutive
     // ones per row (with TXC = 32 a half-wave touches 32 consecutive LDS
@@ -90,8 +93,8 @@ struct RrThread {
     static constexpr int NT = S::NT, RW = S::RW, RH = S::RH, ZW = S::ZW, ZPLANE = S::ZPLANE, RPLANE = S::RPLANE,
                          KZ = S::KZ, KR = S::KR, ZS = S::ZS, RS = S::RS;
 
-    float* zl;  // LDS: 3 slots x 2 colours x ZS
-    float* rl;  // LDS: 2 colours x RS
+    float* zl;  // LDS: 4 slots x 2 colours x ZS
+    float* rl;  // LDS: 2 residual planes (steps alternate) x 2 colours x RS
     Dims D, Dc;
     const float* z;
     const float* b;
@@ -120,7 +123,7 @@ struct RrThread {
                                                   int kc, const float* z_, const float* b_, float* rc_, float s_,
                                                   int tid_) {
         zl = lds;
-        rl = lds + 3 * 2 * ZS;
+        rl = lds + S::ZSLOTS * 2 * ZS;
         D = D_;
         Dc = Dc_;
         z = z_;
@@ -176,7 +179,9 @@ struct RrThread {
         own = I < Dc.w && J < Dc.h;
         ty[0] = ty[1] = ty[2] = ty[3] = 0.0f;
     }
-    __host__ __device__ __forceinline__ int slot(int p) const { return (p - pf0 + 1) % 3; }
+    __host__ __device__ __forceinline__ int slot(int p) const { return (p - pf0 + 1) % S::ZSLOTS; }
+    // the residual plane buffer of step p
+    __host__ __device__ __forceinline__ float* rbuf(int p) const { return rl + ((p - pf0) & 1) * 2 * RS; }
     __host__ __device__ __forceinline__ const float* plane_ptr(const float* f, int p) const {
         return f + (p < 0 ? 0 : (p >= D.d ? D.d - 1 : p)) * hw;
     }
@@ -213,25 +218,25 @@ struct RrThread {
             bpf[Q][1][k] = ldx(p1, roff[k]);
         }
     }
-    // b of plane pf0+Q and z of plane pf0+2+Q into set Q, for Q < PD: the
-    // order of the loop's own loads (b of a plane, then z two planes on), so
+    // b of plane pf0+Q and z of plane pf0+3+Q into set Q, for Q < PD: the
+    // order of the loop's own loads (b of a plane, then z three planes on), so
     // that the loop's first waits are those of every later step
     template <int Q>
     __host__ __device__ __forceinline__ void load_ahead() {
         if constexpr (Q < PD) {
             load_b<Q>(pf0 + Q);
-            load_z<Q>(pf0 + 2 + Q);
+            load_z<Q>(pf0 + 3 + Q);
             load_ahead<Q + 1>();
         }
     }
-    // z planes pf0-1 .. pf0+1 into LDS (all loads issued before the first
-    // fill); z of planes pf0+2 .. and b of planes pf0 .. into the PD
+    // z planes pf0-1 .. pf0+2 into LDS (all loads issued before the first
+    // fill); z of planes pf0+3 .. and b of planes pf0 .. into the PD
     // register sets
     __host__ __device__ __forceinline__ void prologue() {
-        float v[3][2][KZ];
-        bool ok[3];
+        float v[S::ZSLOTS][2][KZ];
+        bool ok[S::ZSLOTS];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < S::ZSLOTS; ++j) {
             load_z<0>(pf0 - 1 + j);
             ok[j] = zok[0];
 #pragma unroll
@@ -241,7 +246,7 @@ struct RrThread {
             }
         }
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < S::ZSLOTS; ++j) {
             zok[0] = ok[j];
 #pragma unroll
             for (int k = 0; k < KZ; ++k) {
@@ -306,25 +311,25 @@ struct RrThread {
     template <int Q>
     __host__ __device__ __forceinline__ void phase1(int p) {
         if (p >= 0 && p < D.d)
-            residual_plane(rl, zl + slot(p) * 2 * ZS, zl + slot(p - 1) * 2 * ZS, zl + slot(p + 1) * 2 * ZS, p,
-                           bpf[Q]);
+            residual_plane(rbuf(p), zl + slot(p) * 2 * ZS, zl + slot(p - 1) * 2 * ZS, zl + slot(p + 1) * 2 * ZS,
+                           p, bpf[Q]);
         load_b<Q>(p + PD);
     }
     // plane p restricted along x and y into the ring; r_c(K) once plane
-    // 2K+2 is in; then the slot of plane p+2 from set Q and z of plane
-    // p+2+PD into set Q
+    // 2K+2 is in; then the slot of plane p+3 (that of plane p-1, last read by
+    // phase 1 of this step) from set Q and z of plane p+3+PD into set Q
     template <int Q>
     __host__ __device__ __forceinline__ void phase2(int p) {
         ty[0] = ty[1];
         ty[1] = ty[2];
         ty[2] = ty[3];
-        ty[3] = p >= 0 && p < D.d ? restrict_xy(rl, p) : 0.0f;
+        ty[3] = p >= 0 && p < D.d ? restrict_xy(rbuf(p), p) : 0.0f;
         // r_c(K) for p = 2K+2 (an always-issued store, dropped on other steps)
         const bool st = p >= 2 * K0 + 2 && !(p & 1);
         const int K = st ? p / 2 - 1 : K0;
         st_if(rco, own ? cidx<LC>(Dc, K, J, I) : 0, r1(ty[0], ty[1], ty[2], ty[3], K > 0, K < Dc.d - 1), st && own);
-        fill_z<Q>(p + 2);
-        load_z<Q>(p + 2 + PD);
+        fill_z<Q>(p + 3);
+        load_z<Q>(p + 3 + PD);
     }
 };
 
@@ -407,9 +412,17 @@ __device__ __forceinline__ void rr2_workgroup(float* lds, const Dims& D, const D
     th.restrict_store();
 }
 
-template <typename TH, int... Q>
+// Steps p .. p+PD-1 (up to p1): a barrier, phase 2 of step p (reads the
+// residual phase 1 of step p wrote), then phase 1 of step p+1. Phase 1 of step
+// p+1 reads z planes p .. p+2, filled by phase 2 of step p-1 or earlier (a
+// barrier behind), and writes the other residual buffer, last read by phase
+// 2 of step p-1; phase 2 of step p fills the slot of plane p-1, last read by
+// phase 1 of step p. So one barrier per step.
+template <int PD, typename TH, int... Q>
 __device__ __forceinline__ void rr_steps(TH& th, int& p, int p1, std::integer_sequence<int, Q...>) {
-    ((p <= p1 ? (th.template phase1<Q>(p), __syncthreads(), th.template phase2<Q>(p), __syncthreads(), ++p) : p),
+    ((p <= p1 ? (__syncthreads(), th.template phase2<Q>(p),
+                 (p + 1 <= p1 ? th.template phase1<(Q + 1) % PD>(p + 1) : (void)0), ++p)
+              : p),
      ...);
 }
 
@@ -422,7 +435,8 @@ __device__ __forceinline__ void rr_workgroup(float* lds, const Dims& D, const Di
     th.prologue();
     __syncthreads();
     int p = th.pf0;
-    while (p <= th.pf1) rr_steps(th, p, th.pf1, std::make_integer_sequence<int, PD>{});
+    th.template phase1<0>(p);
+    while (p <= th.pf1) rr_steps<PD>(th, p, th.pf1, std::make_integer_sequence<int, PD>{});
 }
 
 }  // namespace pano

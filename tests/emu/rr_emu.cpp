@@ -1,9 +1,12 @@
 // CPU emulation of the one-pass residual + restriction (mg_rr.hpp): the
 // device structs RrThread (3D, streaming) and Rr2Thread (2D) built as host
-// code, one workgroup at a time, phase by phase, compared bit for bit with the
-// fine residual followed by the restriction. LDS starts as NaN, so a value the
-// restriction weighs and the kernel never wrote shows up. Run under
-// AddressSanitizer (exact-size arrays) by tests/test_emulation.py.
+// code, one workgroup at a time, compared bit for bit with the fine residual
+// followed by the restriction. Between two of the kernel's barriers the
+// threads run one after another, in increasing and then in decreasing order
+// (3D), so a value one thread reads from another within such a stretch shows
+// up. LDS starts as NaN, so a value the restriction weighs and the kernel
+// never wrote shows up too. Run under AddressSanitizer (exact-size arrays) by
+// tests/test_emulation.py.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
@@ -60,6 +63,7 @@ static void reference(const Dims& D, const Dims& Dc, const std::vector<float>& z
             }
 }
 
+static bool rev = false;  // threads of a barrier-free stretch in decreasing order
 template <int TXC, int TYC, int LC, int PD>
 static bool run_case(const Dims& D, int want, int kmin, unsigned seed) {
     Dims Dc = D;
@@ -93,14 +97,24 @@ static bool run_case(const Dims& D, int want, int kmin, unsigned seed) {
         const int bx = L % g.gx, q = L / g.gx, by = q % g.gy, bz = q / g.gy;
         for (int i = 0; i < RrShape<TXC, TYC>::LDS_FLOATS; ++i) lds[i] = NAN;
         for (int t = 0; t < NT; ++t) th[t].init(lds, D, Dc, bx, by, bz, g.kc, zh, bh, rch, s, t);
+        // the kernel's barriers: after the prologue, after phase 1 of the first
+        // step, then once per step; between two barriers the threads run one
+        // after another (phase 2 of step p, then phase 1 of step p+1), in
+        // increasing or decreasing order, so a read of another thread's value
+        // written within the same stretch shows up as a mismatch
+        auto at = [&](int i) -> TH& { return th[rev ? NT - 1 - i : i]; };
         for (int t = 0; t < NT; ++t) th[t].prologue();
-        for (int p = th[0].pf0; p <= th[0].pf1; ++p) {
-            if (PD == 1 || ((p - th[0].pf0) & 1) == 0) {
-                for (int t = 0; t < NT; ++t) th[t].template phase1<0>(p);
-                for (int t = 0; t < NT; ++t) th[t].template phase2<0>(p);
-            } else {
-                for (int t = 0; t < NT; ++t) th[t].template phase1<PD - 1>(p);
-                for (int t = 0; t < NT; ++t) th[t].template phase2<PD - 1>(p);
+        const int p0 = th[0].pf0, p1 = th[0].pf1;
+        for (int t = 0; t < NT; ++t) th[t].template phase1<0>(p0);
+        for (int p = p0; p <= p1; ++p) {
+            const bool q0 = PD == 1 || ((p - p0) & 1) == 0;  // register set of step p
+            for (int i = 0; i < NT; ++i) {
+                if (q0) at(i).template phase2<0>(p);
+                else at(i).template phase2<PD - 1>(p);
+                if (p + 1 <= p1) {
+                    if (q0 && PD > 1) at(i).template phase1<PD - 1>(p + 1);
+                    else at(i).template phase1<0>(p + 1);
+                }
             }
         }
     }
@@ -213,22 +227,25 @@ int main() {
                            {2, 2, 2}, {6, 4, 2}};
     const int bcs[] = {0, 0x21, 0x3F};
     unsigned seed = 1;
-    for (auto& d : dims)
-        for (int bc : bcs)
-            for (int want : {64, 1024}) {
-                Dims D{};
-                D.w = d[0];
-                D.h = d[1];
-                D.d = d[2];
-                D.bc = bc;
-                for (int kmin : {1, 2}) {
-                    // production: 32 x 8 coarse columns, loads one plane ahead; then a generic tile
-                    fails += !run_case<32, 8, REDBLACK, 1>(D, want, kmin, seed++);
-                    fails += !run_case<32, 8, NATURAL, 1>(D, want, kmin, seed++);
-                    fails += !run_case<16, 16, REDBLACK, 2>(D, want, kmin, seed++);
-                    cases += 3;
+    for (int r = 0; r < 2; ++r) {
+        rev = r != 0;
+        for (auto& d : dims)
+            for (int bc : bcs)
+                for (int want : {64, 1024}) {
+                    Dims D{};
+                    D.w = d[0];
+                    D.h = d[1];
+                    D.d = d[2];
+                    D.bc = bc;
+                    for (int kmin : {1, 2}) {
+                        // production: 32 x 8 coarse columns, loads one plane ahead; then a generic tile
+                        fails += !run_case<32, 8, REDBLACK, 1>(D, want, kmin, seed++);
+                        fails += !run_case<32, 8, NATURAL, 1>(D, want, kmin, seed++);
+                        fails += !run_case<16, 16, REDBLACK, 2>(D, want, kmin, seed++);
+                        cases += 3;
+                    }
                 }
-            }
+    }
     const int d2[][2] = {{4, 4}, {16, 16}, {128, 96}, {56, 40}, {66, 18}, {130, 36}, {2, 2}, {36, 100}};
     for (auto& d : d2)
         for (int bc : {0, 5, 15}) {
