@@ -35,8 +35,7 @@
 // Within a step a thread reads other threads' LDS values only as written in
 // earlier steps (tb_step), so the kernel puts one barrier after each step:
 // [coarse prologue |] prologue | begin + half-sweeps 0 .. NS-1 + end | begin
-// of the next step ... (with the prolongation, odd steps also have one after
-// half-sweep 0, behind the store of their coarse region).
+// of the next step ...
 #pragma once
 
 #include <utility>
@@ -402,8 +401,8 @@ struct TbThread {
     // PRO, before prologue (a barrier between): the regions of coarse planes
     // A .. A+2 (A = floor((t0 - 2) / 2)) into the last x slot, which no step
     // fills before step t0 + SLOTS - 3 (reads of it from the prologue end
-    // before that step's barriers); A+3 into cpf (stored by the first odd
-    // step); x of plane t0-1 into xpf.
+    // before that step's barriers); A+3 into cpf (stored into the coarse
+    // slot for the first odd step); x of plane t0-1 into xpf.
     __host__ __device__ __forceinline__ void prologue_coarse() {
         const int A = (t0 - 2) >> 1;
         float c0[CKN], c1[CKN], c2[CKN];
@@ -460,29 +459,37 @@ struct TbThread {
         }
         load_x(t0 + 1);
         bok = load_b(t0 + 1, bpf);
+        if constexpr (PRO && PAR) {
+            // the first step is odd: its coarse region (A+3) goes in now
+            store_c(lds + S::LDS_FLOATS, cpf);
+            load_c((t0 + 5) >> 1, cpf);
+        }
     }
     // Step t (t - t0 = J mod SLOTS, TP = t & 1): plane t+1 (prefetched) into
     // its slot, which only this thread's own reads of step t touch; prefetch
     // plane t+2. PRO: for odd t plane t+1 has (K0, K1) = ((t+1)/2, (t-1)/2),
-    // the two of the step before swapped; the step stores coarse plane (t+3)/2
-    // into the coarse slot (its last reads, at step t-2, are barriers behind),
-    // prefetches plane (t+5)/2 and, after its first barrier (stage 1),
-    // computes Q((t+3)/2) into q1, which the fill at step t dropped: the next
-    // (even) step fills plane t+2 with ((t+1)/2, (t+3)/2).
+    // the two of the step before swapped; coarse plane (t+3)/2 is in the
+    // coarse slot (stored by step t-1), and in stage 1 the step computes
+    // Q((t+3)/2) into q1, which the fill at step t dropped: the next (even)
+    // step fills plane t+2 with ((t+1)/2, (t+3)/2). Even steps store the
+    // prefetched region of the next odd step and prefetch the one after.
     template <int J, int TP>
     __host__ __device__ __forceinline__ void begin_step(int t) {
-        if constexpr (PRO && TP) {
+        if constexpr (PRO && !TP) {
+            // the region of coarse plane (t+4)/2 for the next (odd) step: the
+            // slot's last reads (step t-1) are a barrier behind, its next
+            // (step t+1) a barrier ahead
             store_c(lds + S::LDS_FLOATS, cpf);
-            load_c((t + 5) >> 1, cpf);
-            q_swap();
+            load_c((t + 6) >> 1, cpf);
         }
+        if constexpr (PRO && TP) q_swap();
         fill<mod(J + 2), TP ^ 1>(t + 1);
         load_x(t + 2);
     }
     // half-sweep H on plane t-H (t - t0 = J mod SLOTS, TP = t & 1)
     template <int H, int J, int TP>
     __host__ __device__ __forceinline__ void stage(int t) {
-        if constexpr (PRO && H == 1 && TP) q_of(lds + S::LDS_FLOATS, q1);  // after the barrier behind the store
+        if constexpr (PRO && H == 1 && TP) q_of(lds + S::LDS_FLOATS, q1);  // the region step t-1 stored
         const int z = t - H;
         const int lo = zs - (NS - 1 - H) > 0 ? zs - (NS - 1 - H) : 0;
         const int hi = ze + (NS - 1 - H) < D.d ? ze + (NS - 1 - H) : D.d;
@@ -805,12 +812,12 @@ inline bool tb_on(bool enabled, int nu, const Dims& D) {
 // refilled two steps after its last in-plane read. So the step's phases need
 // no barrier between them (the CPU emulation runs each step thread by
 // thread, in both thread orders, and gets the same bits). With the
-// prolongation, an odd step's coarse region, stored by begin_step and read
-// by every thread from half-sweep 1 on, takes a barrier after half-sweep 0.
+// prolongation, the coarse region that every thread of an odd step reads is
+// stored by the even step before it (or the prologue), a barrier ahead.
 template <int J, int TP, typename TH, int... H>
 __device__ __forceinline__ void tb_step(TH& th, int t, std::integer_sequence<int, H...>) {
     th.template begin_step<J, TP>(t);
-    ((th.template stage<H, J, TP>(t), (TH::PRO && TP && H == 0) ? __syncthreads() : (void)0), ...);
+    (th.template stage<H, J, TP>(t), ...);
     th.template end_step<J>(t);
     __syncthreads();
 }

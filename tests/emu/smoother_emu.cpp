@@ -3,10 +3,10 @@
 // bit for bit with full-grid red-black half-sweeps, and the z.r partial sums
 // with double sums of the same terms. Every thread runs the code between two
 // of the kernel's barriers before any runs the code after the next one; in
-// 3D a step has one barrier (two on the prolongation's odd steps), and its
-// code runs thread by thread, once in increasing and once in decreasing
-// thread order, so a value one thread reads from another within a step would
-// show up as a mismatch. Run under AddressSanitizer by tests/test_emulation.py.
+// 3D a step has one barrier, and its code runs thread by thread, once in
+// increasing and once in decreasing thread order, so a value one thread reads
+// from another within a step would show up as a mismatch. Run under
+// AddressSanitizer by tests/test_emulation.py.
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
@@ -112,21 +112,10 @@ static void round_(V& th, int& t, int t1) {
         using TT = typename V::value_type;
         const int nth = (int)th.size();
         auto at = [&](int i) -> TT& { return th[g_rev ? nth - 1 - i : i]; };
-        if constexpr (TT::PRO && TP) {  // a barrier after half-sweep 0
-            for (int i = 0; i < nth; ++i) {
-                at(i).template begin_step<J, TP>(t);
-                at(i).template stage<0, J, TP>(t);
-            }
-            for (int i = 0; i < nth; ++i) {
-                stages_of<NS, J, TP, 1>(at(i), t);
-                at(i).template end_step<J>(t);
-            }
-        } else {
-            for (int i = 0; i < nth; ++i) {
-                at(i).template begin_step<J, TP>(t);
-                stages_of<NS, J, TP, 0>(at(i), t);
-                at(i).template end_step<J>(t);
-            }
+        for (int i = 0; i < nth; ++i) {
+            at(i).template begin_step<J, TP>(t);
+            stages_of<NS, J, TP, 0>(at(i), t);
+            at(i).template end_step<J>(t);
         }
         ++t;
         round_<NS, SL, PAR, J + 1>(th, t, t1);
