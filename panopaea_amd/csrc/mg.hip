@@ -642,12 +642,15 @@ __global__ __launch_bounds__(kFdmSplitThreads) void k_fdm_z_cols(int nx, int ny,
     double* sl = fdm_lds;    // sl[k * cw + il]
     double* u = sl + nz * cw;
     double* vs = u + nz * cw;
-    lds_fill(sl, nz * cw, [&](int e) {
-        const int k = e / cw, il = e - k * cw;
-        return data[k * zs + (int64_t)j * nx + i0 + imin(il, nc - 1)];
-    });
+    // the column block and the basis (when it fits) in one round of loads
     const bool sz = nz * nz <= fdm_split_cap(nz, 0);
-    if (sz) lds_fill(vs, nz * nz, [&](int e) { return Vz[e]; });
+    lds_fill2<1, 8>(
+        sl, nz * cw,
+        [&](int e) {
+            const int k = e / cw, il = e - k * cw;
+            return data[k * zs + (int64_t)j * nx + i0 + imin(il, nc - 1)];
+        },
+        vs, sz ? nz * nz : 0, [&](int e) { return Vz[e]; });
     const double* vz = sz ? vs : Vz;
     __syncthreads();
     for (int e = threadIdx.x; e < nz * nc; e += blockDim.x) {
